@@ -1,0 +1,434 @@
+/*
+ * sx_oracle.c -- FP64 CPU restatement of Sassena's scattering-device hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY: the parity checker for the HIP library and the
+ * timed host path of the benchmark's CPU baseline.  The product library never
+ * links or calls it.
+ *
+ * Parity pins (tests/test_oracle.py): sxo_fft against a naive DFT and numpy's
+ * pocketfft; sxo_autocorrelate_fft against sxo_autocorrelate_direct (which
+ * returns the complex conjugate, exactly as the reference's two methods do);
+ * the compute drivers against closed forms (static atoms, single atoms,
+ * Gaussian random walk, Debye limit) and against golden vectors written by an
+ * independent numpy implementation (tests/golden/make_golden.py).
+ *
+ * Compiled with -ffp-contract=off so that q.r and the accumulations round as
+ * the reference's x86-64 baseline build does.
+ */
+#include "sx_oracle.h"
+
+#include <math.h>
+#include <pthread.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define SXO_PI 3.14159265358979323846
+
+/* ---------------------------------------------------------------- DFT --- */
+
+static size_t smallest_factor(size_t n) {
+    if (n % 4 == 0) return 4;
+    if (n % 2 == 0) return 2;
+    for (size_t p = 3; p * p <= n; p += 2)
+        if (n % p == 0) return p;
+    return n;
+}
+
+static size_t largest_factor(size_t n) {
+    size_t best = 1;
+    while (n > 1) {
+        size_t p = smallest_factor(n);
+        if (p > best) best = p;
+        n /= p;
+    }
+    return best;
+}
+
+typedef struct {
+    const double *tw; /* exp(sign 2 pi i e / N), e < N, interleaved */
+    size_t N;
+    double *tmp;      /* 4 * max radix doubles */
+} fft_plan_t;
+
+/* out[k] = sum_j in[j*istride] * W_n^{jk}; out is contiguous, n complex. */
+static void fft_rec(const fft_plan_t *pl, const double *in, size_t istride,
+                    double *out, size_t n) {
+    if (n == 1) {
+        out[0] = in[0];
+        out[1] = in[1];
+        return;
+    }
+    size_t p = smallest_factor(n);
+    size_t m = n / p;
+    for (size_t r = 0; r < p; ++r)
+        fft_rec(pl, in + 2 * r * istride, istride * p, out + 2 * r * m, m);
+    size_t step = pl->N / n; /* W_n^e = W_N^{e*step} */
+    double *y = pl->tmp, *x = pl->tmp + 2 * p;
+    for (size_t k = 0; k < m; ++k) {
+        for (size_t r = 0; r < p; ++r) {
+            y[2 * r] = out[2 * (r * m + k)];
+            y[2 * r + 1] = out[2 * (r * m + k) + 1];
+        }
+        for (size_t s = 0; s < p; ++s) {
+            double ar = 0.0, ai = 0.0;
+            size_t kk = k + s * m;
+            for (size_t r = 0; r < p; ++r) {
+                size_t e = ((r * kk) % n) * step;
+                double wr = pl->tw[2 * e], wi = pl->tw[2 * e + 1];
+                ar += wr * y[2 * r] - wi * y[2 * r + 1];
+                ai += wr * y[2 * r + 1] + wi * y[2 * r];
+            }
+            x[2 * s] = ar;
+            x[2 * s + 1] = ai;
+        }
+        for (size_t s = 0; s < p; ++s) {
+            out[2 * (k + s * m)] = x[2 * s];
+            out[2 * (k + s * m) + 1] = x[2 * s + 1];
+        }
+    }
+}
+
+int sxo_fft(double *data, size_t n, int sign) {
+    if (n == 0) return 0;
+    double *tw = (double *)malloc(sizeof(double) * 2 * n);
+    double *out = (double *)malloc(sizeof(double) * 2 * n);
+    size_t pmax = largest_factor(n);
+    double *tmp = (double *)malloc(sizeof(double) * 4 * (pmax + 1));
+    if (!tw || !out || !tmp) {
+        free(tw); free(out); free(tmp);
+        return -1;
+    }
+    for (size_t e = 0; e < n; ++e) {
+        double ang = 2.0 * SXO_PI * (double)e / (double)n;
+        tw[2 * e] = cos(ang);
+        tw[2 * e + 1] = (sign < 0 ? -1.0 : 1.0) * sin(ang);
+    }
+    fft_plan_t pl = {tw, n, tmp};
+    fft_rec(&pl, data, 1, out, n);
+    memcpy(data, out, sizeof(double) * 2 * n);
+    free(tw); free(out); free(tmp);
+    return 0;
+}
+
+/* ------------------------------------------------------------- smath --- */
+
+/* smath.cpp:141-156: forward FFT of length 2NF, |X|^2 with the imaginary
+   part zeroed, backward FFT, scale 1/(2NF (NF - i)) for i < NF. */
+int sxo_autocorrelate_fft(double *data, size_t NF) {
+    size_t L = 2 * NF;
+    if (sxo_fft(data, L, -1)) return -1;
+    for (size_t i = 0; i < L; ++i) {
+        data[2 * i] = data[2 * i] * data[2 * i] + data[2 * i + 1] * data[2 * i + 1];
+        data[2 * i + 1] = 0.0;
+    }
+    if (sxo_fft(data, L, +1)) return -1;
+    for (size_t i = 0; i < NF; ++i) {
+        double factor = (1.0 / (double)(2 * NF * (NF - i)));
+        data[2 * i] *= factor;
+        data[2 * i + 1] *= factor;
+    }
+    return 0;
+}
+
+/* smath.cpp:51-76: note the imaginary part is -a1r*a2i + a1i*a2r. */
+int sxo_autocorrelate_direct(double *data, size_t NF) {
+    double *loc = (double *)malloc(sizeof(double) * 2 * (NF ? NF : 1));
+    if (!loc) return -1;
+    memcpy(loc, data, sizeof(double) * 2 * NF);
+    for (size_t tau = 0; tau < NF; ++tau) {
+        double re = 0.0, im = 0.0;
+        size_t last = NF - tau;
+        for (size_t k = 0; k < last; ++k) {
+            const double *a1 = loc + 2 * k, *a2 = loc + 2 * (k + tau);
+            re += a1[0] * a2[0] + a1[1] * a2[1];
+            im += -a1[0] * a2[1] + a1[1] * a2[0];
+        }
+        data[2 * tau] = re / (double)last;
+        data[2 * tau + 1] = im / (double)last;
+    }
+    free(loc);
+    return 0;
+}
+
+/* smath.cpp:168-177 */
+void sxo_square_elements(double *data, size_t NF) {
+    for (size_t n = 0; n < NF; ++n) {
+        double r = data[2 * n] * data[2 * n] + data[2 * n + 1] * data[2 * n + 1];
+        data[2 * n] = r;
+        data[2 * n + 1] = 0.0;
+    }
+}
+
+static int dsp(double *at, size_t NF, int dsp_type, int dsp_method) {
+    /* all_vectors_scatter_device.cpp:209-229, self_vectors_scatter_device.cpp:115-135 */
+    if (dsp_type == SXO_DSP_AUTOCORRELATE) {
+        if (dsp_method == SXO_METHOD_DIRECT) return sxo_autocorrelate_direct(at, NF);
+        if (dsp_method == SXO_METHOD_FFT) return sxo_autocorrelate_fft(at, NF);
+        return -2;
+    }
+    if (dsp_type == SXO_DSP_SQUARE) {
+        sxo_square_elements(at, NF);
+        return 0;
+    }
+    if (dsp_type == SXO_DSP_PLAIN) return 0;
+    return -2;
+}
+
+/* all_vectors_scatter_device.cpp:231-236 with smath::reduce (smath.cpp:239-247)
+   and smath::add_elements (smath.cpp:209-215). */
+void sxo_store_one(const double *at, size_t NF, double *a_out) {
+    double sr = 0.0, si = 0.0;
+    for (size_t i = 0; i < NF; ++i) {
+        sr += at[2 * i];
+        si += at[2 * i + 1];
+    }
+    a_out[0] = sr * (1.0 / (double)NF);
+    a_out[1] = si * (1.0 / (double)NF);
+}
+
+/* -------------------------------------------------------- amplitudes --- */
+
+void sxo_scatter_all(const float *coords, size_t NA, size_t NF,
+                     const double *b, const double q[3], double *at) {
+    double qx = q[0], qy = q[1], qz = q[2];
+    for (size_t fi = 0; fi < NF; ++fi) {
+        const float *p_data = coords + fi * NA * 3;
+        double Ar = 0.0, Ai = 0.0;
+        for (size_t j = 0; j < NA; ++j) {
+            double esf = b[j];
+            float x = p_data[3 * j], y = p_data[3 * j + 1], z = p_data[3 * j + 2];
+            double p = x * qx + y * qy + z * qz;
+            Ar += esf * cos(p);
+            Ai += esf * sin(p);
+        }
+        at[2 * fi] = Ar;
+        at[2 * fi + 1] = Ai;
+    }
+}
+
+void sxo_scatter_self(const float *coords, size_t NF, size_t ai, double s,
+                      const double q[3], double *at) {
+    double qx = q[0], qy = q[1], qz = q[2];
+    const float *p_data = coords + ai * NF * 3;
+    for (size_t j = 0; j < NF; ++j) {
+        float x1 = p_data[j * 3], y1 = p_data[j * 3 + 1], z1 = p_data[j * 3 + 2];
+        double p1 = x1 * qx + y1 * qy + z1 * qz;
+        double sp1 = sin(p1), cp1 = cos(p1);
+        at[2 * j] = s * cp1;
+        at[2 * j + 1] = s * sp1;
+    }
+    memset(at + 2 * NF, 0, sizeof(double) * 2 * NF);
+}
+
+/* ----------------------------------------------------------- drivers --- */
+
+typedef struct {
+    int kind;
+    const float *coords;
+    size_t NA, NF, NM;
+    const double *b, *qv;
+    int dsp_type, dsp_method;
+    size_t first, count;     /* vectors (all) or atoms (self) of this worker */
+    double *C;               /* all: per-vector C_m [NM][NF][2] */
+    double *a;               /* all: per-vector a_m [NM][2] */
+    double *at_sum;          /* self: thread-local sum of C [NF][2] */
+    double a_sum[2], a2_sum[2];
+    int err;
+} work_t;
+
+static void *worker(void *arg) {
+    work_t *w = (work_t *)arg;
+    size_t NF = w->NF;
+    double *buf = (double *)calloc(4 * NF + 2, sizeof(double));
+    if (!buf) { w->err = -1; return NULL; }
+    if (w->kind == SXO_ALL) {
+        for (size_t m = w->first; m < w->first + w->count; ++m) {
+            /* scatter (:388-438) then alignpad (:186-207): NF values, NF zeros */
+            memset(buf, 0, sizeof(double) * 4 * NF);
+            sxo_scatter_all(w->coords, w->NA, NF, w->b, w->qv + 3 * m, buf);
+            int e = dsp(buf, NF, w->dsp_type, w->dsp_method);
+            if (e) { w->err = e; break; }
+            memcpy(w->C + 2 * NF * m, buf, sizeof(double) * 2 * NF);
+            sxo_store_one(buf, NF, w->a + 2 * m);
+        }
+    } else {
+        /* self_vectors_scatter_device.cpp:167-193: atoms outer, vectors inner */
+        for (size_t n = w->first; n < w->first + w->count; ++n) {
+            for (size_t m = 0; m < w->NM; ++m) {
+                sxo_scatter_self(w->coords, NF, n, w->b[n], w->qv + 3 * m, buf);
+                int e = dsp(buf, NF, w->dsp_type, w->dsp_method);
+                if (e) { w->err = e; goto done; }
+                double a[2];
+                sxo_store_one(buf, NF, a);
+                w->a_sum[0] += a[0];
+                w->a_sum[1] += a[1];
+                /* a*conj(a): imaginary part is exactly zero */
+                w->a2_sum[0] += a[0] * a[0] + a[1] * a[1];
+                for (size_t i = 0; i < NF; ++i) {
+                    w->at_sum[2 * i] += buf[2 * i];
+                    w->at_sum[2 * i + 1] += buf[2 * i + 1];
+                }
+            }
+        }
+    }
+done:
+    free(buf);
+    return NULL;
+}
+
+static int run_workers(work_t *proto, size_t nitems, int nthreads) {
+    if (nthreads < 1) nthreads = 1;
+    if ((size_t)nthreads > nitems) nthreads = (int)(nitems ? nitems : 1);
+    work_t *ws = (work_t *)calloc((size_t)nthreads, sizeof(work_t));
+    pthread_t *th = (pthread_t *)calloc((size_t)nthreads, sizeof(pthread_t));
+    if (!ws || !th) { free(ws); free(th); return -1; }
+    size_t base = nitems / (size_t)nthreads, rem = nitems % (size_t)nthreads, off = 0;
+    int err = 0;
+    for (int t = 0; t < nthreads; ++t) {
+        ws[t] = *proto;
+        ws[t].first = off;
+        ws[t].count = base + ((size_t)t < rem ? 1 : 0);
+        off += ws[t].count;
+        if (proto->kind == SXO_SELF) {
+            ws[t].at_sum = (double *)calloc(2 * proto->NF + 2, sizeof(double));
+            if (!ws[t].at_sum) err = -1;
+        }
+    }
+    if (!err) {
+        for (int t = 0; t < nthreads; ++t)
+            if (pthread_create(&th[t], NULL, worker, &ws[t])) {
+                /* fall back to running in the caller */
+                worker(&ws[t]);
+                th[t] = 0;
+            }
+        for (int t = 0; t < nthreads; ++t)
+            if (th[t]) pthread_join(th[t], NULL);
+    }
+    if (!err && proto->kind == SXO_SELF) {
+        /* reduce thread partials in thread order */
+        for (int t = 0; t < nthreads; ++t) {
+            for (size_t i = 0; i < 2 * proto->NF; ++i) proto->at_sum[i] += ws[t].at_sum[i];
+            proto->a_sum[0] += ws[t].a_sum[0];
+            proto->a_sum[1] += ws[t].a_sum[1];
+            proto->a2_sum[0] += ws[t].a2_sum[0];
+            proto->a2_sum[1] += ws[t].a2_sum[1];
+        }
+    }
+    for (int t = 0; t < nthreads; ++t) {
+        if (ws[t].err) err = ws[t].err;
+        free(ws[t].at_sum);
+    }
+    free(ws);
+    free(th);
+    return err;
+}
+
+int sxo_compute_all(const float *coords_frames, size_t NA, size_t NF,
+                    const double *b, const double *qv, size_t NM,
+                    int dsp_type, int dsp_method, int nthreads,
+                    double *fqt, double *fq, double *fq2) {
+    if (NF == 0 || NM == 0) return -3;
+    work_t w;
+    memset(&w, 0, sizeof w);
+    w.kind = SXO_ALL;
+    w.coords = coords_frames;
+    w.NA = NA; w.NF = NF; w.NM = NM;
+    w.b = b; w.qv = qv;
+    w.dsp_type = dsp_type; w.dsp_method = dsp_method;
+    w.C = (double *)malloc(sizeof(double) * 2 * NF * NM);
+    w.a = (double *)malloc(sizeof(double) * 2 * NM);
+    if (!w.C || !w.a) { free(w.C); free(w.a); return -1; }
+    int err = run_workers(&w, NM, nthreads);
+    if (!err) {
+        /* store() in vector order (:231-236), then the 1/NM scale (:355-360) */
+        double afr = 0, afi = 0, a2r = 0, a2i = 0;
+        memset(fqt, 0, sizeof(double) * 2 * NF);
+        for (size_t m = 0; m < NM; ++m) {
+            double ar = w.a[2 * m], ai = w.a[2 * m + 1];
+            afr += ar; afi += ai;
+            a2r += ar * ar + ai * ai;
+            const double *C = w.C + 2 * NF * m;
+            for (size_t i = 0; i < 2 * NF; ++i) fqt[i] += C[i];
+        }
+        double factor = 1.0 / (double)NM;
+        for (size_t i = 0; i < 2 * NF; ++i) fqt[i] *= factor;
+        fq[0] = afr * factor; fq[1] = afi * factor;
+        fq2[0] = a2r * factor; fq2[1] = a2i * factor;
+    }
+    free(w.C);
+    free(w.a);
+    return err;
+}
+
+int sxo_compute_self(const float *coords_atoms, size_t NA, size_t NF,
+                     const double *b, const double *qv, size_t NM,
+                     int dsp_type, int dsp_method, int nthreads,
+                     double *fqt, double *fq, double *fq2) {
+    if (NF == 0 || NM == 0) return -3;
+    work_t w;
+    memset(&w, 0, sizeof w);
+    w.kind = SXO_SELF;
+    w.coords = coords_atoms;
+    w.NA = NA; w.NF = NF; w.NM = NM;
+    w.b = b; w.qv = qv;
+    w.dsp_type = dsp_type; w.dsp_method = dsp_method;
+    w.at_sum = (double *)calloc(2 * NF + 2, sizeof(double));
+    if (!w.at_sum) return -1;
+    int err = run_workers(&w, NA, nthreads);
+    if (!err) {
+        /* self_vectors_scatter_device.cpp:233-238: scale by 1/NM only */
+        double factor = 1.0 / (double)NM;
+        for (size_t i = 0; i < 2 * NF; ++i) fqt[i] = w.at_sum[i] * factor;
+        fq[0] = w.a_sum[0] * factor; fq[1] = w.a_sum[1] * factor;
+        fq2[0] = w.a2_sum[0] * factor; fq2[1] = w.a2_sum[1] * factor;
+    }
+    free(w.at_sum);
+    return err;
+}
+
+/* ------------------------------------------------------- q vectors --- */
+
+size_t sxo_init_subvectors(const double q[3], const double *u, size_t nu,
+                           double *out) {
+    if (nu == 0) {
+        out[0] = q[0]; out[1] = q[1]; out[2] = q[2];
+        return 1;
+    }
+    /* CartesianCoor3D::length (coor3d.cpp:58-60) */
+    double ql = sqrt(pow(q[0], 2) + pow(q[1], 2) + pow(q[2], 2));
+    for (size_t i = 0; i < nu; ++i) {
+        out[3 * i] = ql * u[3 * i];
+        out[3 * i + 1] = ql * u[3 * i + 1];
+        out[3 * i + 2] = ql * u[3 * i + 2];
+    }
+    return nu;
+}
+
+void sxo_normalise_rows(double *v, size_t n) {
+    for (size_t i = 0; i < n; ++i) {
+        double *r = v + 3 * i;
+        double ql = sqrt(pow(r[0], 2) + pow(r[1], 2) + pow(r[2], 2));
+        if (ql != 0) {
+            double s = 1.0 / ql;
+            r[0] *= s; r[1] *= s; r[2] *= s;
+        }
+    }
+}
+
+size_t sxo_scan(double from, double to, size_t points, double exponent,
+                const double base[3], double *out) {
+    size_t n = 0;
+#define PUSH(s) do { out[3*n] = (s)*base[0]; out[3*n+1] = (s)*base[1]; out[3*n+2] = (s)*base[2]; ++n; } while (0)
+    if (points == 0) return 0;
+    if (points == 1) { PUSH((from + to) / 2); return n; }
+    if (points == 2) { PUSH(from); PUSH(to); return n; }
+    PUSH(from);
+    for (size_t j = 1; j < points - 1; ++j) {
+        /* single-precision powf, as at parameters.cpp:1151 */
+        double scal = from + (double)powf((float)(j * 1.0 / (double)(points - 1)), (float)exponent) * (to - from);
+        PUSH(scal);
+    }
+    PUSH(to);
+#undef PUSH
+    return n;
+}

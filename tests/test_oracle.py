@@ -1,0 +1,175 @@
+"""Pins the FP64 oracle (oracle/sx_oracle.c) before it is trusted as the
+checker of the HIP path: brute force, closed forms, invariants and the golden
+vectors written by the independent numpy implementation in tests/golden/."""
+import glob
+import math
+import os
+
+import numpy as np
+import pytest
+
+GOLDEN = sorted(glob.glob(os.path.join(os.path.dirname(__file__), "golden", "*.npz")))
+
+
+def naive_dft(x, sign):
+    n = x.size
+    k = np.arange(n)
+    W = np.exp(sign * 2j * np.pi * np.outer(k, k) / n)
+    return W @ x
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 4, 5, 7, 8, 12, 13, 30, 97, 128, 400, 1000])
+def test_fft_vs_naive_dft(oracle_mod, n):
+    rng = np.random.default_rng(n)
+    x = rng.normal(size=n) + 1j * rng.normal(size=n)
+    for sign in (-1, 1):
+        ref = naive_dft(x, sign)
+        got = oracle_mod.fft(x, sign)
+        assert np.max(np.abs(got - ref)) <= 1e-12 * max(1.0, np.max(np.abs(ref)))
+    # and against pocketfft at larger sizes
+    assert np.allclose(oracle_mod.fft(x, -1), np.fft.fft(x), rtol=0, atol=1e-11 * n)
+
+
+@pytest.mark.parametrize("NF", [1, 2, 9, 50, 201])
+def test_fft_correlation_equals_direct_conjugate(oracle_mod, NF):
+    """smath.cpp:141-156 and smath.cpp:51-76 give complex-conjugate results."""
+    rng = np.random.default_rng(NF)
+    a = rng.normal(size=NF) + 1j * rng.normal(size=NF)
+    c_fft = oracle_mod.autocorrelate_fft(a)
+    c_dir = oracle_mod.autocorrelate_direct(a)
+    brute = np.array([np.sum(np.conj(a[:NF - t]) * a[t:]) / (NF - t) for t in range(NF)])
+    assert np.allclose(c_fft, brute, rtol=0, atol=1e-12 * NF)
+    assert np.allclose(c_dir, np.conj(brute), rtol=0, atol=1e-12 * NF)
+
+
+@pytest.mark.parametrize("path", GOLDEN, ids=[os.path.basename(p) for p in GOLDEN])
+def test_oracle_matches_golden(oracle_mod, path):
+    g = np.load(path)
+    kind = str(g["kind"])
+    frames = g["frames"]
+    coords = frames if kind == "all" else np.ascontiguousarray(frames.transpose(1, 0, 2))
+    for tag, (dk, mt) in {"autocorrelate_fftw": ("autocorrelate", "fftw"),
+                          "autocorrelate_direct": ("autocorrelate", "direct"),
+                          "square_fftw": ("square", "fftw"),
+                          "plain_fftw": ("plain", "fftw")}.items():
+        for s, ql in enumerate(g["qlen"]):
+            fqt, fq, fq2 = oracle_mod.compute(kind, coords, g["b"], ql * g["u"], dk, mt, nthreads=2)
+            ref = g[f"fqt_{tag}"][s]
+            scale = max(np.max(np.abs(ref)), 1e-300)
+            assert np.max(np.abs(fqt - ref)) <= 1e-12 * scale, tag
+            assert abs(fq - g[f"fq_{tag}"][s]) <= 1e-12 * scale, tag
+            assert abs(fq2 - g[f"fq2_{tag}"][s]) <= 1e-12 * scale ** 2 + 1e-300, tag
+
+
+def _traj(NA, NF, seed, box=20.0, step=0.2, static=False):
+    rng = np.random.default_rng(seed)
+    start = rng.uniform(0, box, size=(NA, 3))
+    if static:
+        frames = np.repeat(start[None], NF, axis=0)
+    else:
+        frames = start[None] + np.cumsum(rng.normal(0, step, size=(NF, NA, 3)), axis=0)
+    return frames.astype(np.float32)
+
+
+def test_static_atoms_give_constant_correlation(oracle_mod):
+    frames = _traj(6, 20, 1, static=True)
+    b = np.array([1.0, -2.0, 3.0, 0.5, 1.5, -0.7])
+    q = np.array([[0.3, -1.1, 0.7], [1.2, 0.2, -0.4]])
+    fqt, fq, fq2 = oracle_mod.compute("all", frames, b, q)
+    A = [oracle_mod.scatter_all(frames, b, qq)[0] for qq in q]
+    expect = np.mean([abs(x) ** 2 for x in A])
+    assert np.allclose(fqt, expect, rtol=1e-12, atol=0)
+    assert abs(fq - expect) <= 1e-12 * expect
+    # self with one static atom: C(tau) = b^2 for every tau and vector
+    one = frames[:, :1, :].transpose(1, 0, 2)
+    fqt, fq, fq2 = oracle_mod.compute("self", one, b[:1], q)
+    assert np.allclose(fqt, b[0] ** 2, rtol=1e-12, atol=0)
+    assert abs(fq2 - b[0] ** 4) <= 1e-12 * b[0] ** 4
+
+
+def test_single_atom_all_equals_self(oracle_mod):
+    frames = _traj(1, 33, 2)
+    b = np.array([2.5])
+    q = np.array([[0.4, 0.5, -0.6], [-1.0, 0.1, 0.2], [0.0, 0.0, 2.0]])
+    for dk in ("autocorrelate", "square", "plain"):
+        a = oracle_mod.compute("all", frames, b, q, dk)
+        s = oracle_mod.compute("self", frames.transpose(1, 0, 2), b, q, dk)
+        assert np.allclose(a[0], s[0], rtol=0, atol=1e-12)
+        assert abs(a[1] - s[1]) < 1e-12 and abs(a[2] - s[2]) < 1e-12
+    # C(0) = |b|^2 for a single moving atom, whatever the path
+    fqt, _, _ = oracle_mod.compute("all", frames, b, q, "autocorrelate")
+    assert abs(fqt[0] - b[0] ** 2) < 1e-12
+
+
+def test_self_is_linear_in_atoms(oracle_mod):
+    frames = _traj(5, 16, 3)
+    atoms = frames.transpose(1, 0, 2).copy()
+    b = np.array([1.0, 2.0, -1.5, 0.3, 4.0])
+    q = np.array([[0.7, 0.1, 0.2], [-0.3, 0.9, 0.4]])
+    whole = oracle_mod.compute("self", atoms, b, q)
+    parts = [oracle_mod.compute("self", atoms[j:j + 1], b[j:j + 1], q) for j in range(5)]
+    assert np.allclose(whole[0], sum(p[0] for p in parts), rtol=1e-12, atol=1e-12)
+    assert abs(whole[2] - sum(p[2] for p in parts)) < 1e-10
+
+
+def test_square_and_plain_definitions(oracle_mod):
+    frames = _traj(4, 10, 4)
+    b = np.array([1.0, -0.5, 2.0, 0.25])
+    q = np.array([[0.5, 0.5, 0.5], [1.0, -1.0, 0.0]])
+    A = np.array([oracle_mod.scatter_all(frames, b, qq) for qq in q])
+    fqt, fq, fq2 = oracle_mod.compute("all", frames, b, q, "square")
+    assert np.allclose(fqt, np.mean(np.abs(A) ** 2, axis=0), rtol=1e-13, atol=0)
+    fqt, fq, fq2 = oracle_mod.compute("all", frames, b, q, "plain")
+    assert np.allclose(fqt, A.mean(axis=0), rtol=1e-13, atol=1e-14)
+    a = A.mean(axis=1)
+    assert abs(fq2 - np.mean(np.abs(a) ** 2)) < 1e-12
+
+
+def test_gaussian_walk_self_closed_form(oracle_mod):
+    """<C(tau)> = b^2 exp(-q^2 sigma^2 tau / 2) for a Gaussian walk."""
+    NA, NF, sigma, ql = 400, 40, 0.1, 2.0
+    rng = np.random.default_rng(5)
+    frames = (rng.uniform(0, 10, size=(NA, 1, 3)) + np.cumsum(rng.normal(0, sigma, size=(NA, NF, 3)), axis=1)).astype(np.float32)
+    u = rng.normal(size=(20, 3)); u /= np.linalg.norm(u, axis=1)[:, None]
+    fqt, _, _ = oracle_mod.compute("self", frames, np.ones(NA), ql * u, nthreads=4)
+    tau = np.arange(10)
+    expect = NA * np.exp(-ql ** 2 * sigma ** 2 * tau / 2)
+    assert np.allclose(fqt[:10].real, expect, rtol=0.03)
+    assert np.max(np.abs(fqt[:10].imag)) < 0.03 * NA
+
+
+def test_debye_limit_of_orientational_average(oracle_mod):
+    """Static pair: the orientational mean of |A|^2 tends to the Debye sum."""
+    r = np.array([[[0.0, 0.0, 0.0], [1.7, 2.2, -0.9]]], dtype=np.float32)
+    b = np.array([1.3, -0.8])
+    n = 4000  # Fibonacci sphere: a near-uniform deterministic set
+    k = np.arange(n) + 0.5
+    th = np.arccos(1 - 2 * k / n)
+    ph = np.pi * (1 + 5 ** 0.5) * k
+    u = np.stack([np.cos(ph) * np.sin(th), np.sin(ph) * np.sin(th), np.cos(th)], axis=1)
+    ql = 1.9
+    d = np.linalg.norm(r[0, 1].astype(np.float64) - r[0, 0])
+    debye = b[0] ** 2 + b[1] ** 2 + 2 * b[0] * b[1] * math.sin(ql * d) / (ql * d)
+    fqt, fq, _ = oracle_mod.compute("all", np.repeat(r, 3, axis=0), b, ql * u, nthreads=4)
+    assert abs(fq.real - debye) < 1e-4
+
+
+def test_subvectors_and_scans(oracle_mod):
+    q = np.array([0.3, 0.4, 1.2])
+    u = oracle_mod.normalise_rows([[2.0, 0.0, 0.0], [0.0, 0.0, 0.0], [1.0, 1.0, 1.0]])
+    assert np.allclose(u[0], [1, 0, 0]) and np.all(u[1] == 0)
+    assert np.allclose(np.linalg.norm(u[2]), 1.0)
+    qv = oracle_mod.init_subvectors(q, u)
+    assert np.allclose(qv, 1.3 * u)
+    assert np.allclose(oracle_mod.init_subvectors(q, None), [q])
+    assert len(oracle_mod.scan(0.1, 2.0, 0)) == 0
+    assert np.allclose(oracle_mod.scan(0.1, 2.0, 1)[:, 0], [1.05])
+    assert np.allclose(oracle_mod.scan(0.1, 2.0, 2)[:, 0], [0.1, 2.0])
+    s = oracle_mod.scan(0.1, 2.0, 10, base=(0, 0, 1))[:, 2]
+    j = np.arange(10)
+    lin = 0.1 + (j / 9).astype(np.float32).astype(np.float64) * 1.9
+    lin[-1] = 2.0
+    assert np.array_equal(s, lin)
+    s2 = oracle_mod.scan(0.0, 1.0, 5, exponent=2.0)[:, 0]
+    frac = (np.arange(5) / 4).astype(np.float32)
+    assert np.allclose(s2[1:4], (frac * frac)[1:4].astype(np.float64), rtol=0, atol=1e-7)
