@@ -1,0 +1,123 @@
+/*
+ * sassena_amd.h -- C ABI of the MI355X (gfx950) scattering-device hot path.
+ *
+ * One context (sx_ctx) holds one staged trajectory block in HBM and computes
+ * orientationally averaged intermediate scattering functions for q shells.
+ * Plain C types only: no C++ types or exceptions cross this boundary.
+ * Every entry point returns SX_OK (0) or a negative SX_ERR_* code; the text
+ * of the last error is available from sx_last_error().  A context is driven by
+ * one host thread and runs its kernels on its own HIP stream; it is not
+ * thread-safe.
+ *
+ * Reference interfaces replaced (paths relative to the Sassena source tree):
+ *   sx_create/sx_destroy   ScatterDeviceFactory::create + device destructor
+ *                          (include/scatter_devices/scatter_device_factory.hpp:28-33,
+ *                           src/scatter_devices/scatter_device_factory.cpp:145-200,
+ *                           src/scatter_devices/all_vectors_scatter_device.cpp:32-55,134-145)
+ *   sx_stage               DataStagerByFrame::stage / DataStagerByAtom::stage
+ *                          (include/stager/data_stager.hpp:42-100,
+ *                           src/stager/data_stager.cpp:72-118, 214-338)
+ *   sx_set_factors         ScatterFactors::update (src/scatter_devices/scatter_factors.cpp:56-78)
+ *   sx_compute_shell(s)    AllVectorsScatterDevice::compute / SelfVectorsScatterDevice::compute
+ *                          (src/scatter_devices/all_vectors_scatter_device.cpp:238-361,
+ *                           src/scatter_devices/self_vectors_scatter_device.cpp:145-239);
+ *                          outputs in the layout of HDF5WriterClient::write
+ *                          (include/services/file_writer_service.hpp:64)
+ */
+#ifndef SASSENA_AMD_H
+#define SASSENA_AMD_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SX_OK 0
+#define SX_ERR_ARG (-1)         /* invalid argument or size */
+#define SX_ERR_HIP (-2)         /* a HIP runtime call failed */
+#define SX_ERR_STATE (-3)       /* call out of order (e.g. compute before stage) */
+#define SX_ERR_NOMEM (-4)       /* device or host allocation failed */
+#define SX_ERR_UNSUPPORTED (-5) /* configuration not supported by this build */
+
+/* scattering.type (parameters.cpp:372, 400-402) */
+#define SX_SCATTERING_ALL 0
+#define SX_SCATTERING_SELF 1
+/* scattering.dsp.type (parameters.cpp:375, 502-511) */
+#define SX_DSP_AUTOCORRELATE 0
+#define SX_DSP_SQUARE 1
+#define SX_DSP_PLAIN 2
+/* scattering.dsp.method (parameters.cpp:376): "fftw" or "direct".  The
+   direct method's result is the complex conjugate of the fftw method's
+   (smath.cpp:51-76 vs 141-156); both are reproduced. */
+#define SX_METHOD_FFT 0
+#define SX_METHOD_DIRECT 1
+/* host coordinate layouts accepted by sx_stage (float32, Angstrom) */
+#define SX_LAYOUT_FRAMES_AOS 0 /* [NF][NA][3]: DataStagerByFrame (data_stager.cpp:111-113) */
+#define SX_LAYOUT_ATOMS_AOS 1  /* [NA][NF][3]: DataStagerByAtom (data_stager.cpp:327-330) */
+#define SX_LAYOUT_SOA 2        /* [3][NA][NF]: the device layout, copied as is */
+
+typedef struct sx_ctx sx_ctx;
+
+typedef struct sx_config {
+    int device;          /* HIP device ordinal */
+    int scattering_type; /* SX_SCATTERING_* */
+    int dsp_type;        /* SX_DSP_* */
+    int dsp_method;      /* SX_METHOD_* */
+    int profile;         /* nonzero: time every kernel with HIP events */
+    int vector_block;    /* 0: automatic; else q vectors per wavefront (4, 8 or 16) */
+    int atom_split;      /* 0: automatic; else atom ranges per amplitude tile */
+    int reserved[9];
+} sx_config;
+
+/* Per-kernel timing of the last sx_compute_* call (profile != 0). */
+typedef struct sx_timing {
+    double amplitude_ms;   /* amplitude kernel ("all") or fused amplitude+FFT ("self") */
+    double correlate_ms;   /* per-vector FFT power spectrum kernel ("all") */
+    double finalize_ms;    /* shell transform, normalisation and fq/fq2 */
+    double total_ms;       /* first launch to last completion, device clock */
+    size_t amplitude_launches;
+    size_t correlate_launches;
+    size_t terms;          /* atom*frame*vector terms of the call */
+    size_t pad_length;     /* FFT length L used for the time correlation */
+} sx_timing;
+
+const char *sx_version(void);
+
+int sx_create(const sx_config *cfg, sx_ctx **out);
+void sx_destroy(sx_ctx *ctx);
+const char *sx_last_error(const sx_ctx *ctx);
+
+/* Copies NA atoms x NF frames of float32 coordinates into device HBM as SoA
+   x/y/z rows, atom-major.  The caller keeps ownership of xyz.  Replaces any
+   previously staged block. */
+int sx_stage(sx_ctx *ctx, const float *xyz, size_t NA, size_t NF, int layout);
+
+/* Scattering factor b_j of every staged atom (same for all shells). */
+int sx_set_factors(sx_ctx *ctx, const double *b, size_t NA);
+
+/* One q shell: qv holds NM orientation-scaled vectors [NM][3] (the output of
+   AbstractVectorsScatterDevice::init_subvectors).  fqt receives NF complex
+   values interleaved (re, im), fq and fq2 one complex value each: exactly the
+   arguments of HDF5WriterClient::write (fq0 is fqt[0]). */
+int sx_compute_shell(sx_ctx *ctx, const double *qv, size_t NM,
+                     double *fqt, double *fq, double *fq2);
+
+/* nshells shells of NM vectors each in one batched launch sequence.
+   qv: [nshells][NM][3]; b_shell: NULL (use sx_set_factors) or
+   [nshells][NA] factors per shell (X-ray form factors depend on |q|);
+   fqt: [nshells][NF][2]; fq, fq2: [nshells][2]. */
+int sx_compute_shells(sx_ctx *ctx, size_t nshells, const double *qv, size_t NM,
+                      const double *b_shell, double *fqt, double *fq, double *fq2);
+
+/* Timing of the last compute call (zeros unless cfg.profile was set). */
+int sx_get_timing(const sx_ctx *ctx, sx_timing *out);
+
+/* FFT length the context uses for NF frames (>= 2NF), 0 if unsupported. */
+size_t sx_pad_length(size_t NF);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SASSENA_AMD_H */

@@ -1,0 +1,456 @@
+// sx_capi.hip -- the C ABI (include/sassena_amd.h) over the gfx950 kernels.
+//
+// A context owns one HIP stream (default priority), the staged SoA block and
+// the work buffers, which grow on demand and are kept between calls.  Host
+// code here only validates, lays out small tables (q/(2 pi), twiddles) and
+// sequences launches; every arithmetic step of the path runs on the GPU.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/sassena_amd.h"
+#include "sx_internal.h"
+
+namespace {
+
+constexpr double kTwoPi = 6.283185307179586476925286766559;
+constexpr size_t kStageChunkFloats = size_t(64) << 20;   // 256 MB staging chunks
+constexpr size_t kAmpBudgetBytes = size_t(24) << 30;      // amplitude buffer per batch
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t cap = 0;
+    hipError_t ensure(size_t bytes) {
+        if (bytes <= cap && p) return hipSuccess;
+        if (p) {
+            hipError_t e = hipFree(p);
+            p = nullptr;
+            cap = 0;
+            if (e != hipSuccess) return e;
+        }
+        hipError_t e = hipMalloc(&p, bytes ? bytes : 16);
+        if (e == hipSuccess) cap = bytes;
+        return e;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+    template <typename T>
+    T *as() const {
+        return static_cast<T *>(p);
+    }
+};
+
+size_t round_up(size_t x, size_t m) { return (x + m - 1) / m * m; }
+
+}  // namespace
+
+struct sx_ctx {
+    sx_config cfg{};
+    std::string err;
+    hipStream_t stream = nullptr;
+    size_t NA = 0, NF = 0, NAp = 0, NFp = 0;
+    bool staged = false, have_b = false;
+    sx::FftPlan plan;
+    bool plan_ok = false;
+    DevBuf coords, b, twL, twL64, tw64, w;
+    DevBuf qrev, brow, bshell, amp, ppart, apart, fqt, fq, fq2, stage;
+    sx_timing timing{};
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+};
+
+namespace {
+
+int fail(sx_ctx *c, int code, const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    if (c) c->err = buf;
+    return code;
+}
+
+#define SX_HIP(ctx, expr)                                                                              \
+    do {                                                                                               \
+        hipError_t e_ = (expr);                                                                        \
+        if (e_ != hipSuccess)                                                                          \
+            return fail((ctx), e_ == hipErrorOutOfMemory ? SX_ERR_NOMEM : SX_ERR_HIP, "%s failed: %s (%s:%d)", \
+                        #expr, hipGetErrorString(e_), __FILE__, __LINE__);                             \
+    } while (0)
+
+int pick_vector_block(const sx_ctx *c) {
+    if (c->cfg.vector_block == 4 || c->cfg.vector_block == 8 || c->cfg.vector_block == 16)
+        return c->cfg.vector_block;
+    return 8;
+}
+
+int pick_split(const sx_ctx *c, int V, int vb) {
+    if (c->cfg.atom_split > 0) return std::max(1, std::min(c->cfg.atom_split, (int)(c->NAp / sx::kAtomChunk)));
+    const long ftiles = (long)(c->NFp / sx::kWave);
+    const long vgroups = (V + 4L * vb - 1) / (4L * vb);
+    const long base = ftiles * vgroups;
+    long split = (2048 + base - 1) / base;
+    split = std::min<long>(split, 16);
+    split = std::min<long>(split, (long)(c->NAp / 64));
+    return (int)std::max<long>(split, 1);
+}
+
+int prepare_tables(sx_ctx *c) {
+    c->plan_ok = sx::make_fft_plan(c->NF, &c->plan);
+    if (!c->plan_ok) return SX_OK;  // only the autocorrelation needs it
+    const int L = c->plan.L;
+    std::vector<float2> tw(L);
+    std::vector<double2> twf(L), tw64(L);
+    for (int e = 0; e < L; ++e) {
+        const double ang = kTwoPi * (double)e / (double)L;
+        const double cs = std::cos(ang), sn = std::sin(ang);
+        tw[e] = make_float2((float)cs, (float)-sn);  // exp(-2 pi i e / L)
+        twf[e] = make_double2(cs, -sn);              // exp(-2 pi i e / L)
+        tw64[e] = make_double2(cs, sn);              // exp(+2 pi i e / L)
+    }
+    SX_HIP(c, c->twL.ensure(sizeof(float2) * L));
+    SX_HIP(c, c->tw64.ensure(sizeof(double2) * L));
+    SX_HIP(c, c->twL64.ensure(sizeof(double2) * L));
+    SX_HIP(c, hipMemcpyAsync(c->twL64.p, twf.data(), sizeof(double2) * L, hipMemcpyHostToDevice, c->stream));
+    SX_HIP(c, c->w.ensure(sizeof(double2) * L));
+    SX_HIP(c, hipMemcpyAsync(c->twL.p, tw.data(), sizeof(float2) * L, hipMemcpyHostToDevice, c->stream));
+    SX_HIP(c, hipMemcpyAsync(c->tw64.p, tw64.data(), sizeof(double2) * L, hipMemcpyHostToDevice, c->stream));
+    SX_HIP(c, sx::launch_weights(L, (int)c->NF, c->w.as<double2>(), c->stream));
+    SX_HIP(c, hipStreamSynchronize(c->stream));
+    return SX_OK;
+}
+
+float elapsed(hipEvent_t a, hipEvent_t b) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, a, b) != hipSuccess) return 0.f;
+    return ms;
+}
+
+// One batch of shells [s0, s0+ns) through the configured path; outputs land in
+// the device buffers at shell offset 0 and are copied to host by the caller.
+int run_batch(sx_ctx *c, int ns, size_t NM, const double *qv, const double *b_shell, double *fqt_h, double *fq_h,
+              double *fq2_h) {
+    const bool self = c->cfg.scattering_type == SX_SCATTERING_SELF;
+    const int dsp = c->cfg.dsp_type;
+    const int V = (int)(ns * NM);
+    const int NF = (int)c->NF;
+    const bool prof = c->cfg.profile != 0;
+
+    // q / (2 pi): the kernels evaluate phases in turns
+    std::vector<double> qh(3 * (size_t)V);
+    for (size_t i = 0; i < qh.size(); ++i) qh[i] = qv[i] / kTwoPi;
+    SX_HIP(c, c->qrev.ensure(sizeof(double) * qh.size()));
+    SX_HIP(c, hipMemcpyAsync(c->qrev.p, qh.data(), sizeof(double) * qh.size(), hipMemcpyHostToDevice, c->stream));
+
+    const float *bptr = c->b.as<float>();
+    const int *browp = nullptr;
+    if (b_shell) {
+        std::vector<float> bs((size_t)ns * c->NAp, 0.f);
+        for (int s = 0; s < ns; ++s)
+            for (size_t j = 0; j < c->NA; ++j) bs[(size_t)s * c->NAp + j] = (float)b_shell[(size_t)s * c->NA + j];
+        SX_HIP(c, c->bshell.ensure(sizeof(float) * bs.size()));
+        SX_HIP(c, hipMemcpyAsync(c->bshell.p, bs.data(), sizeof(float) * bs.size(), hipMemcpyHostToDevice,
+                                 c->stream));
+        bptr = c->bshell.as<float>();
+        if (!self) {
+            std::vector<int> br(V);
+            for (int v = 0; v < V; ++v) br[v] = v / (int)NM;
+            SX_HIP(c, c->brow.ensure(sizeof(int) * V));
+            SX_HIP(c, hipMemcpyAsync(c->brow.p, br.data(), sizeof(int) * V, hipMemcpyHostToDevice, c->stream));
+            browp = c->brow.as<int>();
+        }
+    }
+
+    SX_HIP(c, c->fqt.ensure(sizeof(double) * 2 * (size_t)ns * NF));
+    SX_HIP(c, c->fq.ensure(sizeof(double) * 2 * ns));
+    SX_HIP(c, c->fq2.ensure(sizeof(double) * 2 * ns));
+
+    sx::FinalArgs fa{};
+    fa.NF = NF;
+    fa.NM = (int)NM;
+    fa.nshells = ns;
+    fa.conj = (dsp == SX_DSP_AUTOCORRELATE && c->cfg.dsp_method == SX_METHOD_DIRECT) ? 1 : 0;
+    fa.fqt = c->fqt.as<double>();
+    fa.fq = c->fq.as<double>();
+    fa.fq2 = c->fq2.as<double>();
+
+    if (prof) SX_HIP(c, hipEventRecord(c->ev[0], c->stream));
+    if (!self) {
+        const int vb = pick_vector_block(c);
+        const int split = pick_split(c, V, vb);
+        const size_t split_stride = (size_t)V * c->NFp;
+        SX_HIP(c, c->amp.ensure(sizeof(float2) * split_stride * split));
+        sx::AmpArgs aa;
+        aa.coords = c->coords.as<float>();
+        aa.NAp = c->NAp;
+        aa.NFp = c->NFp;
+        aa.NF = NF;
+        aa.qrev = c->qrev.as<double>();
+        aa.b = bptr;
+        aa.brow = browp;
+        aa.V = V;
+        aa.vb = vb;
+        aa.split = split;
+        aa.amp = c->amp.as<float2>();
+        SX_HIP(c, sx::launch_amp_all(aa, c->stream));
+        c->timing.amplitude_launches++;
+        if (prof) SX_HIP(c, hipEventRecord(c->ev[1], c->stream));
+        int chunks = (int)std::min<size_t>(NM, std::max<size_t>(1, (1024 + ns - 1) / ns));
+        SX_HIP(c, c->apart.ensure(sizeof(double) * 4 * (size_t)ns * chunks));
+        fa.chunks = chunks;
+        fa.apart = c->apart.as<double>();
+        if (dsp == SX_DSP_AUTOCORRELATE) {
+            const sx::FftPlan &P = c->plan;
+            SX_HIP(c, c->ppart.ensure(sizeof(double) * (size_t)ns * chunks * P.L));
+            sx::CorrArgs ca{};
+            ca.plan = P;
+            ca.NF = NF;
+            ca.NM = (int)NM;
+            ca.nshells = ns;
+            ca.chunks = chunks;
+            ca.twL = c->twL.as<float2>();
+            ca.twL64 = c->twL64.as<double2>();
+            ca.w = c->w.as<double2>();
+            ca.ppart = c->ppart.as<double>();
+            ca.apart = c->apart.as<double>();
+            ca.amp = c->amp.as<float2>();
+            ca.NFp = c->NFp;
+            ca.split = split;
+            ca.split_stride = split_stride;
+            SX_HIP(c, sx::launch_corr_all(ca, c->stream));
+            fa.ppart = c->ppart.as<double>();
+        } else {
+            SX_HIP(c, sx::launch_series_all(dsp == SX_DSP_SQUARE ? 1 : 2, c->amp.as<float2>(), c->NFp, split,
+                                            split_stride, NF, (int)NM, ns, chunks, c->fqt.as<double>(),
+                                            c->apart.as<double>(), c->stream));
+            fa.ppart = nullptr;
+        }
+        c->timing.correlate_launches++;
+    } else {
+        if (prof) SX_HIP(c, hipEventRecord(c->ev[1], c->stream));
+        const long pairs = (long)c->NA * (long)NM;
+        int chunks = (int)std::min<long>(pairs, std::max<long>(1, (2048 + ns - 1) / ns));
+        const sx::FftPlan &P = c->plan;
+        SX_HIP(c, c->apart.ensure(sizeof(double) * 4 * (size_t)ns * chunks));
+        SX_HIP(c, c->ppart.ensure(sizeof(double) * (size_t)ns * chunks * P.L));
+        sx::CorrArgs ca{};
+        ca.plan = P;
+        ca.NF = NF;
+        ca.NM = (int)NM;
+        ca.nshells = ns;
+        ca.chunks = chunks;
+        ca.twL = c->twL.as<float2>();
+        ca.w = c->w.as<double2>();
+        ca.ppart = c->ppart.as<double>();
+        ca.apart = c->apart.as<double>();
+        ca.coords = c->coords.as<float>();
+        ca.NAp = c->NAp;
+        ca.NFp = c->NFp;
+        ca.NA = (int)c->NA;
+        ca.qrev = c->qrev.as<double>();
+        ca.b = bptr;
+        ca.brow_per_shell = b_shell ? 1 : 0;
+        SX_HIP(c, sx::launch_corr_self(ca, c->stream));
+        c->timing.correlate_launches++;
+        fa.chunks = chunks;
+        fa.apart = c->apart.as<double>();
+        fa.ppart = c->ppart.as<double>();
+    }
+    if (prof) SX_HIP(c, hipEventRecord(c->ev[2], c->stream));
+    fa.L = c->plan_ok ? c->plan.L : 0;
+    fa.tw64 = c->tw64.as<double2>();
+    SX_HIP(c, sx::launch_final(fa, c->stream));
+    if (prof) SX_HIP(c, hipEventRecord(c->ev[3], c->stream));
+
+    SX_HIP(c, hipMemcpyAsync(fqt_h, c->fqt.p, sizeof(double) * 2 * (size_t)ns * NF, hipMemcpyDeviceToHost,
+                             c->stream));
+    SX_HIP(c, hipMemcpyAsync(fq_h, c->fq.p, sizeof(double) * 2 * ns, hipMemcpyDeviceToHost, c->stream));
+    SX_HIP(c, hipMemcpyAsync(fq2_h, c->fq2.p, sizeof(double) * 2 * ns, hipMemcpyDeviceToHost, c->stream));
+    SX_HIP(c, hipStreamSynchronize(c->stream));
+    if (prof) {
+        if (self) {
+            c->timing.amplitude_ms += elapsed(c->ev[1], c->ev[2]);
+        } else {
+            c->timing.amplitude_ms += elapsed(c->ev[0], c->ev[1]);
+            c->timing.correlate_ms += elapsed(c->ev[1], c->ev[2]);
+        }
+        c->timing.finalize_ms += elapsed(c->ev[2], c->ev[3]);
+        c->timing.total_ms += elapsed(c->ev[0], c->ev[3]);
+    }
+    c->timing.terms += (size_t)V * c->NA * c->NF;
+    return SX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *sx_version(void) { return "sassena_amd 0.1.0 (gfx950)"; }
+
+const char *sx_last_error(const sx_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+size_t sx_pad_length(size_t NF) {
+    sx::FftPlan p;
+    return sx::make_fft_plan(NF, &p) ? (size_t)p.L : 0;
+}
+
+int sx_create(const sx_config *cfg, sx_ctx **out) {
+    if (!cfg || !out) return SX_ERR_ARG;
+    *out = nullptr;
+    if (cfg->scattering_type != SX_SCATTERING_ALL && cfg->scattering_type != SX_SCATTERING_SELF) return SX_ERR_ARG;
+    if (cfg->dsp_type < SX_DSP_AUTOCORRELATE || cfg->dsp_type > SX_DSP_PLAIN) return SX_ERR_ARG;
+    if (cfg->dsp_method != SX_METHOD_FFT && cfg->dsp_method != SX_METHOD_DIRECT) return SX_ERR_ARG;
+    if (cfg->scattering_type == SX_SCATTERING_SELF && cfg->dsp_type != SX_DSP_AUTOCORRELATE)
+        return SX_ERR_UNSUPPORTED;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || cfg->device < 0 || cfg->device >= ndev) return SX_ERR_HIP;
+    if (hipSetDevice(cfg->device) != hipSuccess) return SX_ERR_HIP;
+    sx_ctx *c = new (std::nothrow) sx_ctx();
+    if (!c) return SX_ERR_NOMEM;
+    c->cfg = *cfg;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return SX_ERR_HIP;
+    }
+    for (auto &e : c->ev)
+        if (hipEventCreate(&e) != hipSuccess) {
+            sx_destroy(c);
+            return SX_ERR_HIP;
+        }
+    *out = c;
+    return SX_OK;
+}
+
+void sx_destroy(sx_ctx *c) {
+    if (!c) return;
+    (void)hipSetDevice(c->cfg.device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (DevBuf *d : {&c->coords, &c->b, &c->twL, &c->twL64, &c->tw64, &c->w, &c->qrev, &c->brow, &c->bshell, &c->amp,
+                      &c->ppart, &c->apart, &c->fqt, &c->fq, &c->fq2, &c->stage})
+        d->release();
+    for (auto &e : c->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+int sx_stage(sx_ctx *c, const float *xyz, size_t NA, size_t NF, int layout) {
+    if (!c) return SX_ERR_ARG;
+    if (!xyz || NA == 0 || NF == 0) return fail(c, SX_ERR_ARG, "sx_stage: empty coordinate block");
+    if (NA > (size_t)1 << 28 || NF > (size_t)1 << 24)
+        return fail(c, SX_ERR_ARG, "sx_stage: block too large (NA=%zu NF=%zu)", NA, NF);
+    if (layout != SX_LAYOUT_FRAMES_AOS && layout != SX_LAYOUT_ATOMS_AOS && layout != SX_LAYOUT_SOA)
+        return fail(c, SX_ERR_ARG, "sx_stage: unknown layout %d", layout);
+    SX_HIP(c, hipSetDevice(c->cfg.device));
+    c->staged = false;
+    c->have_b = false;
+    c->NA = NA;
+    c->NF = NF;
+    c->NAp = round_up(NA, 64);
+    c->NFp = round_up(NF, sx::kWave);
+    const size_t plane = c->NAp * c->NFp;
+    c->coords.release();
+    SX_HIP(c, c->coords.ensure(sizeof(float) * 3 * plane));
+    SX_HIP(c, hipMemsetAsync(c->coords.p, 0, sizeof(float) * 3 * plane, c->stream));
+    float *dst = c->coords.as<float>();
+    if (layout == SX_LAYOUT_SOA) {
+        for (int k = 0; k < 3; ++k)
+            SX_HIP(c, hipMemcpy2DAsync(dst + k * plane, sizeof(float) * c->NFp, xyz + k * NA * NF, sizeof(float) * NF,
+                                       sizeof(float) * NF, NA, hipMemcpyHostToDevice, c->stream));
+    } else if (layout == SX_LAYOUT_FRAMES_AOS) {
+        // frames per chunk; the transpose grid covers 64 frames per y block (<= 65535)
+        const size_t per = std::min<size_t>(std::max<size_t>(1, kStageChunkFloats / (3 * NA)), size_t(65535) * 64);
+        SX_HIP(c, c->stage.ensure(sizeof(float) * 3 * NA * std::min(per, NF)));
+        for (size_t f0 = 0; f0 < NF; f0 += per) {
+            const size_t nf = std::min(per, NF - f0);
+            SX_HIP(c, hipMemcpyAsync(c->stage.p, xyz + f0 * NA * 3, sizeof(float) * 3 * NA * nf,
+                                     hipMemcpyHostToDevice, c->stream));
+            SX_HIP(c, sx::launch_stage_frames(c->stage.as<float>(), NA, f0, nf, dst, c->NAp, c->NFp, c->stream));
+            SX_HIP(c, hipStreamSynchronize(c->stream));
+        }
+    } else {
+        // atoms per chunk; one y block per atom (<= 65535)
+        const size_t per = std::min<size_t>(std::max<size_t>(1, kStageChunkFloats / (3 * NF)), size_t(65535));
+        SX_HIP(c, c->stage.ensure(sizeof(float) * 3 * NF * std::min(per, NA)));
+        for (size_t a0 = 0; a0 < NA; a0 += per) {
+            const size_t na = std::min(per, NA - a0);
+            SX_HIP(c, hipMemcpyAsync(c->stage.p, xyz + a0 * NF * 3, sizeof(float) * 3 * NF * na,
+                                     hipMemcpyHostToDevice, c->stream));
+            SX_HIP(c, sx::launch_stage_atoms(c->stage.as<float>(), a0, na, NF, dst, c->NAp, c->NFp, c->stream));
+            SX_HIP(c, hipStreamSynchronize(c->stream));
+        }
+    }
+    SX_HIP(c, hipStreamSynchronize(c->stream));
+    c->stage.release();
+    int rc = prepare_tables(c);
+    if (rc != SX_OK) return rc;
+    c->staged = true;
+    return SX_OK;
+}
+
+int sx_set_factors(sx_ctx *c, const double *b, size_t NA) {
+    if (!c) return SX_ERR_ARG;
+    if (!c->staged) return fail(c, SX_ERR_STATE, "sx_set_factors: stage coordinates first");
+    if (!b || NA != c->NA) return fail(c, SX_ERR_ARG, "sx_set_factors: expected %zu factors, got %zu", c->NA, NA);
+    SX_HIP(c, hipSetDevice(c->cfg.device));
+    std::vector<float> bh(c->NAp, 0.f);
+    for (size_t j = 0; j < NA; ++j) bh[j] = (float)b[j];
+    SX_HIP(c, c->b.ensure(sizeof(float) * c->NAp));
+    SX_HIP(c, hipMemcpyAsync(c->b.p, bh.data(), sizeof(float) * c->NAp, hipMemcpyHostToDevice, c->stream));
+    SX_HIP(c, hipStreamSynchronize(c->stream));
+    c->have_b = true;
+    return SX_OK;
+}
+
+int sx_compute_shells(sx_ctx *c, size_t nshells, const double *qv, size_t NM, const double *b_shell, double *fqt,
+                      double *fq, double *fq2) {
+    if (!c) return SX_ERR_ARG;
+    if (!c->staged) return fail(c, SX_ERR_STATE, "compute: no staged coordinates");
+    if (!b_shell && !c->have_b) return fail(c, SX_ERR_STATE, "compute: scattering factors not set");
+    if (nshells == 0 || NM == 0 || !qv || !fqt || !fq || !fq2)
+        return fail(c, SX_ERR_ARG, "compute: empty shell list or null output");
+    if (NM > (size_t)1 << 20 || nshells > (size_t)1 << 20)
+        return fail(c, SX_ERR_ARG, "compute: too many vectors (NM=%zu, shells=%zu)", NM, nshells);
+    const bool needs_fft = c->cfg.dsp_type == SX_DSP_AUTOCORRELATE;
+    if (needs_fft && !c->plan_ok)
+        return fail(c, SX_ERR_UNSUPPORTED, "compute: NF=%zu exceeds the supported correlation length", c->NF);
+    for (size_t i = 0; i < 3 * nshells * NM; ++i)
+        if (!std::isfinite(qv[i])) return fail(c, SX_ERR_ARG, "compute: non-finite q vector component");
+    SX_HIP(c, hipSetDevice(c->cfg.device));
+    c->timing = sx_timing{};
+    c->timing.pad_length = c->plan_ok ? (size_t)c->plan.L : 0;
+    // batch shells so that the amplitude buffer stays within budget
+    size_t per = nshells;
+    if (c->cfg.scattering_type == SX_SCATTERING_ALL) {
+        const size_t per_shell = sizeof(float2) * NM * c->NFp * 16;  // worst-case split
+        per = std::max<size_t>(1, std::min(nshells, kAmpBudgetBytes / std::max<size_t>(per_shell, 1)));
+    }
+    for (size_t s0 = 0; s0 < nshells; s0 += per) {
+        const int ns = (int)std::min(per, nshells - s0);
+        int rc = run_batch(c, ns, NM, qv + 3 * NM * s0, b_shell ? b_shell + c->NA * s0 : nullptr,
+                           fqt + 2 * c->NF * s0, fq + 2 * s0, fq2 + 2 * s0);
+        if (rc != SX_OK) return rc;
+    }
+    return SX_OK;
+}
+
+int sx_compute_shell(sx_ctx *c, const double *qv, size_t NM, double *fqt, double *fq, double *fq2) {
+    return sx_compute_shells(c, 1, qv, NM, nullptr, fqt, fq, fq2);
+}
+
+int sx_get_timing(const sx_ctx *c, sx_timing *out) {
+    if (!c || !out) return SX_ERR_ARG;
+    *out = c->timing;
+    return SX_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,98 @@
+// sx_internal.h -- shared declarations of the gfx950 hot-path kernels and the
+// C-ABI host layer.  Device data layout (all in HBM, one context):
+//
+//   coords  float  [3][NAp][NFp]   SoA x/y/z rows, atom-major; NFp = NF rounded
+//                                  up to 64 (one wavefront of frames), NAp = NA
+//                                  rounded up to the atom chunk; padding is 0
+//   b       float  [nb][NAp]       scattering factors (padding atoms have b = 0)
+//   qrev    double [V][3]          q vectors divided by 2*pi (phase in turns)
+//   amp     float2 [split][V][NFp] coherent amplitudes A_v(t) per atom range
+//   ppart   double [nshells][chunks][L]   partial power spectra sum |X_f|^2
+//   apart   double [nshells][chunks][4]   partial sum a, sum |a|^2 per chunk
+//   out     double fqt [nshells][NF][2], fq/fq2 [nshells][2]
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+namespace sx {
+
+constexpr int kWave = 64;         // CDNA wavefront width
+constexpr int kAtomChunk = 8;     // atoms per FP32 chunk before the Kahan update
+constexpr int kMaxPasses = 16;
+
+// Stockham plan of one half-length FFT (Lh points); the full correlation
+// length is L = 2*Lh >= 2*NF.  Radices are ordered powers of two first and at
+// most one odd radix last, so every stride before the last pass is a power of
+// two.
+struct FftPlan {
+    int Lh = 0;
+    int L = 0;
+    int T = 0;      // threads per block
+    int E = 0;      // elements per thread per half
+    int npass = 0;
+    int radix[kMaxPasses] = {};
+    int ns[kMaxPasses] = {};
+};
+
+// Returns false if NF cannot be served by a supported FFT length.
+bool make_fft_plan(size_t NF, FftPlan *plan);
+
+// ---- launchers (all asynchronous on `stream`; return hipError_t) ----
+
+hipError_t launch_stage_frames(const float *src, size_t NA, size_t NF0, size_t nf,
+                               float *coords, size_t NAp, size_t NFp, hipStream_t stream);
+hipError_t launch_stage_atoms(const float *src, size_t a0, size_t na, size_t NF,
+                              float *coords, size_t NAp, size_t NFp, hipStream_t stream);
+
+struct AmpArgs {
+    const float *coords;   // [3][NAp][NFp]
+    size_t NAp, NFp;
+    int NF;
+    const double *qrev;    // [V][3]
+    const float *b;        // [nb][NAp]
+    const int *brow;       // [V] row of b per vector, or nullptr (row 0)
+    int V;
+    int vb;                // vectors per wavefront: 4, 8 or 16
+    int split;             // atom ranges
+    float2 *amp;           // [split][V][NFp]
+};
+hipError_t launch_amp_all(const AmpArgs &a, hipStream_t stream);
+
+struct CorrArgs {
+    FftPlan plan;
+    int NF, NM, nshells, chunks;
+    const float2 *twL;     // [L] exp(-2 pi i e / L), float
+    const double2 *twL64;  // [L] exp(-2 pi i e / L), double
+    const double2 *w;      // [L] weights of a = mean_tau C(tau)
+    double *ppart;         // [nshells][chunks][L]
+    double *apart;         // [nshells][chunks][4]
+    // "all" inputs
+    const float2 *amp; size_t NFp; int split; size_t split_stride;
+    // "self" inputs
+    const float *coords; size_t NAp; int NA;
+    const double *qrev; const float *b; int brow_per_shell;
+};
+hipError_t launch_corr_all(const CorrArgs &a, hipStream_t stream);
+hipError_t launch_corr_self(const CorrArgs &a, hipStream_t stream);
+
+hipError_t launch_weights(int L, int NF, double2 *w, hipStream_t stream);
+
+struct FinalArgs {
+    int L, NF, NM, nshells, chunks;
+    const double *ppart;   // nullptr: fqt already written by the series kernel
+    const double *apart;
+    const double2 *tw64;   // [L] exp(+2 pi i e / L), double
+    int conj;              // 1 for the direct method (conjugate result)
+    double *fqt, *fq, *fq2;
+};
+hipError_t launch_final(const FinalArgs &a, hipStream_t stream);
+
+// dsp.type square (mode 1) / plain (mode 2) on coherent amplitudes
+hipError_t launch_series_all(int mode, const float2 *amp, size_t NFp, int split,
+                             size_t split_stride, int NF, int NM, int nshells, int chunks,
+                             double *fqt, double *apart, hipStream_t stream);
+
+}  // namespace sx

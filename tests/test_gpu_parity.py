@@ -1,0 +1,191 @@
+"""Parity of the gfx950 path (through the C ABI) with the FP64 oracle and the
+golden fixtures.  Tolerance: the per-shell error max_tau |F - F_ref| /
+|F_ref(0)| must stay below 1e-5 (FP32 sincos, Kahan sums and FP32 FFT against
+FP64 references); fq and fq2 are held to the same relative bound."""
+import glob
+import os
+import sys
+
+import numpy as np
+import pytest
+
+sys.path.insert(0, os.path.dirname(__file__))
+from util import shell_error, small_trajectory, unit_vectors  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5
+GOLDEN = sorted(glob.glob(os.path.join(os.path.dirname(__file__), "golden", "*.npz")))
+
+
+@pytest.fixture(scope="module")
+def sx():
+    import sassena_amd
+    return sassena_amd
+
+
+def check_close(fqt, fq, fq2, rfqt, rfq, rfq2, tol=TOL):
+    """One shell: fqt by the per-shell error, fq against |F_ref(0)|, fq2
+    relative to itself (with a floor of tol * 1e-3 |F_ref(0)|^2)."""
+    scale = abs(rfqt[0])
+    err = shell_error(fqt, rfqt)[0]
+    assert err < tol, f"fqt shell error {err:.3e}"
+    assert abs(fq - rfq) <= tol * scale, (fq, rfq)
+    assert abs(fq2 - rfq2) <= tol * (abs(rfq2) + 1e-3 * scale ** 2), (fq2, rfq2)
+
+
+@pytest.mark.parametrize("path", GOLDEN, ids=[os.path.basename(p) for p in GOLDEN])
+@pytest.mark.parametrize("tag", ["autocorrelate_fftw", "autocorrelate_direct", "square_fftw", "plain_fftw"])
+def test_golden(sx, path, tag):
+    g = np.load(path)
+    kind = str(g["kind"])
+    dsp, method = tag.rsplit("_", 1)
+    if kind == "self" and dsp != "autocorrelate":
+        pytest.skip("self scattering supports dsp.type=autocorrelate only in this build")
+    qv = g["qlen"][:, None, None] * g["u"][None]
+    with sx.ScatterDevice(kind, dsp, method) as dev:
+        dev.stage(g["frames"], layout="frames")
+        dev.set_factors(g["b"])
+        fqt, fq, fq2 = dev.compute_shells(qv)
+    ref = g[f"fqt_{tag}"]
+    for s in range(len(g["qlen"])):
+        check_close(fqt[s], fq[s], fq2[s], ref[s], g[f"fq_{tag}"][s], g[f"fq2_{tag}"][s])
+
+
+@pytest.mark.parametrize("dsp", ["autocorrelate", "square", "plain"])
+def test_c1_all_against_oracle(sx, oracle_mod, dsp):
+    """The full CPU-runnable case: 1000 atoms, 200 frames, 10 shells x 50 vectors."""
+    from sassena_amd import synth
+    T = synth.case("c1")
+    frames = synth.to_frames_aos(T.xyz)
+    qv = np.array([T.qvectors(s) for s in range(T.nshells)])
+    with sx.ScatterDevice("all", dsp) as dev:
+        dev.stage(T.xyz, layout="soa")
+        dev.set_factors(T.b)
+        fqt, fq, fq2 = dev.compute_shells(qv)
+    for s in range(T.nshells):
+        r = oracle_mod.compute("all", frames, T.b, qv[s], dsp, nthreads=8)
+        check_close(fqt[s], fq[s], fq2[s], *r)
+
+
+def test_self_against_oracle(sx, oracle_mod):
+    frames, b = small_trajectory(120, 300, seed=7, box=30.0, step=0.2)
+    atoms = np.ascontiguousarray(frames.transpose(1, 0, 2))
+    u = unit_vectors(12, 8)
+    qlen = np.array([0.3, 1.1, 2.9])
+    qv = qlen[:, None, None] * u[None]
+    with sx.ScatterDevice("self") as dev:
+        dev.stage(atoms, layout="atoms")
+        dev.set_factors(b)
+        fqt, fq, fq2 = dev.compute_shells(qv)
+    for s in range(len(qlen)):
+        r = oracle_mod.compute("self", atoms, b, qv[s], nthreads=8)
+        check_close(fqt[s], fq[s], fq2[s], *r)
+
+
+@pytest.mark.parametrize("NF", [1, 2, 63, 64, 65, 255, 256, 257, 777, 1500, 2100, 3000, 4096, 5001])
+def test_frame_counts_against_oracle(sx, oracle_mod, NF):
+    """Every FFT length class (2^k, 3*2^k, 5*2^k), ragged frame tiles, NF=1."""
+    NA = 37
+    frames, b = small_trajectory(NA, NF, seed=NF)
+    qv = 1.7 * unit_vectors(3, NF)
+    with sx.ScatterDevice("all") as dev:
+        dev.stage(frames, layout="frames")
+        dev.set_factors(b)
+        fqt, fq, fq2 = dev.compute_shell(qv)
+    r = oracle_mod.compute("all", frames, b, qv, nthreads=8)
+    check_close(fqt, fq, fq2, *r)
+
+
+@pytest.mark.parametrize("vb,split", [(4, 1), (8, 3), (16, 2), (8, 16)])
+def test_tile_shapes_agree(sx, oracle_mod, vb, split):
+    frames, b = small_trajectory(300, 130, seed=3)
+    qv = 0.9 * unit_vectors(37, 4)
+    with sx.ScatterDevice("all", vector_block=vb, atom_split=split) as dev:
+        dev.stage(frames, layout="frames")
+        dev.set_factors(b)
+        fqt, fq, fq2 = dev.compute_shell(qv)
+    r = oracle_mod.compute("all", frames, b, qv, nthreads=8)
+    check_close(fqt, fq, fq2, *r)
+
+
+def test_layouts_are_identical(sx):
+    frames, b = small_trajectory(70, 90, seed=5)
+    qv = 1.3 * unit_vectors(5, 6)
+    soa = np.ascontiguousarray(frames.transpose(2, 1, 0))
+    atoms = np.ascontiguousarray(frames.transpose(1, 0, 2))
+    out = []
+    for arr, lay in ((frames, "frames"), (atoms, "atoms"), (soa, "soa")):
+        with sx.ScatterDevice("all") as dev:
+            dev.stage(arr, layout=lay)
+            dev.set_factors(b)
+            out.append(dev.compute_shell(qv))
+    for o in out[1:]:
+        assert np.array_equal(o[0], out[0][0]) and o[1] == out[0][1] and o[2] == out[0][2]
+
+
+def test_single_atom_and_single_vector(sx, oracle_mod):
+    frames, b = small_trajectory(1, 50, seed=9)
+    qv = np.array([[0.3, -0.2, 1.4]])
+    for kind in ("all", "self"):
+        with sx.ScatterDevice(kind) as dev:
+            dev.stage(frames, layout="frames")
+            dev.set_factors(b)
+            fqt, fq, fq2 = dev.compute_shell(qv)
+        coords = frames if kind == "all" else frames.transpose(1, 0, 2)
+        check_close(fqt, fq, fq2, *oracle_mod.compute(kind, coords, b, qv))
+        assert abs(fqt[0] - b[0] ** 2) < 1e-5 * b[0] ** 2
+
+
+def test_per_shell_factors(sx, oracle_mod):
+    frames, b = small_trajectory(50, 40, seed=11)
+    qv = np.array([0.5, 2.0])[:, None, None] * unit_vectors(4, 12)[None]
+    bs = np.stack([b, b * np.linspace(0.5, 1.5, b.size)])
+    for kind in ("all", "self"):
+        with sx.ScatterDevice(kind) as dev:
+            dev.stage(frames, layout="frames")
+            fqt, fq, fq2 = dev.compute_shells(qv, b_shell=bs)
+        coords = frames if kind == "all" else frames.transpose(1, 0, 2)
+        for s in range(2):
+            check_close(fqt[s], fq[s], fq2[s], *oracle_mod.compute(kind, coords, bs[s], qv[s]))
+
+
+def test_errors_are_reported(sx):
+    with sx.ScatterDevice("all") as dev:
+        with pytest.raises(sx.SxError) as e:
+            dev.compute_shell(np.ones((2, 3)))
+        assert e.value.code == -3
+        frames, b = small_trajectory(10, 20, seed=1)
+        dev.stage(frames, layout="frames")
+        with pytest.raises(sx.SxError):
+            dev.compute_shell(np.ones((2, 3)))  # no factors yet
+        dev.set_factors(b)
+        with pytest.raises(sx.SxError) as e:
+            dev.compute_shell(np.array([[np.nan, 0, 0]]))
+        assert e.value.code == -1
+        with pytest.raises(sx.SxError):
+            dev.set_factors(b[:5])
+    with sx.ScatterDevice("all") as dev:
+        frames, b = small_trajectory(2, 9000, seed=1)
+        dev.stage(frames, layout="frames")
+        dev.set_factors(b)
+        with pytest.raises(sx.SxError) as e:
+            dev.compute_shell(np.ones((1, 3)))
+        assert e.value.code == -5
+
+
+def test_c2_full_size_subset_against_oracle(sx, oracle_mod):
+    """Headline case at full size (20000 atoms x 2000 frames): two vectors of
+    the largest shell, and a repeat call that must reproduce bit for bit."""
+    from sassena_amd import synth
+    T = synth.case("c2")
+    frames = synth.to_frames_aos(T.xyz)
+    qv = T.qvectors(T.nshells - 1)[:2]
+    with sx.ScatterDevice("all") as dev:
+        dev.stage(T.xyz, layout="soa")
+        dev.set_factors(T.b)
+        r1 = dev.compute_shell(qv)
+        r2 = dev.compute_shell(qv)
+    assert np.array_equal(r1[0], r2[0])
+    ref = oracle_mod.compute("all", frames, T.b, qv, nthreads=8)
+    check_close(*r1, *ref)
