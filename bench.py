@@ -43,7 +43,7 @@ def cpu_model():
 def cpu_baseline(T, threads):
     import oracle
     frames = np.ascontiguousarray(np.transpose(T.xyz, (2, 1, 0)))
-    k = 2 * threads
+    k = 8 * threads
     qv = T.qvectors(T.nshells - 1)[:k]
     t0 = time.perf_counter()
     oracle.compute("all", frames, T.b, qv, "autocorrelate", "fftw", nthreads=threads)
@@ -62,6 +62,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--vector-block", type=int, default=0)
     ap.add_argument("--atom-split", type=int, default=0)
+    ap.add_argument("--generic", action="store_true",
+                    help="pass the 8000 expanded q vectors (sx_compute_shells) instead of |q| x orientations")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU only: the hot path shards by q shell with "
@@ -76,13 +78,17 @@ def main():
                         atom_split=args.atom_split)
     dev.stage(T.xyz, layout="soa")
     dev.set_factors(T.b)
+    if args.generic:
+        step = lambda: dev.compute_shells(qv)
+    else:
+        step = lambda: dev.compute_oriented(T.qlen, T.orient)
     for _ in range(args.warmup):
-        dev.compute_shells(qv)
+        step()
     amp_ms = corr_ms = fin_ms = dev_ms = 0.0
     amp_launches = 0
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        dev.compute_shells(qv)   # returns after the stream has drained
+        step()   # returns after the device stream has drained
         tm = dev.timing()
         amp_ms += tm["amplitude_ms"]
         corr_ms += tm["correlate_ms"]
@@ -109,12 +115,13 @@ def main():
         "dtype": "f32 sincos / f64 phase",
         "data": "synthetic",
         "config": {"workload": "coherent all, 20000 atoms x 2000 frames, 40 shells x 200 vectors, pad 4096",
+                   "entry": "sx_compute_shells" if args.generic else "sx_compute_shells_oriented",
                    "NA": T.NA, "NF": T.NF, "shells": T.nshells, "vectors": T.NM, "terms_per_step": T.terms()},
         "kernel_ms_per_step": {"amplitude": amp_ms / args.steps, "correlate": corr_ms / args.steps,
                                "finalize": fin_ms / args.steps, "device_total": dev_ms / args.steps},
         "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": TRANS_PEAK / 1e12,
                      "unit": "Tsincos-ops/s", "frac": achieved / TRANS_PEAK, "traffic": None,
-                     "kernel": "amp_all_kernel",
+                     "kernel": "amp_all_kernel" if args.generic else "amp_oriented_kernel",
                      "note": "transcendental-issue bound: 2 hardware sin/cos per term at quarter rate"},
     }
     dev.close()

@@ -110,6 +110,15 @@ int sx_compute_shell(sx_ctx *ctx, const double *qv, size_t NM,
 int sx_compute_shells(sx_ctx *ctx, size_t nshells, const double *qv, size_t NM,
                       const double *b_shell, double *fqt, double *fq, double *fq2);
 
+/* Shells that share one orientation set, the reference's orientation types
+   "file" and "sphere" (abstract_vectors_scatter_device.cpp:115-129): vector m
+   of shell s is q_{s,m} = qlen[s] * u[m].  u: [NM][3] unit rows (normalised
+   as parameters.cpp:932-944 does), qlen: [nshells] (|q| of each shell);
+   other arguments as sx_compute_shells.  Same results as sx_compute_shells
+   on the expanded vectors, with one FP64 product per term for the phase. */
+int sx_compute_shells_oriented(sx_ctx *ctx, size_t nshells, const double *qlen, const double *u, size_t NM,
+                               const double *b_shell, double *fqt, double *fq, double *fq2);
+
 /* Timing of the last compute call (zeros unless cfg.profile was set). */
 int sx_get_timing(const sx_ctx *ctx, sx_timing *out);
 

@@ -23,7 +23,8 @@ LAYOUT = {"frames": 0, "atoms": 1, "soa": 2}
 
 # Every symbol include/sassena_amd.h declares.
 EXPORTS = ("sx_version", "sx_create", "sx_destroy", "sx_last_error", "sx_stage", "sx_set_factors",
-           "sx_compute_shell", "sx_compute_shells", "sx_get_timing", "sx_pad_length")
+           "sx_compute_shell", "sx_compute_shells", "sx_compute_shells_oriented", "sx_get_timing",
+           "sx_pad_length")
 
 
 class SxError(RuntimeError):
@@ -68,11 +69,12 @@ def load():
     L.sx_set_factors.argtypes = [vp, dp, sz]
     L.sx_compute_shell.argtypes = [vp, dp, sz, dp, dp, dp]
     L.sx_compute_shells.argtypes = [vp, sz, dp, sz, dp, dp, dp, dp]
+    L.sx_compute_shells_oriented.argtypes = [vp, sz, dp, dp, sz, dp, dp, dp, dp]
     L.sx_get_timing.argtypes = [vp, ctypes.POINTER(sx_timing)]
     L.sx_pad_length.argtypes = [sz]
     L.sx_pad_length.restype = sz
     for name in ("sx_create", "sx_stage", "sx_set_factors", "sx_compute_shell", "sx_compute_shells",
-                 "sx_get_timing"):
+                 "sx_compute_shells_oriented", "sx_get_timing"):
         getattr(L, name).restype = ctypes.c_int
     _lib = L
     return L

@@ -62,7 +62,7 @@ struct sx_ctx {
     sx::FftPlan plan;
     bool plan_ok = false;
     DevBuf coords, b, twL, twL64, tw64, w;
-    DevBuf qrev, brow, bshell, amp, ppart, apart, fqt, fq, fq2, stage;
+    DevBuf qrev, qlen, uvec, brow, bshell, amp, ppart, apart, fqt, fq, fq2, stage;
     sx_timing timing{};
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
 };
@@ -91,6 +91,16 @@ int pick_vector_block(const sx_ctx *c) {
     if (c->cfg.vector_block == 4 || c->cfg.vector_block == 8 || c->cfg.vector_block == 16)
         return c->cfg.vector_block;
     return 8;
+}
+
+// Atom ranges per amplitude tile: enough workgroups (>= 2048, eight
+// wavefronts per SIMD) to fill the 256 CUs, at most 16 partial sums.
+int pick_split_blocks(const sx_ctx *c, long base) {
+    if (c->cfg.atom_split > 0) return std::max(1, std::min(c->cfg.atom_split, (int)(c->NAp / 16)));
+    long split = (2048 + base - 1) / base;
+    split = std::min<long>(split, 16);
+    split = std::min<long>(split, (long)(c->NAp / 64));
+    return (int)std::max<long>(split, 1);
 }
 
 int pick_split(const sx_ctx *c, int V, int vb) {
@@ -137,8 +147,8 @@ float elapsed(hipEvent_t a, hipEvent_t b) {
 
 // One batch of shells [s0, s0+ns) through the configured path; outputs land in
 // the device buffers at shell offset 0 and are copied to host by the caller.
-int run_batch(sx_ctx *c, int ns, size_t NM, const double *qv, const double *b_shell, double *fqt_h, double *fq_h,
-              double *fq2_h) {
+int run_batch(sx_ctx *c, int ns, size_t NM, const double *qv, const double *qlen, const double *u,
+              const double *b_shell, double *fqt_h, double *fq_h, double *fq2_h) {
     const bool self = c->cfg.scattering_type == SX_SCATTERING_SELF;
     const int dsp = c->cfg.dsp_type;
     const int V = (int)(ns * NM);
@@ -147,7 +157,22 @@ int run_batch(sx_ctx *c, int ns, size_t NM, const double *qv, const double *b_sh
 
     // q / (2 pi): the kernels evaluate phases in turns
     std::vector<double> qh(3 * (size_t)V);
-    for (size_t i = 0; i < qh.size(); ++i) qh[i] = qv[i] / kTwoPi;
+    if (qv) {
+        for (size_t i = 0; i < qh.size(); ++i) qh[i] = qv[i] / kTwoPi;
+    } else {
+        for (int s = 0; s < ns; ++s)
+            for (size_t m = 0; m < NM; ++m)
+                for (int k = 0; k < 3; ++k) qh[(s * NM + m) * 3 + k] = qlen[s] * u[3 * m + k] / kTwoPi;
+    }
+    const bool oriented = qv == nullptr;
+    if (oriented) {
+        std::vector<double> ql(ns);
+        for (int s = 0; s < ns; ++s) ql[s] = qlen[s] / kTwoPi;
+        SX_HIP(c, c->qlen.ensure(sizeof(double) * ns));
+        SX_HIP(c, hipMemcpyAsync(c->qlen.p, ql.data(), sizeof(double) * ns, hipMemcpyHostToDevice, c->stream));
+        SX_HIP(c, c->uvec.ensure(sizeof(double) * 3 * NM));
+        SX_HIP(c, hipMemcpyAsync(c->uvec.p, u, sizeof(double) * 3 * NM, hipMemcpyHostToDevice, c->stream));
+    }
     SX_HIP(c, c->qrev.ensure(sizeof(double) * qh.size()));
     SX_HIP(c, hipMemcpyAsync(c->qrev.p, qh.data(), sizeof(double) * qh.size(), hipMemcpyHostToDevice, c->stream));
 
@@ -186,22 +211,39 @@ int run_batch(sx_ctx *c, int ns, size_t NM, const double *qv, const double *b_sh
     if (prof) SX_HIP(c, hipEventRecord(c->ev[0], c->stream));
     if (!self) {
         const int vb = pick_vector_block(c);
-        const int split = pick_split(c, V, vb);
+        const int split = oriented ? pick_split_blocks(c, sx::oriented_base_blocks(c->NFp, (int)NM, ns))
+                                   : pick_split(c, V, vb);
         const size_t split_stride = (size_t)V * c->NFp;
         SX_HIP(c, c->amp.ensure(sizeof(float2) * split_stride * split));
-        sx::AmpArgs aa;
-        aa.coords = c->coords.as<float>();
-        aa.NAp = c->NAp;
-        aa.NFp = c->NFp;
-        aa.NF = NF;
-        aa.qrev = c->qrev.as<double>();
-        aa.b = bptr;
-        aa.brow = browp;
-        aa.V = V;
-        aa.vb = vb;
-        aa.split = split;
-        aa.amp = c->amp.as<float2>();
-        SX_HIP(c, sx::launch_amp_all(aa, c->stream));
+        if (oriented) {
+            sx::OrientArgs oa;
+            oa.coords = c->coords.as<float>();
+            oa.NAp = c->NAp;
+            oa.NFp = c->NFp;
+            oa.qlen = c->qlen.as<double>();
+            oa.u = c->uvec.as<double>();
+            oa.b = bptr;
+            oa.NM = (int)NM;
+            oa.ns = ns;
+            oa.split = split;
+            oa.per_shell_b = b_shell ? 1 : 0;
+            oa.amp = c->amp.as<float2>();
+            SX_HIP(c, sx::launch_amp_oriented(oa, c->stream));
+        } else {
+            sx::AmpArgs aa;
+            aa.coords = c->coords.as<float>();
+            aa.NAp = c->NAp;
+            aa.NFp = c->NFp;
+            aa.NF = NF;
+            aa.qrev = c->qrev.as<double>();
+            aa.b = bptr;
+            aa.brow = browp;
+            aa.V = V;
+            aa.vb = vb;
+            aa.split = split;
+            aa.amp = c->amp.as<float2>();
+            SX_HIP(c, sx::launch_amp_all(aa, c->stream));
+        }
         c->timing.amplitude_launches++;
         if (prof) SX_HIP(c, hipEventRecord(c->ev[1], c->stream));
         int chunks = (int)std::min<size_t>(NM, std::max<size_t>(1, (1024 + ns - 1) / ns));
@@ -334,7 +376,7 @@ void sx_destroy(sx_ctx *c) {
     if (!c) return;
     (void)hipSetDevice(c->cfg.device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
-    for (DevBuf *d : {&c->coords, &c->b, &c->twL, &c->twL64, &c->tw64, &c->w, &c->qrev, &c->brow, &c->bshell, &c->amp,
+    for (DevBuf *d : {&c->coords, &c->b, &c->twL, &c->twL64, &c->tw64, &c->w, &c->qrev, &c->qlen, &c->uvec, &c->brow, &c->bshell, &c->amp,
                       &c->ppart, &c->apart, &c->fqt, &c->fq, &c->fq2, &c->stage})
         d->release();
     for (auto &e : c->ev)
@@ -411,20 +453,31 @@ int sx_set_factors(sx_ctx *c, const double *b, size_t NA) {
     return SX_OK;
 }
 
-int sx_compute_shells(sx_ctx *c, size_t nshells, const double *qv, size_t NM, const double *b_shell, double *fqt,
-                      double *fq, double *fq2) {
+}  // extern "C"
+
+namespace {
+
+int compute_impl(sx_ctx *c, size_t nshells, const double *qv, const double *qlen, const double *u, size_t NM,
+                 const double *b_shell, double *fqt, double *fq, double *fq2) {
     if (!c) return SX_ERR_ARG;
     if (!c->staged) return fail(c, SX_ERR_STATE, "compute: no staged coordinates");
     if (!b_shell && !c->have_b) return fail(c, SX_ERR_STATE, "compute: scattering factors not set");
-    if (nshells == 0 || NM == 0 || !qv || !fqt || !fq || !fq2)
+    if (nshells == 0 || NM == 0 || (!qv && (!qlen || !u)) || !fqt || !fq || !fq2)
         return fail(c, SX_ERR_ARG, "compute: empty shell list or null output");
     if (NM > (size_t)1 << 20 || nshells > (size_t)1 << 20)
         return fail(c, SX_ERR_ARG, "compute: too many vectors (NM=%zu, shells=%zu)", NM, nshells);
     const bool needs_fft = c->cfg.dsp_type == SX_DSP_AUTOCORRELATE;
     if (needs_fft && !c->plan_ok)
         return fail(c, SX_ERR_UNSUPPORTED, "compute: NF=%zu exceeds the supported correlation length", c->NF);
-    for (size_t i = 0; i < 3 * nshells * NM; ++i)
-        if (!std::isfinite(qv[i])) return fail(c, SX_ERR_ARG, "compute: non-finite q vector component");
+    if (qv) {
+        for (size_t i = 0; i < 3 * nshells * NM; ++i)
+            if (!std::isfinite(qv[i])) return fail(c, SX_ERR_ARG, "compute: non-finite q vector component");
+    } else {
+        for (size_t i = 0; i < nshells; ++i)
+            if (!std::isfinite(qlen[i])) return fail(c, SX_ERR_ARG, "compute: non-finite |q|");
+        for (size_t i = 0; i < 3 * NM; ++i)
+            if (!std::isfinite(u[i])) return fail(c, SX_ERR_ARG, "compute: non-finite orientation component");
+    }
     SX_HIP(c, hipSetDevice(c->cfg.device));
     c->timing = sx_timing{};
     c->timing.pad_length = c->plan_ok ? (size_t)c->plan.L : 0;
@@ -436,11 +489,28 @@ int sx_compute_shells(sx_ctx *c, size_t nshells, const double *qv, size_t NM, co
     }
     for (size_t s0 = 0; s0 < nshells; s0 += per) {
         const int ns = (int)std::min(per, nshells - s0);
-        int rc = run_batch(c, ns, NM, qv + 3 * NM * s0, b_shell ? b_shell + c->NA * s0 : nullptr,
-                           fqt + 2 * c->NF * s0, fq + 2 * s0, fq2 + 2 * s0);
+        int rc = run_batch(c, ns, NM, qv ? qv + 3 * NM * s0 : nullptr, qlen ? qlen + s0 : nullptr, u,
+                           b_shell ? b_shell + c->NA * s0 : nullptr, fqt + 2 * c->NF * s0, fq + 2 * s0,
+                           fq2 + 2 * s0);
         if (rc != SX_OK) return rc;
     }
     return SX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int sx_compute_shells(sx_ctx *c, size_t nshells, const double *qv, size_t NM, const double *b_shell, double *fqt,
+                      double *fq, double *fq2) {
+    if (c && !qv) return fail(c, SX_ERR_ARG, "sx_compute_shells: null q vectors");
+    return compute_impl(c, nshells, qv, nullptr, nullptr, NM, b_shell, fqt, fq, fq2);
+}
+
+int sx_compute_shells_oriented(sx_ctx *c, size_t nshells, const double *qlen, const double *u, size_t NM,
+                               const double *b_shell, double *fqt, double *fq, double *fq2) {
+    if (c && (!qlen || !u)) return fail(c, SX_ERR_ARG, "sx_compute_shells_oriented: null |q| or orientations");
+    return compute_impl(c, nshells, nullptr, qlen, u, NM, b_shell, fqt, fq, fq2);
 }
 
 int sx_compute_shell(sx_ctx *c, const double *qv, size_t NM, double *fqt, double *fq, double *fq2) {

@@ -485,7 +485,7 @@ struct SelfInput {
         double p = qx * (double)X[n];
         p = fma(qy, (double)Y[n], p);
         p = fma(qz, (double)Z[n], p);
-        const float f = (float)(p - __builtin_rint(p));
+        const float f = (float)__builtin_amdgcn_fract(p);
         return make_float2(b * __builtin_amdgcn_cosf(f), b * __builtin_amdgcn_sinf(f));
     }
 };

@@ -61,6 +61,20 @@ struct AmpArgs {
 };
 hipError_t launch_amp_all(const AmpArgs &a, hipStream_t stream);
 
+// Oriented shells: v = s * NM + m has q_v = |q_s| u_m.  NAp must be a multiple
+// of 16 (it is a multiple of 64).
+struct OrientArgs {
+    const float *coords;   // [3][NAp][NFp]
+    size_t NAp, NFp;
+    const double *qlen;    // [ns] |q_s| / 2 pi
+    const double *u;       // [NM][3]
+    const float *b;        // [ns][NAp] if per_shell_b else [NAp]
+    int NM, ns, split, per_shell_b;
+    float2 *amp;           // [split][ns*NM][NFp]
+};
+hipError_t launch_amp_oriented(const OrientArgs &a, hipStream_t stream);
+int oriented_base_blocks(size_t NFp, int NM, int ns);
+
 struct CorrArgs {
     FftPlan plan;
     int NF, NM, nshells, chunks;

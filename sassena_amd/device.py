@@ -119,6 +119,26 @@ class ScatterDevice:
                     "sx_compute_shells")
         return (fqt[..., 0] + 1j * fqt[..., 1], fq[:, 0] + 1j * fq[:, 1], fq2[:, 0] + 1j * fq2[:, 1])
 
+    def compute_oriented(self, qlen, u, b_shell=None):
+        """Shells sharing one orientation set: q_{s,m} = qlen[s] * u[m].
+
+        qlen [nshells], u [NM][3] unit rows -> (fqt [nshells][NF], fq, fq2)."""
+        ql = np.ascontiguousarray(qlen, dtype=np.float64).ravel()
+        uu = np.ascontiguousarray(u, dtype=np.float64).reshape(-1, 3)
+        ns, NM = ql.size, uu.shape[0]
+        fqt = np.zeros((ns, self.NF, 2), np.float64)
+        fq = np.zeros((ns, 2), np.float64)
+        fq2 = np.zeros((ns, 2), np.float64)
+        bp = None
+        if b_shell is not None:
+            bs = np.ascontiguousarray(b_shell, dtype=np.float64)
+            if bs.shape != (ns, self.NA):
+                raise ValueError("b_shell must be [nshells][NA]")
+            bp = _dp(bs)
+        self._check(self._L.sx_compute_shells_oriented(self._h, ns, _dp(ql), _dp(uu), NM, bp, _dp(fqt), _dp(fq),
+                                                       _dp(fq2)), "sx_compute_shells_oriented")
+        return (fqt[..., 0] + 1j * fqt[..., 1], fq[:, 0] + 1j * fq[:, 1], fq2[:, 0] + 1j * fq2[:, 1])
+
     def timing(self):
         t = _lib.sx_timing()
         self._check(self._L.sx_get_timing(self._h, ctypes.byref(t)), "sx_get_timing")

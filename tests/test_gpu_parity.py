@@ -52,8 +52,9 @@ def test_golden(sx, path, tag):
         check_close(fqt[s], fq[s], fq2[s], ref[s], g[f"fq_{tag}"][s], g[f"fq2_{tag}"][s])
 
 
+@pytest.mark.parametrize("oriented", [False, True])
 @pytest.mark.parametrize("dsp", ["autocorrelate", "square", "plain"])
-def test_c1_all_against_oracle(sx, oracle_mod, dsp):
+def test_c1_all_against_oracle(sx, oracle_mod, dsp, oriented):
     """The full CPU-runnable case: 1000 atoms, 200 frames, 10 shells x 50 vectors."""
     from sassena_amd import synth
     T = synth.case("c1")
@@ -62,7 +63,10 @@ def test_c1_all_against_oracle(sx, oracle_mod, dsp):
     with sx.ScatterDevice("all", dsp) as dev:
         dev.stage(T.xyz, layout="soa")
         dev.set_factors(T.b)
-        fqt, fq, fq2 = dev.compute_shells(qv)
+        if oriented:
+            fqt, fq, fq2 = dev.compute_oriented(T.qlen, T.orient)
+        else:
+            fqt, fq, fq2 = dev.compute_shells(qv)
     for s in range(T.nshells):
         r = oracle_mod.compute("all", frames, T.b, qv[s], dsp, nthreads=8)
         check_close(fqt[s], fq[s], fq2[s], *r)
@@ -189,3 +193,36 @@ def test_c2_full_size_subset_against_oracle(sx, oracle_mod):
     assert np.array_equal(r1[0], r2[0])
     ref = oracle_mod.compute("all", frames, T.b, qv, nthreads=8)
     check_close(*r1, *ref)
+
+
+@pytest.mark.parametrize("NM,ns,split", [(1, 1, 0), (7, 3, 0), (9, 17, 2), (50, 9, 5)])
+def test_oriented_ragged_groups(sx, oracle_mod, NM, ns, split):
+    """Orientation and shell counts that do not fill the 8 x 8 vector tiles."""
+    frames, b = small_trajectory(90, 70, seed=NM + ns)
+    u = unit_vectors(NM, ns)
+    qlen = np.linspace(0.2, 3.5, ns)
+    bs = np.stack([b * (1 + 0.1 * s) for s in range(ns)])
+    for per_shell in (False, True):
+        with sx.ScatterDevice("all", atom_split=split) as dev:
+            dev.stage(frames, layout="frames")
+            dev.set_factors(b)
+            fqt, fq, fq2 = dev.compute_oriented(qlen, u, b_shell=bs if per_shell else None)
+        for s in range(ns):
+            bb = bs[s] if per_shell else b
+            check_close(fqt[s], fq[s], fq2[s], *oracle_mod.compute("all", frames, bb, qlen[s] * u))
+
+
+def test_c2_full_size_oriented_subset(sx, oracle_mod):
+    """Full-size headline trajectory, oriented entry point, three shells x two
+    orientations, against the oracle."""
+    from sassena_amd import synth
+    T = synth.case("c2")
+    frames = synth.to_frames_aos(T.xyz)
+    qlen = T.qlen[[0, 20, 39]]
+    u = T.orient[:2]
+    with sx.ScatterDevice("all") as dev:
+        dev.stage(T.xyz, layout="soa")
+        dev.set_factors(T.b)
+        fqt, fq, fq2 = dev.compute_oriented(qlen, u)
+    for s in range(3):
+        check_close(fqt[s], fq[s], fq2[s], *oracle_mod.compute("all", frames, T.b, qlen[s] * u, nthreads=8))
