@@ -29,8 +29,46 @@ HIPCC_FLAGS = [
 ]
 
 
+HOST_LIB = os.path.join(PKG, "libsassena_host.so")
+SX_RUN = os.path.join(PKG, "bin", "sx_run")
+CONDA = os.environ.get("SX_HDF5_PREFIX", "/opt/conda")
+
+
 def sources():
     return sorted(glob.glob(os.path.join(PKG, "csrc", "*.hip")))
+
+
+def host_sources():
+    return sorted(p for p in glob.glob(os.path.join(PKG, "host", "*.cpp")) if not p.endswith("sx_run.cpp"))
+
+
+def _run(cmd, verbose):
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"{cmd[0]} failed ({res.returncode}):\n{res.stderr}")
+
+
+def build_host(verbose: bool = False) -> str:
+    """The C++ host layer (device classes, config, I/O) and the sx_run driver,
+    linked against libsassena_amd.so and HDF5 (+HL)."""
+    cxx = os.environ.get("CXX", "g++")
+    inc = ["-I", os.path.join(ROOT, "include"), "-I", os.path.join(PKG, "host"), "-I", os.path.join(CONDA, "include")]
+    common = ["-O2", "-std=c++17", "-Wall", "-Wextra", "-fPIC"]
+    # HDF5 by full path: a -L on the HDF5 prefix would let the linker pick
+    # that prefix's older C++ runtime instead of the system one ROCm needs.
+    h5 = [os.path.join(CONDA, "lib", "libhdf5_hl.so"), os.path.join(CONDA, "lib", "libhdf5.so")]
+    libs = ["-L", PKG, "-lsassena_amd", *h5, "-Wl,-rpath,$ORIGIN", f"-Wl,-rpath,{os.path.join(CONDA, 'lib')}"]
+    tmp = HOST_LIB + ".tmp"
+    _run([cxx, *common, "-shared", *inc, "-o", tmp, *host_sources(), *libs], verbose)
+    os.replace(tmp, HOST_LIB)
+    os.makedirs(os.path.dirname(SX_RUN), exist_ok=True)
+    tmp = SX_RUN + ".tmp"
+    _run([cxx, *common, *inc, "-o", tmp, os.path.join(PKG, "host", "sx_run.cpp"), "-L", PKG, "-lsassena_host",
+          "-lsassena_amd", "-Wl,-rpath,$ORIGIN/..", f"-Wl,-rpath-link,{os.path.join(CONDA, 'lib')}"], verbose)
+    os.replace(tmp, SX_RUN)
+    return SX_RUN
 
 
 def build(verbose: bool = False) -> str:
@@ -47,6 +85,7 @@ def build(verbose: bool = False) -> str:
     if verbose and errs.strip():
         print(errs, file=sys.stderr)
     os.replace(tmp, LIB)
+    build_host(verbose)
     return LIB
 
 

@@ -277,6 +277,31 @@ int run_batch(sx_ctx *c, int ns, size_t NM, const double *qv, const double *qlen
             fa.ppart = nullptr;
         }
         c->timing.correlate_launches++;
+    } else if (dsp != SX_DSP_AUTOCORRELATE) {
+        if (prof) SX_HIP(c, hipEventRecord(c->ev[1], c->stream));
+        const long pairs = (long)c->NA * (long)NM;
+        int chunks = (int)std::min<long>(pairs, std::max<long>(1, (1024 + ns - 1) / ns));
+        SX_HIP(c, c->apart.ensure(sizeof(double) * 4 * (size_t)ns * chunks));
+        SX_HIP(c, c->ppart.ensure(sizeof(double2) * (size_t)ns * chunks * NF));
+        sx::CorrArgs ca{};
+        ca.NF = NF;
+        ca.NM = (int)NM;
+        ca.nshells = ns;
+        ca.chunks = chunks;
+        ca.apart = c->apart.as<double>();
+        ca.coords = c->coords.as<float>();
+        ca.NAp = c->NAp;
+        ca.NFp = c->NFp;
+        ca.NA = (int)c->NA;
+        ca.qrev = c->qrev.as<double>();
+        ca.b = bptr;
+        ca.brow_per_shell = b_shell ? 1 : 0;
+        SX_HIP(c, sx::launch_series_self(dsp == SX_DSP_SQUARE ? 1 : 2, ca, c->ppart.as<double2>(),
+                                         c->fqt.as<double>(), c->stream));
+        c->timing.correlate_launches++;
+        fa.chunks = chunks;
+        fa.apart = c->apart.as<double>();
+        fa.ppart = nullptr;
     } else {
         if (prof) SX_HIP(c, hipEventRecord(c->ev[1], c->stream));
         const long pairs = (long)c->NA * (long)NM;
@@ -351,8 +376,6 @@ int sx_create(const sx_config *cfg, sx_ctx **out) {
     if (cfg->scattering_type != SX_SCATTERING_ALL && cfg->scattering_type != SX_SCATTERING_SELF) return SX_ERR_ARG;
     if (cfg->dsp_type < SX_DSP_AUTOCORRELATE || cfg->dsp_type > SX_DSP_PLAIN) return SX_ERR_ARG;
     if (cfg->dsp_method != SX_METHOD_FFT && cfg->dsp_method != SX_METHOD_DIRECT) return SX_ERR_ARG;
-    if (cfg->scattering_type == SX_SCATTERING_SELF && cfg->dsp_type != SX_DSP_AUTOCORRELATE)
-        return SX_ERR_UNSUPPORTED;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || cfg->device < 0 || cfg->device >= ndev) return SX_ERR_HIP;
     if (hipSetDevice(cfg->device) != hipSuccess) return SX_ERR_HIP;
@@ -469,6 +492,8 @@ int compute_impl(sx_ctx *c, size_t nshells, const double *qv, const double *qlen
     const bool needs_fft = c->cfg.dsp_type == SX_DSP_AUTOCORRELATE;
     if (needs_fft && !c->plan_ok)
         return fail(c, SX_ERR_UNSUPPORTED, "compute: NF=%zu exceeds the supported correlation length", c->NF);
+    if (!needs_fft && c->cfg.scattering_type == SX_SCATTERING_SELF && c->NF > (size_t)sx::kMaxSeriesFrames)
+        return fail(c, SX_ERR_UNSUPPORTED, "compute: self square/plain supports NF <= %d", sx::kMaxSeriesFrames);
     if (qv) {
         for (size_t i = 0; i < 3 * nshells * NM; ++i)
             if (!std::isfinite(qv[i])) return fail(c, SX_ERR_ARG, "compute: non-finite q vector component");

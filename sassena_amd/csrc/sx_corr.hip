@@ -773,4 +773,99 @@ hipError_t launch_series_all(int mode, const float2 *amp, size_t NFp, int split,
     return hipGetLastError();
 }
 
+
+// ------------------------------------------------ self square / plain -----
+
+// "self" with dsp.type square or plain (self_vectors_scatter_device.cpp:115-135):
+// per pair the series S(t) = |b e^{i phi}|^2 or b e^{i phi}; one block per
+// (shell, chunk of pairs) keeps its partial sum over pairs of S(t) in LDS
+// (each frame owned by one thread) and the per-pair a = mean_t S.
+__global__ __launch_bounds__(256) void self_series_kernel(int mode, CorrArgs a, double2 *__restrict__ rows) {
+    extern __shared__ double2 row[];
+    __shared__ double2 red[4];
+    const int tid = threadIdx.x;
+    const int s = blockIdx.x / a.chunks, c = blockIdx.x % a.chunks;
+    const long npairs = (long)a.NA * a.NM;
+    const long p0 = npairs * c / a.chunks, p1 = npairs * (c + 1) / a.chunks;
+    for (int t = tid; t < a.NF; t += 256) row[t] = make_double2(0.0, 0.0);
+    double sa_r = 0.0, sa_i = 0.0, sa2 = 0.0;
+    const size_t plane = a.NAp * a.NFp;
+    for (long pr = p0; pr < p1; ++pr) {
+        const int j = (int)(pr / a.NM), m = (int)(pr % a.NM);
+        const double *q = a.qrev + ((size_t)s * a.NM + m) * 3;
+        SelfInput in;
+        in.X = a.coords + (size_t)j * a.NFp;
+        in.Y = in.X + plane;
+        in.Z = in.X + 2 * plane;
+        in.qx = q[0];
+        in.qy = q[1];
+        in.qz = q[2];
+        in.b = a.b[(size_t)(a.brow_per_shell ? s : 0) * a.NAp + j];
+        double ar = 0.0, ai = 0.0;
+        for (int t = tid; t < a.NF; t += 256) {
+            const float2 x = in(t);
+            double sr, si;
+            if (mode == 1) {
+                sr = (double)x.x * x.x + (double)x.y * x.y;
+                si = 0.0;
+            } else {
+                sr = x.x;
+                si = x.y;
+            }
+            row[t].x += sr;
+            row[t].y += si;
+            ar += sr;
+            ai += si;
+        }
+        const double2 r = block_sum2(ar, ai, red, tid, 256);
+        if (tid == 0) {
+            const double xr = r.x / a.NF, xi = r.y / a.NF;
+            sa_r += xr;
+            sa_i += xi;
+            sa2 += xr * xr + xi * xi;
+        }
+    }
+    double2 *dst = rows + ((size_t)s * a.chunks + c) * a.NF;
+    for (int t = tid; t < a.NF; t += 256) dst[t] = row[t];
+    if (tid == 0) {
+        double *ap = a.apart + ((size_t)s * a.chunks + c) * 4;
+        ap[0] = sa_r;
+        ap[1] = sa_i;
+        ap[2] = sa2;
+        ap[3] = 0.0;
+    }
+}
+
+// fqt(t) = (1/NM) sum over chunks of the partial series sums
+__global__ __launch_bounds__(256) void series_rows_final_kernel(int NF, int NM, int chunks,
+                                                                const double2 *__restrict__ rows,
+                                                                double *__restrict__ fqt) {
+    const int s = blockIdx.y;
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    if (t >= NF) return;
+    double re = 0.0, im = 0.0;
+    for (int c = 0; c < chunks; ++c) {
+        const double2 v = rows[((size_t)s * chunks + c) * NF + t];
+        re += v.x;
+        im += v.y;
+    }
+    const double f = 1.0 / (double)NM;
+    fqt[((size_t)s * NF + t) * 2] = re * f;
+    fqt[((size_t)s * NF + t) * 2 + 1] = im * f;
+}
+
+hipError_t launch_series_self(int mode, const CorrArgs &a, double2 *rows, double *fqt, hipStream_t stream) {
+    const size_t lds = sizeof(double2) * (size_t)a.NF;
+    hipError_t e =
+        hipFuncSetAttribute((const void *)self_series_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(self_series_kernel, dim3((unsigned)(a.nshells * a.chunks)), dim3(256), lds, stream, mode, a,
+                       rows);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(series_rows_final_kernel, dim3((unsigned)((a.NF + 255) / 256), (unsigned)a.nshells), dim3(256),
+                       0, stream, a.NF, a.NM, a.chunks, rows, fqt);
+    return hipGetLastError();
+}
+
 }  // namespace sx

@@ -109,4 +109,8 @@ hipError_t launch_series_all(int mode, const float2 *amp, size_t NFp, int split,
                              size_t split_stride, int NF, int NM, int nshells, int chunks,
                              double *fqt, double *apart, hipStream_t stream);
 
+// "self" with dsp.type square (mode 1) / plain (mode 2); NF <= kMaxSeriesFrames
+constexpr int kMaxSeriesFrames = 8192;
+hipError_t launch_series_self(int mode, const CorrArgs &a, double2 *rows, double *fqt, hipStream_t stream);
+
 }  // namespace sx

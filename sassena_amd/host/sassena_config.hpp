@@ -1,0 +1,49 @@
+// sassena_config.hpp -- the scatter.xml subset of the hot path.
+//
+// A small XML element tree (elements, text and attributes; comments,
+// processing instructions and CDATA are skipped) and the reader of the keys
+// Params::read_xml takes for scattering, stager and sample input
+// (parameters.cpp:345-606).  Relative file names resolve against the
+// directory of the configuration file (parameters.cpp:903-908).
+#pragma once
+
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "sassena_host.hpp"
+
+namespace sassena_amd {
+
+struct XMLElement {
+    std::string name;
+    std::string text;
+    std::vector<std::pair<std::string, std::string>> attributes;
+    std::vector<std::unique_ptr<XMLElement>> children;
+    // first descendant on a '/'-separated relative path, or nullptr
+    const XMLElement *find(const std::string &path) const;
+    std::vector<const XMLElement *> find_all(const std::string &path) const;
+    bool exists(const std::string &path) const { return find(path) != nullptr; }
+    std::string get(const std::string &path, const std::string &def) const;
+    double get(const std::string &path, double def) const;
+    long get(const std::string &path, long def) const;
+    bool get(const std::string &path, bool def) const;
+};
+
+std::unique_ptr<XMLElement> parse_xml(const std::string &text);
+std::unique_ptr<XMLElement> parse_xml_file(const std::string &path);
+
+// Input and output files named in the configuration.
+struct RunFiles {
+    std::string structure_file, structure_format = "elements";  // sample.structure
+    std::string frames_file, frames_format = "raw";             // sample.framesets.frameset
+    std::string signal_file = "signal.h5";                      // scattering.signal.file
+    bool fqt = true, fq0 = true, fq = true, fq2 = true;
+};
+
+// Reads the configuration (root element <root>); fills params (generating the
+// q vectors and orientation vectors as the reference does) and files.
+void read_config(const std::string &path, Params &params, RunFiles &files);
+void read_config_text(const std::string &text, const std::string &basedir, Params &params, RunFiles &files);
+
+}  // namespace sassena_amd

@@ -1,0 +1,127 @@
+// sx_run.cpp -- command-line driver of the hot path (the role of the
+// reference's sassena main, sassena.cpp:132-417, for the scattering-device
+// path only): read scatter.xml, load the target's structure and trajectory,
+// build the device through ScatterDeviceFactory, run it and write the signal.
+//
+//   sx_run --config scatter.xml            run and write the signal file
+//   sx_run --config scatter.xml --vectors  print q vectors and orientations
+//   sx_run --h5-to-raw signal.h5 out.raw   convert a signal file
+//   sx_run --convert-trajectory in fmt out.raw   re-read a trajectory
+//   sx_run --write-test-signal file NF nq  exercise a signal writer
+//   sx_run --structure file.pdb            print the elements of a PDB
+//
+// Structure formats: "elements" (one element symbol per line, the target
+// atoms in order).  Trajectory formats: "raw" (SXRAW1 header, uint64 NA, NF,
+// float32 frames [NF][NA][3]) and "dcd".  Signal files: ".h5" (the
+// reference's HDF5 layout) or anything else for the raw SXSIG1 layout.
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <iostream>
+#include <string>
+#include <vector>
+
+#include "sassena_config.hpp"
+#include "sassena_host.hpp"
+#include "sassena_io.hpp"
+
+using namespace sassena_amd;
+
+static void print_vectors(const Params &p) {
+    std::printf("qvectors %zu\n", p.scattering.qvectors.size());
+    for (const auto &q : p.scattering.qvectors) std::printf("%.17g %.17g %.17g\n", q.x, q.y, q.z);
+    std::printf("orientations %zu\n", p.scattering.vectors.vectors.size());
+    for (const auto &u : p.scattering.vectors.vectors) std::printf("%.17g %.17g %.17g\n", u.x, u.y, u.z);
+}
+
+int main(int argc, char **argv) {
+    std::string config = "scatter.xml";
+    bool vectors_only = false;
+    try {
+        if (argc == 4 && !std::strcmp(argv[1], "--h5-to-raw")) {
+            convert_signal_h5_to_raw(argv[2], argv[3]);
+            return 0;
+        }
+        if (argc == 5 && !std::strcmp(argv[1], "--convert-trajectory")) {
+            // trajectory (any readable format) -> raw, through the readers
+            std::unique_ptr<ICoordinateSource> src = open_trajectory(argv[2], argv[3]);
+            std::vector<float> all(src->atoms() * src->frames() * 3);
+            for (size_t f = 0; f < src->frames(); ++f) src->read_frame(f, all.data() + f * src->atoms() * 3);
+            RawTrajectory::write(argv[4], src->atoms(), src->frames(), all.data());
+            return 0;
+        }
+        if (argc == 5 && !std::strcmp(argv[1], "--write-test-signal")) {
+            // writes nq deterministic shells of NF frames through the signal writer
+            RunFiles files;
+            files.signal_file = argv[2];
+            const size_t NF = std::stoul(argv[3]), nq = std::stoul(argv[4]);
+            std::unique_ptr<IResumableWriter> w = open_signal_writer(files, NF, "");
+            std::vector<CartesianCoor3D> qs;
+            for (size_t i = 0; i < nq; ++i) qs.emplace_back(0.1 * (i + 1), 0.0, 0.5);
+            for (const auto &q : remaining_qvectors(*w, qs)) {
+                std::vector<std::complex<double>> fqt(NF);
+                for (size_t t = 0; t < NF; ++t) fqt[t] = {q.x * 1000 + t, -(double)t};
+                w->write(q, fqt, {q.x, 1.0}, {q.x * q.x, 0.0});
+            }
+            w->flush();
+            return 0;
+        }
+        if (argc == 3 && !std::strcmp(argv[1], "--structure")) {
+            for (const auto &e : read_structure(argv[2], "pdb")) std::printf("%s\n", e.c_str());
+            return 0;
+        }
+    } catch (const std::exception &e) {
+        std::fprintf(stderr, "sx_run: %s\n", e.what());
+        return 1;
+    }
+    for (int i = 1; i < argc; ++i) {
+        if (!std::strcmp(argv[i], "--config") && i + 1 < argc) config = argv[++i];
+        else if (!std::strcmp(argv[i], "--vectors")) vectors_only = true;
+        else {
+            std::fprintf(stderr, "usage: %s --config scatter.xml [--vectors]\n", argv[0]);
+            return 2;
+        }
+    }
+    try {
+        Params params;
+        RunFiles files;
+        read_config(config, params, files);
+        if (vectors_only) {
+            print_vectors(params);
+            return 0;
+        }
+        if (params.scattering.qvectors.empty()) throw terminate_request("No qvectors generated");
+        std::vector<std::string> elements = read_structure(files.structure_file, files.structure_format);
+        std::unique_ptr<ICoordinateSource> src = open_trajectory(files.frames_file, files.frames_format);
+        if (src->atoms() != elements.size())
+            throw terminate_request("structure has " + std::to_string(elements.size()) + " atoms, trajectory " +
+                                    std::to_string(src->atoms()));
+        Database db = Database::neutron_coherent();
+        std::vector<size_t> ids;
+        for (const auto &e : elements) ids.push_back(db.id(e));
+        ScatterFactors sf(db, ids, params.scattering.background_factor);
+
+        // resume: q vectors already in the signal file are skipped (sassena.cpp:274-305)
+        std::unique_ptr<ISignalWriter> writer = open_signal_writer(files, src->frames(), config);
+        std::vector<CartesianCoor3D> todo = remaining_qvectors(*writer, params.scattering.qvectors);
+        std::fprintf(stderr, "sx_run: %zu atoms, %zu frames, %zu of %zu q vectors to compute\n", src->atoms(),
+                     src->frames(), todo.size(), params.scattering.qvectors.size());
+        if (todo.empty()) return 0;
+        std::unique_ptr<IScatterDevice> dev = ScatterDeviceFactory::create(params, *src, sf, *writer, todo);
+        dev->run();
+        for (const auto &kv : dev->getTimer())
+            std::fprintf(stderr, "timer %-12s count %zu total %.4f s\n", kv.first.c_str(), kv.second.count,
+                         kv.second.sum);
+        return 0;
+    } catch (const device_error &e) {
+        std::fprintf(stderr, "sx_run: device error %d: %s\n", e.code, e.what());
+        return 3;
+    } catch (const terminate_request &e) {
+        std::fprintf(stderr, "sx_run: terminate request: %s\n", e.what());
+        return 1;
+    } catch (const std::exception &e) {
+        std::fprintf(stderr, "sx_run: %s\n", e.what());
+        return 1;
+    }
+}

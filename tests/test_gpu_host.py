@@ -1,0 +1,97 @@
+"""End-to-end runs of the C++ host layer on the GPU: scatter.xml -> DCD ->
+ScatterDeviceFactory -> device -> HDF5 signal file, checked shell by shell
+against the FP64 oracle, including the resume of a partly written file."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+sys.path.insert(0, os.path.dirname(__file__))
+from hostio import read_raw_signal, sx_run, write_dcd  # noqa: E402
+from util import shell_error  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+B = {"H": -3.7390, "C": 6.6460, "N": 9.36, "O": 5.803}
+
+
+def setup_case(tmp_path, NA=150, NF=120, NM=16, seed=5):
+    rng = np.random.default_rng(seed)
+    frames = (rng.uniform(0, 25, size=(1, NA, 3)) + np.cumsum(rng.normal(0, 0.15, size=(NF, NA, 3)), axis=0))
+    frames = frames.astype(np.float32)
+    elements = rng.choice(list(B), size=NA)
+    write_dcd(tmp_path / "traj.dcd", frames, unit_cell=True)
+    (tmp_path / "atoms.txt").write_text("\n".join(elements) + "\n")
+    u = rng.normal(size=(NM, 3))
+    (tmp_path / "orient.txt").write_text("\n".join(" ".join(f"{v:.17g}" for v in r) for r in u))
+    b = np.array([B[e] for e in elements])
+    return frames, b, u / np.linalg.norm(u, axis=1)[:, None]
+
+
+def config(kind, qfile, orientation="vectors", extra=""):
+    orient = {"vectors": "<type>vectors</type><vectors><type>file</type><file>orient.txt</file></vectors>",
+              "none": "<type>none</type>"}[orientation]
+    return f"""<?xml version="1.0"?>
+<root>
+  <sample>
+    <structure><file>atoms.txt</file><format>elements</format></structure>
+    <framesets><frameset><file>traj.dcd</file><format>dcd</format></frameset></framesets>
+  </sample>
+  <scattering>
+    <type>{kind}</type>
+    <vectors><type>file</type><file>{qfile}</file></vectors>
+    <average><orientation>{orient}</orientation></average>
+    <dsp><type>autocorrelate</type><method>fftw</method></dsp>
+    <signal><file>signal.h5</file></signal>
+    {extra}
+  </scattering>
+</root>"""
+
+
+def read_signal(tmp_path):
+    sx_run("--h5-to-raw", str(tmp_path / "signal.h5"), str(tmp_path / "signal.raw"))
+    return read_raw_signal(tmp_path / "signal.raw")
+
+
+def check_shells(oracle_mod, kind, frames, b, u, q, fqt, fq, fq2):
+    coords = frames if kind == "all" else np.ascontiguousarray(frames.transpose(1, 0, 2))
+    for i in range(q.shape[0]):
+        qv = oracle_mod.init_subvectors(q[i], u)
+        r = oracle_mod.compute(kind, coords, b, qv, nthreads=8)
+        assert shell_error(fqt[i], r[0])[0] < 1e-5
+        assert abs(fq[i] - r[1]) < 1e-5 * abs(r[0][0])
+        assert abs(fq2[i] - r[2]) < 1e-5 * (abs(r[2]) + 1e-3 * abs(r[0][0]) ** 2)
+
+
+@pytest.mark.parametrize("kind", ["all", "self"])
+def test_sx_run_dcd_to_hdf5_with_resume(tmp_path, oracle_mod, kind):
+    frames, b, u = setup_case(tmp_path)
+    qs = np.array([[0.4, 0.0, 0.0], [0.0, 1.1, 0.3], [2.2, -0.5, 0.7], [0.0, 0.0, 3.0], [1.0, 1.0, 1.0]])
+    (tmp_path / "q3.txt").write_text("\n".join(" ".join(f"{v:.17g}" for v in r) for r in qs[:3]))
+    (tmp_path / "q5.txt").write_text("\n".join(" ".join(f"{v:.17g}" for v in r) for r in qs))
+    (tmp_path / "a.xml").write_text(config(kind, "q3.txt"))
+    (tmp_path / "b.xml").write_text(config(kind, "q5.txt"))
+    sx_run("--config", str(tmp_path / "a.xml"), cwd=tmp_path)
+    q, fqt, fq, fq2 = read_signal(tmp_path)
+    assert np.array_equal(q, qs[:3])
+    res = sx_run("--config", str(tmp_path / "b.xml"), cwd=tmp_path)
+    assert "2 of 5 q vectors" in res.stderr
+    q, fqt, fq, fq2 = read_signal(tmp_path)
+    assert np.array_equal(q, qs)
+    check_shells(oracle_mod, kind, frames, b, u, q, fqt, fq, fq2)
+    res = sx_run("--config", str(tmp_path / "b.xml"), cwd=tmp_path)
+    assert "0 of 5 q vectors" in res.stderr
+
+
+def test_sx_run_scans_without_orientations(tmp_path, oracle_mod):
+    frames, b, u = setup_case(tmp_path, NA=80, NF=90)
+    xml = config("all", "unused", orientation="none").replace(
+        "<vectors><type>file</type><file>unused</file></vectors>",
+        "<vectors><type>scans</type><scans><scan><from>0.2</from><to>2.5</to><points>7</points>"
+        "<base><x>0.6</x><y>0.8</y><z>0</z></base></scan></scans></vectors>")
+    (tmp_path / "s.xml").write_text(xml)
+    sx_run("--config", str(tmp_path / "s.xml"), cwd=tmp_path)
+    q, fqt, fq, fq2 = read_signal(tmp_path)
+    assert q.shape == (7, 3)
+    check_shells(oracle_mod, "all", frames, b, None, q, fqt, fq, fq2)

@@ -40,8 +40,6 @@ def test_golden(sx, path, tag):
     g = np.load(path)
     kind = str(g["kind"])
     dsp, method = tag.rsplit("_", 1)
-    if kind == "self" and dsp != "autocorrelate":
-        pytest.skip("self scattering supports dsp.type=autocorrelate only in this build")
     qv = g["qlen"][:, None, None] * g["u"][None]
     with sx.ScatterDevice(kind, dsp, method) as dev:
         dev.stage(g["frames"], layout="frames")
@@ -72,18 +70,19 @@ def test_c1_all_against_oracle(sx, oracle_mod, dsp, oriented):
         check_close(fqt[s], fq[s], fq2[s], *r)
 
 
-def test_self_against_oracle(sx, oracle_mod):
+@pytest.mark.parametrize("dsp", ["autocorrelate", "square", "plain"])
+def test_self_against_oracle(sx, oracle_mod, dsp):
     frames, b = small_trajectory(120, 300, seed=7, box=30.0, step=0.2)
     atoms = np.ascontiguousarray(frames.transpose(1, 0, 2))
     u = unit_vectors(12, 8)
     qlen = np.array([0.3, 1.1, 2.9])
     qv = qlen[:, None, None] * u[None]
-    with sx.ScatterDevice("self") as dev:
+    with sx.ScatterDevice("self", dsp) as dev:
         dev.stage(atoms, layout="atoms")
         dev.set_factors(b)
         fqt, fq, fq2 = dev.compute_shells(qv)
     for s in range(len(qlen)):
-        r = oracle_mod.compute("self", atoms, b, qv[s], nthreads=8)
+        r = oracle_mod.compute("self", atoms, b, qv[s], dsp, nthreads=8)
         check_close(fqt[s], fq[s], fq2[s], *r)
 
 

@@ -1,0 +1,135 @@
+"""CPU tests of the C++ host layer (sassena_amd/host): configuration reading,
+the q-vector and orientation generators against the oracle, trajectory
+readers, signal files and resume bookkeeping.  No GPU call is made."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+sys.path.insert(0, os.path.dirname(__file__))
+from hostio import (read_raw_signal, read_raw_trajectory, sx_run, write_dcd,  # noqa: E402
+                    write_raw_trajectory)
+
+
+def vectors(tmp_path, xml):
+    cfg = tmp_path / "scatter.xml"
+    cfg.write_text(xml)
+    out = sx_run("--config", str(cfg), "--vectors").stdout.split("\n")
+    nq = int(out[0].split()[1])
+    q = np.array([[float(v) for v in l.split()] for l in out[1:1 + nq]]).reshape(-1, 3)
+    no = int(out[1 + nq].split()[1])
+    u = np.array([[float(v) for v in l.split()] for l in out[2 + nq:2 + nq + no]]).reshape(-1, 3)
+    return q, u
+
+
+def test_scans_match_oracle(tmp_path, oracle_mod):
+    q, _ = vectors(tmp_path, """<?xml version="1.0"?>
+<root><!-- comment --><scattering><vectors><type>scans</type><scans>
+  <scan><from>0.1</from><to>2.0</to><points>10</points><base><x>0</x><y>0</y><z>1</z></base></scan>
+</scans></vectors></scattering></root>""")
+    ref = oracle_mod.scan(0.1, 2.0, 10, base=(0, 0, 1))
+    assert np.array_equal(q, ref)
+
+
+def test_two_scans_outer_sum_and_exponent(tmp_path, oracle_mod):
+    q, _ = vectors(tmp_path, """<root><scattering><vectors><type>scans</type><scans>
+  <scan><from>0</from><to>1</to><points>4</points><exponent>2.0</exponent></scan>
+  <scan><from>0.5</from><to>1.5</to><points>3</points><base><x>0</x><y>1</y><z>0</z></base></scan>
+</scans></vectors></scattering></root>""")
+    a = oracle_mod.scan(0, 1, 4, 2.0)
+    b = oracle_mod.scan(0.5, 1.5, 3, 1.0, base=(0, 1, 0))
+    ref = np.array([x + y for x in a for y in b])
+    assert np.array_equal(q, ref)
+
+
+def test_single_and_file_vectors(tmp_path):
+    q, _ = vectors(tmp_path, "<root><scattering><vectors><single><x>0.3</x><y>0.1</y><z>-2</z></single>"
+                             "</vectors></scattering></root>")
+    assert np.array_equal(q, [[0.3, 0.1, -2.0]])
+    (tmp_path / "q.txt").write_text("1 2 3\n4 5 6\n")
+    q, _ = vectors(tmp_path, "<root><scattering><vectors><type>file</type><file>q.txt</file></vectors>"
+                             "</scattering></root>")
+    assert np.array_equal(q, [[1, 2, 3], [4, 5, 6]])
+
+
+def test_orientation_file_rows_normalised(tmp_path, oracle_mod):
+    rows = [[2.0, 0, 0], [0, 0, 0], [1.0, 2.0, 2.0], [-0.3, 0.4, 0.0]]
+    (tmp_path / "o.txt").write_text("\n".join(" ".join(map(str, r)) for r in rows))
+    _, u = vectors(tmp_path, """<root><scattering><vectors><single/></vectors><average><orientation>
+  <type>vectors</type><vectors><type>file</type><file>o.txt</file></vectors></orientation></average>
+</scattering></root>""")
+    assert np.allclose(u, oracle_mod.normalise_rows(rows), rtol=0, atol=1e-15)
+
+
+def test_sphere_and_cylinder_generators(tmp_path):
+    xml = """<root><scattering><vectors><single/></vectors><average><orientation><type>vectors</type>
+  <vectors><type>{t}</type><algorithm>{a}</algorithm><resolution>{r}</resolution><seed>{s}</seed></vectors>
+</orientation></average></scattering></root>"""
+    _, u1 = vectors(tmp_path, xml.format(t="sphere", a="boost_uniform_on_sphere", r=500, s=7))
+    _, u2 = vectors(tmp_path, xml.format(t="sphere", a="boost_uniform_on_sphere", r=500, s=7))
+    _, u3 = vectors(tmp_path, xml.format(t="sphere", a="boost_uniform_on_sphere", r=500, s=8))
+    assert u1.shape == (500, 3) and np.array_equal(u1, u2) and not np.array_equal(u1, u3)
+    assert np.allclose(np.linalg.norm(u1, axis=1), 1.0)
+    assert np.all(np.abs(u1.mean(axis=0)) < 0.12)          # isotropic to sampling error
+    _, c = vectors(tmp_path, xml.format(t="cylinder", a="raster_linear", r=2, s=0))
+    phi = np.arange(0, 2 * np.pi, 2 * np.pi / 720)
+    assert c.shape[0] in (720, 721)
+    assert np.allclose(c[:720, 0], np.cos(phi[:720])) and np.all(c[:, 2] == 0)
+
+
+def test_xml_errors_are_terminate_requests(tmp_path):
+    cfg = tmp_path / "bad.xml"
+    cfg.write_text("<root><scattering><type>all</scattering></root>")
+    res = sx_run("--config", str(cfg), "--vectors", check=False)
+    assert res.returncode == 1 and "mismatched" in res.stderr
+    cfg.write_text("<root><scattering><target>x</target></scattering></root>")
+    res = sx_run("--config", str(cfg), "--vectors", check=False)
+    assert res.returncode == 1 and "obsolete" in res.stderr
+    cfg.write_text("<root><scattering><vectors><type>scans</type><scans/></vectors></scattering></root>")
+    res = sx_run("--config", str(cfg), "--vectors", check=False)
+    assert res.returncode == 1 and "No q vectors" in res.stderr
+
+
+@pytest.mark.parametrize("cell", [False, True])
+def test_dcd_reader(tmp_path, cell):
+    rng = np.random.default_rng(3)
+    frames = rng.normal(size=(7, 11, 3)).astype(np.float32) * 10
+    write_dcd(tmp_path / "t.dcd", frames, unit_cell=cell)
+    sx_run("--convert-trajectory", str(tmp_path / "t.dcd"), "dcd", str(tmp_path / "t.raw"))
+    assert np.array_equal(read_raw_trajectory(tmp_path / "t.raw"), frames)
+
+
+def test_raw_trajectory_roundtrip(tmp_path):
+    frames = np.arange(2 * 5 * 3, dtype=np.float32).reshape(2, 5, 3)
+    write_raw_trajectory(tmp_path / "a.raw", frames)
+    sx_run("--convert-trajectory", str(tmp_path / "a.raw"), "raw", str(tmp_path / "b.raw"))
+    assert np.array_equal(read_raw_trajectory(tmp_path / "b.raw"), frames)
+
+
+@pytest.mark.parametrize("ext", ["h5", "sig"])
+def test_signal_writer_and_resume(tmp_path, ext):
+    f = str(tmp_path / f"signal.{ext}")
+    sx_run("--write-test-signal", f, "6", "3")
+    sx_run("--write-test-signal", f, "6", "5")      # appends only shells 4 and 5
+    raw = f if ext == "sig" else str(tmp_path / "signal.raw")
+    if ext == "h5":
+        sx_run("--h5-to-raw", f, raw)
+    q, fqt, fq, fq2 = read_raw_signal(raw)
+    assert np.allclose(q[:, 0], 0.1 * np.arange(1, 6)) and q.shape == (5, 3)
+    t = np.arange(6)
+    for i in range(5):
+        assert np.array_equal(fqt[i], q[i, 0] * 1000 + t - 1j * t)
+        assert fq[i] == q[i, 0] + 1j and fq2[i] == q[i, 0] ** 2
+    res = sx_run("--write-test-signal", f, "7", "2", check=False)   # wrong NF for this file
+    assert res.returncode == 1
+
+
+def test_pdb_structure(tmp_path):
+    (tmp_path / "s.pdb").write_text(
+        "ATOM      1  N   ALA A   1      11.104   6.134  -6.504  1.00  0.00           N\n"
+        "ATOM      2  CA  ALA A   1      11.639   6.071  -5.147  1.00  0.00           C\n"
+        "HETATM    3  O   HOH W   1       1.000   2.000   3.000  1.00  0.00\n"
+        "ATOM      4 1HB  ALA A   1       1.000   2.000   3.000  1.00  0.00\n")
+    out = sx_run("--structure", str(tmp_path / "s.pdb")).stdout.split()
+    assert out == ["N", "C", "O", "H"]
