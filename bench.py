@@ -27,6 +27,10 @@ sys.path.insert(0, ROOT)
 # MI355X: 256 CUs, 2.4 GHz, 64 lanes/CU; v_sin_f32 / v_cos_f32 issue at a
 # quarter of the full VALU rate.
 TRANS_PEAK = 256 * 64 * 2.4e9 / 4.0   # transcendental ops/s
+# HBM bytes per amplitude launch on this workload, from rocprofv3 --pmc
+# FETCH_SIZE (x2.0) and WRITE_SIZE (x1.0), the factors a 1 GiB device copy
+# reads for the same counters on MI355X.
+AMP_TRAFFIC_BYTES = 2.35e9 + 0.131e9
 
 
 def cpu_model():
@@ -62,9 +66,23 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--vector-block", type=int, default=0)
     ap.add_argument("--atom-split", type=int, default=0)
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run a device-to-device copy of MIB MiB (counter calibration) and exit")
     ap.add_argument("--generic", action="store_true",
                     help="pass the 8000 expanded q vectors (sx_compute_shells) instead of |q| x orientations")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        # a plain copy kernel of known size: the FETCH_SIZE / WRITE_SIZE
+        # counters of the hot-path kernels are scaled by what they read here
+        import torch
+        n = args.calibrate_copy * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibrate_copy_bytes": 4 * n}))
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU only: the hot path shards by q shell with "
                                    "no exchange, see the design notes; run with --gpus 1"}))
@@ -120,7 +138,10 @@ def main():
         "kernel_ms_per_step": {"amplitude": amp_ms / args.steps, "correlate": corr_ms / args.steps,
                                "finalize": fin_ms / args.steps, "device_total": dev_ms / args.steps},
         "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": TRANS_PEAK / 1e12,
-                     "unit": "Tsincos-ops/s", "frac": achieved / TRANS_PEAK, "traffic": None,
+                     "unit": "Tsincos-ops/s", "frac": achieved / TRANS_PEAK,
+                     "traffic": AMP_TRAFFIC_BYTES if not (args.generic or args.vector_block or args.atom_split)
+                     else None,
+                     "traffic_unit": "bytes/launch (rocprofv3 FETCH_SIZE+WRITE_SIZE, copy-calibrated)",
                      "kernel": "amp_all_kernel" if args.generic else "amp_oriented_kernel",
                      "note": "transcendental-issue bound: 2 hardware sin/cos per term at quarter rate"},
     }
