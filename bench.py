@@ -58,6 +58,86 @@ def cpu_baseline(T, threads):
                       f"FP64 libm sin/cos, {threads} threads on {cpu_model()}"}
 
 
+def bench_c4(args):
+    """64 trajectories (atoms log-uniform 500-50,000, frames 256-4,096, box
+    (NA/0.1)^(1/3) A, sigma 0.1 walk, 10-60 shells over 0.1-4.0 A^-1 with
+    50-300 orientations): one captured graph per step; per-trajectory
+    latency from each trajectory run alone through the same kernels."""
+    from sassena_amd import Batch, synth
+    t0 = time.perf_counter()
+    trajs = synth.c4_batch(scale=args.c4_scale)
+    gen_s = time.perf_counter() - t0
+    bt = Batch()
+    bt.stage([(T.xyz, T.b, T.qlen, T.orient) for T in trajs])
+    for _ in range(args.warmup):
+        bt.run()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        bt.run()
+    wall = time.perf_counter() - t0
+    tm = bt.timing()
+    lat = []
+    for k in range(len(trajs)):
+        bt.run_one(k)                       # builds and warms this trajectory's graph
+        lat.append(bt.run_one(k)[1])
+    bt.close()
+    terms = tm["terms"]
+    amp_s = tm["amplitude_ms"] * 1e-3
+    out = {
+        "metric": "atom*frame*q-vector terms/s (64 uneven coherent trajectories, one graph launch)",
+        "value": terms * args.steps / wall,
+        "unit": "terms/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 sincos / f64 phase", "data": "synthetic",
+        "config": {"workload": "64 uneven coherent trajectories (batched graph, persistent amplitude kernel)",
+                   "trajectories": len(trajs), "terms_per_step": terms, "tiles": tm["tiles"],
+                   "tile_shape": tm["tile_shape"], "scale": args.c4_scale, "generation_s": gen_s},
+        "graph_ms": tm["total_ms"],
+        "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
+                       "min": float(np.min(lat)), "max": float(np.max(lat))},
+        "roofline": {"bound": "valu", "achieved": 2.0 * terms / amp_s / 1e12, "peak": TRANS_PEAK / 1e12,
+                     "unit": "Tsincos-ops/s", "frac": 2.0 * terms / amp_s / TRANS_PEAK, "traffic": None,
+                     "kernel": "amp_persistent_kernel"},
+    }
+    print(json.dumps(out), flush=True)
+
+
+def bench_c5(args):
+    """100,000 atoms in a 100 A box, sigma 0.1 walk over 10,000 frames, 60
+    shells over 0.05-5.0 A^-1, 1,000 orientations: the network is trained on
+    frames [0, 2000) (1,000- and 16-orientation estimates there) and corrects
+    the 16-orientation estimate on frames [2000, 10000), judged against the
+    plain 1,000-orientation average on those frames."""
+    from sassena_amd import cv, synth
+    rng = np.random.default_rng(505)
+    NA = max(1000, int(100000 * args.c5_scale))
+    NF = max(500, int(10000 * args.c5_scale))
+    box = (NA / 0.1) ** (1.0 / 3.0)
+    t0 = time.perf_counter()
+    b = synth.scattering_lengths(rng, NA)
+    xyz = synth.random_walk(rng, NA, NF, box, 0.1)
+    orient = synth.orientations(rng, 1000)
+    qlen = synth.linear_shells(0.05, 5.0, 60)
+    gen_s = time.perf_counter() - t0
+    r = cv.run(xyz, b, qlen, orient, ntrain_frames=NF // 5, steps=3000, batch=4096)
+    tm = r["timing_s"]
+    cv_eval = tm["small_eval"] + tm["predict"]
+    cv_total = cv_eval + tm["many_train"] + tm["small_train"] + tm["train"]
+    out = {
+        "metric": "F(q,t) shell error of the learned control variate vs the 1000-orientation average, and wall time",
+        "value": tm["many_eval"] / cv_eval, "unit": "x less wall time on the evaluated frames",
+        "n_gpus": 1, "higher_is_better": True, "data": "synthetic",
+        "config": {"workload": "learned control variate", "NA": NA, "NF": NF, "shells": 60, "orientations": 1000,
+                   "small_orientations": 16, "train_frames": NF // 5, "generation_s": gen_s},
+        "timing_s": tm, "speedup_including_training": tm["many_eval"] / cv_total,
+        "error_small": r["error_small"], "error_cv": r["error_cv"], "bias_cv": r["bias_cv"],
+        "train_loss": r["train_loss"],
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -66,6 +146,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--vector-block", type=int, default=0)
     ap.add_argument("--atom-split", type=int, default=0)
+    ap.add_argument("--workload", default="c2", choices=["c2", "c4", "c5"],
+                    help="c2: the headline coherent case; c4: 64 uneven trajectories in one batched graph; "
+                         "c5: learned control variate against the 1,000-orientation average")
+    ap.add_argument("--c5-scale", type=float, default=1.0, help="shrink the c5 atoms and frames (tests)")
+    ap.add_argument("--c4-scale", type=float, default=1.0, help="shrink the c4 trajectories (tests)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (counter calibration) and exit")
     ap.add_argument("--generic", action="store_true",
@@ -89,6 +174,11 @@ def main():
         sys.exit(2)
 
     from sassena_amd import ScatterDevice, synth
+
+    if args.workload == "c4":
+        return bench_c4(args)
+    if args.workload == "c5":
+        return bench_c5(args)
 
     T = synth.case("c2")
     qv = np.array([T.qvectors(s) for s in range(T.nshells)])

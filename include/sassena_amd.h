@@ -67,7 +67,9 @@ typedef struct sx_config {
     int profile;         /* nonzero: time every kernel with HIP events */
     int vector_block;    /* 0: automatic; else q vectors per wavefront (4, 8 or 16) */
     int atom_split;      /* 0: automatic; else atom ranges per amplitude tile */
-    int reserved[9];
+    int tile_shape;      /* oriented shells: 0 autotuned, 1..4 fixed (2x8, 4x4, 1x8, 1x16
+                            orientations x shells per wavefront) */
+    int reserved[8];
 } sx_config;
 
 /* Per-kernel timing of the last sx_compute_* call (profile != 0). */
@@ -124,6 +126,79 @@ int sx_get_timing(const sx_ctx *ctx, sx_timing *out);
 
 /* FFT length the context uses for NF frames (>= 2NF), 0 if unsupported. */
 size_t sx_pad_length(size_t NF);
+
+/* ------------------------------------------------------------------------
+   Batched trajectories.  Many independent coherent "all" trajectories of
+   uneven size (atoms, frames, shells, orientations) run as one captured HIP
+   graph: a persistent amplitude kernel draining a per-XCD work queue of
+   (trajectory, frame tile, orientation/shell group, atom range) tiles, then
+   the correlation and finalisation of every trajectory's shells.  Each
+   trajectory follows sx_compute_shells_oriented semantics.  There is no
+   single counterpart in the reference, which runs one trajectory per
+   invocation (AbstractScatterDevice::run, abstract_scatter_device.cpp:105-143);
+   per trajectory the results are those of AllVectorsScatterDevice::compute. */
+
+typedef struct sx_batch sx_batch;
+
+typedef struct sx_trajectory {
+    const float *xyz;     /* host coordinates, layout as sx_stage */
+    size_t NA, NF;
+    int layout;
+    const double *b;      /* [NA] scattering factors */
+    const double *qlen;   /* [nshells] |q| */
+    size_t nshells;
+    const double *u;      /* [NM][3] unit orientations */
+    size_t NM;
+    double *fqt;          /* outputs: [nshells][NF][2], [nshells][2], [nshells][2] */
+    double *fq;
+    double *fq2;
+} sx_trajectory;
+
+typedef struct sx_batch_timing {
+    double total_ms;      /* graph launch to completion (HIP events) */
+    double amplitude_ms;  /* persistent amplitude kernel */
+    size_t terms;         /* sum over trajectories of shells*NM*NA*NF */
+    size_t tiles;         /* amplitude work items in the queue */
+    int tile_shape;       /* oriented tile shape used (0..3) */
+} sx_batch_timing;
+
+int sx_batch_create(const sx_config *cfg, sx_batch **out);
+void sx_batch_destroy(sx_batch *batch);
+const char *sx_batch_last_error(const sx_batch *batch);
+/* Stages every trajectory, autotunes the tile shape (cfg.tile_shape == 0)
+   and captures the batch graph.  The trajectory array (and its output
+   buffers) must stay valid until the last sx_batch_run* call. */
+int sx_batch_stage(sx_batch *batch, const sx_trajectory *trajs, size_t ntraj);
+/* One launch of the whole batch; outputs are copied to every trajectory. */
+int sx_batch_run(sx_batch *batch);
+/* Trajectory k alone through the same kernels (its own captured graph);
+   *ms receives its device latency. */
+int sx_batch_run_one(sx_batch *batch, size_t k, double *ms);
+int sx_batch_get_timing(const sx_batch *batch, sx_batch_timing *out);
+
+/* ------------------------------------------------------------------------
+   The network of the learned control variate.  No counterpart in the
+   reference.  A 3-layer MLP, 16 inputs (zero padded), two hidden tanh layers
+   of width 128, one output; W1 and W2 are BF16 MFMA operands with FP32
+   accumulation, the rest is FP32.  Parameters are one flat FP32 array:
+   W1[16][128], b1[128], W2[128][128], b2[128], W3[128], b3 (row-major, input
+   index first).  Training minimises the mean squared error with Adam
+   (beta 0.9 / 0.999, cosine-decayed rate) on batches drawn on the device. */
+
+typedef struct sx_mlp sx_mlp;
+
+size_t sx_mlp_param_count(void);
+int sx_mlp_create(int device, unsigned long long seed, sx_mlp **out);
+void sx_mlp_destroy(sx_mlp *net);
+const char *sx_mlp_last_error(const sx_mlp *net);
+int sx_mlp_set_params(sx_mlp *net, const float *params, size_t count);
+int sx_mlp_get_params(sx_mlp *net, float *params, size_t count);
+/* x: [rows][16], t: [rows]; grads: [param_count] of the mean squared error
+   over the rows; *loss receives that error. */
+int sx_mlp_gradients(sx_mlp *net, const float *x, const float *t, size_t rows, float *grads, float *loss);
+int sx_mlp_train(sx_mlp *net, const float *x, const float *t, size_t rows, int steps, int batch, float lr,
+                 unsigned long long seed, float *final_loss);
+int sx_mlp_predict(sx_mlp *net, const float *x, size_t rows, float *y);
 
 #ifdef __cplusplus
 }

@@ -24,7 +24,10 @@ LAYOUT = {"frames": 0, "atoms": 1, "soa": 2}
 # Every symbol include/sassena_amd.h declares.
 EXPORTS = ("sx_version", "sx_create", "sx_destroy", "sx_last_error", "sx_stage", "sx_set_factors",
            "sx_compute_shell", "sx_compute_shells", "sx_compute_shells_oriented", "sx_get_timing",
-           "sx_pad_length")
+           "sx_pad_length", "sx_batch_create", "sx_batch_destroy", "sx_batch_last_error", "sx_batch_stage",
+           "sx_batch_run", "sx_batch_run_one", "sx_batch_get_timing", "sx_mlp_param_count", "sx_mlp_create",
+           "sx_mlp_destroy", "sx_mlp_last_error", "sx_mlp_set_params", "sx_mlp_get_params", "sx_mlp_gradients",
+           "sx_mlp_train", "sx_mlp_predict")
 
 
 class SxError(RuntimeError):
@@ -36,7 +39,7 @@ class SxError(RuntimeError):
 class sx_config(ctypes.Structure):
     _fields_ = [("device", ctypes.c_int), ("scattering_type", ctypes.c_int), ("dsp_type", ctypes.c_int),
                 ("dsp_method", ctypes.c_int), ("profile", ctypes.c_int), ("vector_block", ctypes.c_int),
-                ("atom_split", ctypes.c_int), ("reserved", ctypes.c_int * 9)]
+                ("atom_split", ctypes.c_int), ("tile_shape", ctypes.c_int), ("reserved", ctypes.c_int * 8)]
 
 
 class sx_timing(ctypes.Structure):

@@ -15,7 +15,7 @@
 // (orientation types file and sphere, abstract_vectors_scatter_device.cpp:115-129),
 // amp_oriented_kernel forms d = u_m . r once per atom and orientation and the
 // phase of each shell as the single FP64 product (|q_s| / 2 pi) d.
-#include "sx_internal.h"
+#include "sx_amp_device.h"
 
 namespace sx {
 
@@ -175,20 +175,8 @@ __global__ __launch_bounds__(256) void amp_all_kernel(AmpKernelArgs a) {
 }
 
 
-struct OrientKernelArgs {
-    const float *X, *Y, *Z;
-    size_t NFp;
-    const double *qlen;    // [ns] |q_s| / 2 pi
-    const double *u;       // [NM][3] unit orientations
-    const float *b;        // [nb][NAp]
-    size_t NAp;
-    int NM, ns, V, ftiles, ogroups, sgroups, split, atoms_per_split, per_shell_b;
-    float2 *amp;           // [split][V][NFp], v = s * NM + m
-};
-
 template <int VM, int VS, bool PER_ROW_B>
 __global__ __launch_bounds__(256) void amp_oriented_kernel(OrientKernelArgs a) {
-    constexpr int CH = 16;  // atoms per FP32 chunk
     const int nitems = a.ftiles * a.split * a.ogroups * a.sgroups;
     const int per = (nitems + 7) >> 3;
     const int item = (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
@@ -200,98 +188,28 @@ __global__ __launch_bounds__(256) void amp_oriented_kernel(OrientKernelArgs a) {
     const int sp = rest % a.split;
     const int ft = rest / a.split;
 
-    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int lane = threadIdx.x & 63;
-    const int mbase = (og * 4 + wave) * VM;
-    const int sbase = sg * VS;
-    const size_t t = (size_t)ft * kWave + lane;
-
-    double ux[VM], uy[VM], uz[VM], ql[VS];
-#pragma unroll
-    for (int i = 0; i < VM; ++i) {
-        const int m = min(mbase + i, a.NM - 1);
-        ux[i] = a.u[3 * m];
-        uy[i] = a.u[3 * m + 1];
-        uz[i] = a.u[3 * m + 2];
-    }
-#pragma unroll
-    for (int k = 0; k < VS; ++k) ql[k] = a.qlen[min(sbase + k, a.ns - 1)];
-
-    float sr[VM][VS], si[VM][VS], kr[VM][VS], ki[VM][VS];
-#pragma unroll
-    for (int i = 0; i < VM; ++i)
-#pragma unroll
-        for (int k = 0; k < VS; ++k) sr[i][k] = si[i][k] = kr[i][k] = ki[i][k] = 0.f;
-
-    const size_t j0 = (size_t)sp * a.atoms_per_split;
-    const size_t j1 = min(a.NAp, j0 + (size_t)a.atoms_per_split);
-    const float *__restrict__ X = a.X + t;
-    const float *__restrict__ Y = a.Y + t;
-    const float *__restrict__ Z = a.Z + t;
-
-    for (size_t j = j0; j < j1; j += CH) {
-        float xs[CH], ys[CH], zs[CH];
-#pragma unroll
-        for (int c = 0; c < CH; ++c) {
-            xs[c] = X[(j + c) * a.NFp];
-            ys[c] = Y[(j + c) * a.NFp];
-            zs[c] = Z[(j + c) * a.NFp];
-        }
-        float cr[VM][VS], ci[VM][VS];
-#pragma unroll
-        for (int i = 0; i < VM; ++i)
-#pragma unroll
-            for (int k = 0; k < VS; ++k) cr[i][k] = ci[i][k] = 0.f;
-#pragma unroll
-        for (int c = 0; c < CH; ++c) {
-            const double xd = xs[c], yd = ys[c], zd = zs[c];
-            float bk[VS];
-#pragma unroll
-            for (int k = 0; k < VS; ++k)
-                bk[k] = PER_ROW_B ? a.b[(size_t)min(sbase + k, a.ns - 1) * a.NAp + j + c] : a.b[j + c];
-#pragma unroll
-            for (int i = 0; i < VM; ++i) {
-                double d = ux[i] * xd;
-                d = fma(uy[i], yd, d);
-                d = fma(uz[i], zd, d);
-#pragma unroll
-                for (int k = 0; k < VS; ++k) {
-                    const float f = (float)__builtin_amdgcn_fract(ql[k] * d);
-                    cr[i][k] = fmaf(bk[k], __builtin_amdgcn_cosf(f), cr[i][k]);
-                    ci[i][k] = fmaf(bk[k], __builtin_amdgcn_sinf(f), ci[i][k]);
-                }
-            }
-        }
-#pragma unroll
-        for (int i = 0; i < VM; ++i)
-#pragma unroll
-            for (int k = 0; k < VS; ++k) {
-                float y = cr[i][k] - kr[i][k];
-                float s = sr[i][k] + y;
-                kr[i][k] = (s - sr[i][k]) - y;
-                sr[i][k] = s;
-                y = ci[i][k] - ki[i][k];
-                s = si[i][k] + y;
-                ki[i][k] = (s - si[i][k]) - y;
-                si[i][k] = s;
-            }
-    }
-
-#pragma unroll
-    for (int i = 0; i < VM; ++i)
-#pragma unroll
-        for (int k = 0; k < VS; ++k) {
-            const int m = mbase + i, sh = sbase + k;
-            if (m < a.NM && sh < a.ns) {
-                const size_t v = (size_t)sh * a.NM + m;
-                a.amp[((size_t)sp * a.V + v) * a.NFp + t] =
-                    make_float2(sr[i][k] - kr[i][k], si[i][k] - ki[i][k]);
-            }
-        }
+    oriented_tile<VM, VS, PER_ROW_B>(a, ft, sp, og, sg);
 }
 
-hipError_t launch_amp_oriented(const OrientArgs &a, hipStream_t stream) {
-    constexpr int VM = 2, VS = 8, CH = 16;
+// Oriented tile shapes (orientations x shells per wavefront), picked by the
+// host autotuner: 0: 2x8, 1: 4x4, 2: 1x8, 3: 1x16.
+static const int kShapeVM[kOrientShapes] = {2, 4, 1, 1};
+static const int kShapeVS[kOrientShapes] = {8, 4, 8, 16};
+
+int orient_shape_vm(int shape) { return kShapeVM[shape]; }
+int orient_shape_vs(int shape) { return kShapeVS[shape]; }
+
+template <int VM, int VS>
+static void launch_oriented_shape(const OrientKernelArgs &k, bool per_row, int blocks, hipStream_t stream) {
+    if (per_row)
+        hipLaunchKernelGGL((amp_oriented_kernel<VM, VS, true>), dim3(blocks), dim3(256), 0, stream, k);
+    else
+        hipLaunchKernelGGL((amp_oriented_kernel<VM, VS, false>), dim3(blocks), dim3(256), 0, stream, k);
+}
+
+OrientKernelArgs make_orient_args(const OrientArgs &a) {
+    constexpr int CH = 16;
+    const int VM = kShapeVM[a.shape], VS = kShapeVS[a.shape];
     OrientKernelArgs k;
     k.X = a.coords;
     k.Y = a.coords + a.NAp * a.NFp;
@@ -313,17 +231,27 @@ hipError_t launch_amp_oriented(const OrientArgs &a, hipStream_t stream) {
     k.atoms_per_split = (int)aps;
     k.per_shell_b = a.per_shell_b;
     k.amp = a.amp;
+    return k;
+}
+
+hipError_t launch_amp_oriented(const OrientArgs &a, hipStream_t stream) {
+    if (a.shape < 0 || a.shape >= kOrientShapes) return hipErrorInvalidValue;
+    const OrientKernelArgs k = make_orient_args(a);
     const int nitems = k.ftiles * k.split * k.ogroups * k.sgroups;
     const int blocks = ((nitems + 7) / 8) * 8;
-    if (a.per_shell_b)
-        hipLaunchKernelGGL((amp_oriented_kernel<VM, VS, true>), dim3(blocks), dim3(256), 0, stream, k);
-    else
-        hipLaunchKernelGGL((amp_oriented_kernel<VM, VS, false>), dim3(blocks), dim3(256), 0, stream, k);
+    const bool per_row = a.per_shell_b != 0;
+    switch (a.shape) {
+        case 0: launch_oriented_shape<2, 8>(k, per_row, blocks, stream); break;
+        case 1: launch_oriented_shape<4, 4>(k, per_row, blocks, stream); break;
+        case 2: launch_oriented_shape<1, 8>(k, per_row, blocks, stream); break;
+        default: launch_oriented_shape<1, 16>(k, per_row, blocks, stream); break;
+    }
     return hipGetLastError();
 }
 
-int oriented_base_blocks(size_t NFp, int NM, int ns) {
-    return (int)(NFp / kWave) * ((NM + 4 * 2 - 1) / (4 * 2)) * ((ns + 8 - 1) / 8);
+int oriented_base_blocks(size_t NFp, int NM, int ns, int shape) {
+    const int VM = kShapeVM[shape], VS = kShapeVS[shape];
+    return (int)(NFp / kWave) * ((NM + 4 * VM - 1) / (4 * VM)) * ((ns + VS - 1) / VS);
 }
 
 hipError_t launch_amp_all(const AmpArgs &a, hipStream_t stream) {

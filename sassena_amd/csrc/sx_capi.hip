@@ -7,6 +7,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
+#include <map>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -14,58 +16,15 @@
 #include <string>
 #include <vector>
 
-#include "../../include/sassena_amd.h"
-#include "sx_internal.h"
+#include "sx_ctx.h"
 
-namespace {
+using namespace sxh;
 
-constexpr double kTwoPi = 6.283185307179586476925286766559;
-constexpr size_t kStageChunkFloats = size_t(64) << 20;   // 256 MB staging chunks
-constexpr size_t kAmpBudgetBytes = size_t(24) << 30;      // amplitude buffer per batch
-
-struct DevBuf {
-    void *p = nullptr;
-    size_t cap = 0;
-    hipError_t ensure(size_t bytes) {
-        if (bytes <= cap && p) return hipSuccess;
-        if (p) {
-            hipError_t e = hipFree(p);
-            p = nullptr;
-            cap = 0;
-            if (e != hipSuccess) return e;
-        }
-        hipError_t e = hipMalloc(&p, bytes ? bytes : 16);
-        if (e == hipSuccess) cap = bytes;
-        return e;
-    }
-    void release() {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        cap = 0;
-    }
-    template <typename T>
-    T *as() const {
-        return static_cast<T *>(p);
-    }
-};
-
-size_t round_up(size_t x, size_t m) { return (x + m - 1) / m * m; }
-
-}  // namespace
-
-struct sx_ctx {
-    sx_config cfg{};
-    std::string err;
-    hipStream_t stream = nullptr;
-    size_t NA = 0, NF = 0, NAp = 0, NFp = 0;
-    bool staged = false, have_b = false;
-    sx::FftPlan plan;
-    bool plan_ok = false;
-    DevBuf coords, b, twL, twL64, tw64, w;
-    DevBuf qrev, qlen, uvec, brow, bshell, amp, ppart, apart, fqt, fq, fq2, stage;
-    sx_timing timing{};
-    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
-};
+float sxh::elapsed(hipEvent_t a, hipEvent_t b) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, a, b) != hipSuccess) return 0.f;
+    return ms;
+}
 
 namespace {
 
@@ -96,6 +55,7 @@ int pick_vector_block(const sx_ctx *c) {
 // Atom ranges per amplitude tile: enough workgroups (>= 2048, eight
 // wavefronts per SIMD) to fill the 256 CUs, at most 16 partial sums.
 int pick_split_blocks(const sx_ctx *c, long base) {
+    base = std::max<long>(base, 1);
     if (c->cfg.atom_split > 0) return std::max(1, std::min(c->cfg.atom_split, (int)(c->NAp / 16)));
     long split = (2048 + base - 1) / base;
     split = std::min<long>(split, 16);
@@ -139,10 +99,63 @@ int prepare_tables(sx_ctx *c) {
     return SX_OK;
 }
 
-float elapsed(hipEvent_t a, hipEvent_t b) {
-    float ms = 0.f;
-    if (hipEventElapsedTime(&ms, a, b) != hipSuccess) return 0.f;
-    return ms;
+
+// Tile shape of the oriented amplitude kernel.  A fixed cfg.tile_shape wins;
+// otherwise, for calls of at least kTuneTerms terms, every shape is timed once
+// on the real data (HIP events on the context stream) and the fastest is kept
+// for this problem size.  Smaller calls use shape 0 (2 x 8).
+constexpr double kTuneTerms = 2e10;
+
+int pick_shape(sx_ctx *c, int ns, int NM, const float *bptr, bool per_shell, int *shape) {
+    if (c->cfg.tile_shape > 0) {
+        if (c->cfg.tile_shape > sx::kOrientShapes)
+            return fail(c, SX_ERR_ARG, "tile_shape %d out of range", c->cfg.tile_shape);
+        *shape = c->cfg.tile_shape - 1;
+        return SX_OK;
+    }
+    *shape = 0;
+    const double terms = (double)ns * NM * (double)c->NA * (double)c->NF;
+    if (terms < kTuneTerms) return SX_OK;
+    const std::array<size_t, 5> key = {c->NFp, c->NAp, (size_t)NM, (size_t)ns, per_shell ? 1u : 0u};
+    auto it = c->tuned.find(key);
+    if (it != c->tuned.end()) {
+        *shape = it->second;
+        return SX_OK;
+    }
+    float best = 1e30f;
+    hipEvent_t e0, e1;
+    SX_HIP(c, hipEventCreate(&e0));
+    SX_HIP(c, hipEventCreate(&e1));
+    for (int sh = 0; sh < sx::kOrientShapes; ++sh) {
+        const int split = pick_split_blocks(c, sx::oriented_base_blocks(c->NFp, NM, ns, sh));
+        SX_HIP(c, c->amp.ensure(sizeof(float2) * (size_t)ns * NM * c->NFp * split));
+        sx::OrientArgs oa;
+        oa.coords = c->coords.as<float>();
+        oa.NAp = c->NAp;
+        oa.NFp = c->NFp;
+        oa.qlen = c->qlen.as<double>();
+        oa.u = c->uvec.as<double>();
+        oa.b = bptr;
+        oa.NM = NM;
+        oa.ns = ns;
+        oa.split = split;
+        oa.per_shell_b = per_shell ? 1 : 0;
+        oa.shape = sh;
+        oa.amp = c->amp.as<float2>();
+        SX_HIP(c, hipEventRecord(e0, c->stream));
+        SX_HIP(c, sx::launch_amp_oriented(oa, c->stream));
+        SX_HIP(c, hipEventRecord(e1, c->stream));
+        SX_HIP(c, hipEventSynchronize(e1));
+        const float ms = elapsed(e0, e1);
+        if (ms < best) {
+            best = ms;
+            *shape = sh;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    c->tuned[key] = *shape;
+    return SX_OK;
 }
 
 // One batch of shells [s0, s0+ns) through the configured path; outputs land in
@@ -211,7 +224,12 @@ int run_batch(sx_ctx *c, int ns, size_t NM, const double *qv, const double *qlen
     if (prof) SX_HIP(c, hipEventRecord(c->ev[0], c->stream));
     if (!self) {
         const int vb = pick_vector_block(c);
-        const int split = oriented ? pick_split_blocks(c, sx::oriented_base_blocks(c->NFp, (int)NM, ns))
+        int shape = 0;
+        if (oriented) {
+            int rc = pick_shape(c, ns, (int)NM, bptr, b_shell != nullptr, &shape);
+            if (rc != SX_OK) return rc;
+        }
+        const int split = oriented ? pick_split_blocks(c, sx::oriented_base_blocks(c->NFp, (int)NM, ns, shape))
                                    : pick_split(c, V, vb);
         const size_t split_stride = (size_t)V * c->NFp;
         SX_HIP(c, c->amp.ensure(sizeof(float2) * split_stride * split));
@@ -227,6 +245,7 @@ int run_batch(sx_ctx *c, int ns, size_t NM, const double *qv, const double *qlen
             oa.ns = ns;
             oa.split = split;
             oa.per_shell_b = b_shell ? 1 : 0;
+            oa.shape = shape;
             oa.amp = c->amp.as<float2>();
             SX_HIP(c, sx::launch_amp_oriented(oa, c->stream));
         } else {

@@ -1,0 +1,68 @@
+// sx_ctx.h -- the context behind the C ABI (one staged trajectory block on
+// one GPU) and host helpers shared by the single-trajectory and the batched
+// entry points.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <array>
+#include <cstddef>
+#include <map>
+#include <string>
+
+#include "../../include/sassena_amd.h"
+#include "sx_internal.h"
+
+namespace sxh {
+
+constexpr double kTwoPi = 6.283185307179586476925286766559;
+constexpr size_t kStageChunkFloats = size_t(64) << 20;   // 256 MB staging chunks
+constexpr size_t kAmpBudgetBytes = size_t(24) << 30;      // amplitude buffer per batch
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t cap = 0;
+    hipError_t ensure(size_t bytes) {
+        if (bytes <= cap && p) return hipSuccess;
+        if (p) {
+            hipError_t e = hipFree(p);
+            p = nullptr;
+            cap = 0;
+            if (e != hipSuccess) return e;
+        }
+        hipError_t e = hipMalloc(&p, bytes ? bytes : 16);
+        if (e == hipSuccess) cap = bytes;
+        return e;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+    template <typename T>
+    T *as() const {
+        return static_cast<T *>(p);
+    }
+};
+
+inline size_t round_up(size_t x, size_t m) { return (x + m - 1) / m * m; }
+
+float elapsed(hipEvent_t a, hipEvent_t b);
+
+}  // namespace sxh
+
+struct sx_ctx {
+    sx_config cfg{};
+    std::string err;
+    hipStream_t stream = nullptr;
+    size_t NA = 0, NF = 0, NAp = 0, NFp = 0;
+    bool staged = false, have_b = false;
+    sx::FftPlan plan;
+    bool plan_ok = false;
+    sxh::DevBuf coords, b, twL, twL64, tw64, w;
+    sxh::DevBuf qrev, qlen, uvec, brow, bshell, amp, ppart, apart, fqt, fq, fq2, stage;
+    sx_timing timing{};
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    // autotuned oriented tile shape per (NFp, NAp, NM, shells, per-shell b)
+    std::map<std::array<size_t, 5>, int> tuned;
+};

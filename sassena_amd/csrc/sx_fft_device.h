@@ -1,0 +1,496 @@
+// sx_fft_device.h -- device-side building blocks of the time autocorrelation:
+// complex arithmetic in FP32/FP64, radix 2/3/4/5/8 butterflies, Stockham
+// passes in LDS, the per-series power spectrum and the per-block bodies that
+// the single-trajectory and the batched kernels share.
+#pragma once
+
+#include "sx_internal.h"
+
+namespace sx {
+
+// ------------------------------------------------------ complex arithmetic --
+// The same Stockham code runs in FP32 (float2) and FP64 (double2).
+
+template <typename C>
+struct Cx;
+template <>
+struct Cx<float2> {
+    using R = float;
+    static __device__ __forceinline__ float2 make(float x, float y) { return make_float2(x, y); }
+};
+template <>
+struct Cx<double2> {
+    using R = double;
+    static __device__ __forceinline__ double2 make(double x, double y) { return make_double2(x, y); }
+};
+
+template <typename C>
+__device__ __forceinline__ C cmul(C a, C b) {
+    return Cx<C>::make(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+template <typename C>
+__device__ __forceinline__ C cadd(C a, C b) {
+    return Cx<C>::make(a.x + b.x, a.y + b.y);
+}
+template <typename C>
+__device__ __forceinline__ C csub(C a, C b) {
+    return Cx<C>::make(a.x - b.x, a.y - b.y);
+}
+// multiply by -i (forward transform rotation)
+template <typename C>
+__device__ __forceinline__ C cmul_mi(C a) {
+    return Cx<C>::make(a.y, -a.x);
+}
+
+// ------------------------------------------------------------ butterflies --
+// Forward DFTs of radix 2, 3, 4, 5, 8 (sign -1), in place.
+
+template <int R, typename C>
+struct Dft;
+
+template <typename C>
+struct Dft<2, C> {
+    static __device__ __forceinline__ void run(C *x) {
+        const C a = x[0], b = x[1];
+        x[0] = cadd(a, b);
+        x[1] = csub(a, b);
+    }
+};
+
+template <typename C>
+struct Dft<4, C> {
+    static __device__ __forceinline__ void run(C *x) {
+        const C a = cadd(x[0], x[2]), b = csub(x[0], x[2]);
+        const C c = cadd(x[1], x[3]), d = cmul_mi(csub(x[1], x[3]));
+        x[0] = cadd(a, c);
+        x[2] = csub(a, c);
+        x[1] = cadd(b, d);
+        x[3] = csub(b, d);
+    }
+};
+
+template <typename C>
+struct Dft<8, C> {
+    static __device__ __forceinline__ void run(C *x) {
+        using R = typename Cx<C>::R;
+        C e[4] = {x[0], x[2], x[4], x[6]};
+        C o[4] = {x[1], x[3], x[5], x[7]};
+        Dft<4, C>::run(e);
+        Dft<4, C>::run(o);
+        const R s = R(0.70710678118654752440);
+        // o_k *= W8^k, W8 = exp(-i pi/4)
+        o[1] = Cx<C>::make(s * (o[1].x + o[1].y), s * (o[1].y - o[1].x));
+        o[2] = cmul_mi(o[2]);
+        o[3] = Cx<C>::make(s * (o[3].y - o[3].x), -s * (o[3].x + o[3].y));
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            x[k] = cadd(e[k], o[k]);
+            x[k + 4] = csub(e[k], o[k]);
+        }
+    }
+};
+
+template <typename C>
+struct Dft<3, C> {
+    static __device__ __forceinline__ void run(C *x) {
+        using R = typename Cx<C>::R;
+        const R s = R(0.86602540378443864676);  // sin(2 pi / 3)
+        const C t = cadd(x[1], x[2]), u = csub(x[1], x[2]);
+        const C a = Cx<C>::make(x[0].x - R(0.5) * t.x, x[0].y - R(0.5) * t.y);
+        x[0] = cadd(x[0], t);
+        // y1 = a - i s u, y2 = a + i s u
+        x[1] = Cx<C>::make(a.x + s * u.y, a.y - s * u.x);
+        x[2] = Cx<C>::make(a.x - s * u.y, a.y + s * u.x);
+    }
+};
+
+template <typename C>
+struct Dft<5, C> {
+    static __device__ __forceinline__ void run(C *x) {
+        using R = typename Cx<C>::R;
+        const R c1 = R(0.30901699437494742410), c2 = R(-0.80901699437494742410);
+        const R s1 = R(0.95105651629515357212), s2 = R(0.58778525229247312917);
+        const C t1 = cadd(x[1], x[4]), t2 = cadd(x[2], x[3]);
+        const C t3 = csub(x[1], x[4]), t4 = csub(x[2], x[3]);
+        const C a1 = Cx<C>::make(x[0].x + c1 * t1.x + c2 * t2.x, x[0].y + c1 * t1.y + c2 * t2.y);
+        const C a2 = Cx<C>::make(x[0].x + c2 * t1.x + c1 * t2.x, x[0].y + c2 * t1.y + c1 * t2.y);
+        const C b1 = Cx<C>::make(s1 * t3.x + s2 * t4.x, s1 * t3.y + s2 * t4.y);
+        const C b2 = Cx<C>::make(s2 * t3.x - s1 * t4.x, s2 * t3.y - s1 * t4.y);
+        x[0] = cadd(x[0], cadd(t1, t2));
+        // y1 = a1 - i b1, y4 = a1 + i b1, y2 = a2 - i b2, y3 = a2 + i b2
+        x[1] = Cx<C>::make(a1.x + b1.y, a1.y - b1.x);
+        x[4] = Cx<C>::make(a1.x - b1.y, a1.y + b1.x);
+        x[2] = Cx<C>::make(a2.x + b2.y, a2.y - b2.x);
+        x[3] = Cx<C>::make(a2.x - b2.y, a2.y + b2.x);
+    }
+};
+
+// ------------------------------------------------------------ passes ------
+// A sweep transforms NH halves at once (NH = 2: both halves of the padded
+// series share every barrier; NH = 1: one half per sweep, half the LDS).
+// Half k of the sweep lives at buf[k*Lh, (k+1)*Lh) and is global half h0+k.
+
+// Middle pass p: read radix-R groups from LDS, twiddle, butterfly, write back.
+template <typename C, int NH, int E, int R>
+__device__ __forceinline__ void pass_mid(const FftPlan &P, int p, C (&v)[NH][E], C *buf,
+                                         const C *__restrict__ tw, int tid) {
+    if constexpr (E % R == 0) {
+        const int Lh = P.Lh, T = P.T, Ns = P.ns[p];
+        const int stride = Lh / R;
+#pragma unroll
+        for (int i = 0; i < E / R; ++i) {
+            const int j = tid + i * T;
+#pragma unroll
+            for (int q = 0; q < R; ++q)
+#pragma unroll
+                for (int k = 0; k < NH; ++k) v[k][i * R + q] = buf[k * Lh + j + q * stride];
+        }
+        __syncthreads();
+        const int tstep = P.L / (Ns * R);
+#pragma unroll
+        for (int i = 0; i < E / R; ++i) {
+            const int j = tid + i * T;
+            const int kk = j & (Ns - 1);
+#pragma unroll
+            for (int q = 1; q < R; ++q) {
+                const C w = tw[q * kk * tstep];
+#pragma unroll
+                for (int k = 0; k < NH; ++k) v[k][i * R + q] = cmul(v[k][i * R + q], w);
+            }
+            const int o = (j - kk) * R + kk;
+#pragma unroll
+            for (int k = 0; k < NH; ++k) {
+                Dft<R, C>::run(&v[k][i * R]);
+#pragma unroll
+                for (int q = 0; q < R; ++q) buf[k * Lh + o + q * Ns] = v[k][i * R + q];
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// Last pass: its outputs land on bins f' = j + q*Lh/R of each half, the same
+// bins for every series, so the power spectrum accumulates in registers; the
+// full-length bin is f = 2 f' + h.
+template <typename C, int NH, int E, int R, typename Acc>
+__device__ __forceinline__ void pass_last(const FftPlan &P, int h0, C (&v)[NH][E], Acc (&acc)[2][E],
+                                          const C *buf, const C *__restrict__ tw,
+                                          const double2 *__restrict__ w, double &ar, double &ai, int tid) {
+    if constexpr (E % R == 0) {
+        const int Lh = P.Lh, T = P.T;
+        const int stride = Lh / R;  // == Ns of this pass
+#pragma unroll
+        for (int i = 0; i < E / R; ++i) {
+            const int j = tid + i * T;
+#pragma unroll
+            for (int q = 0; q < R; ++q)
+#pragma unroll
+                for (int k = 0; k < NH; ++k) v[k][i * R + q] = buf[k * Lh + j + q * stride];
+        }
+        __syncthreads();  // the buffer may now be overwritten by the next sweep
+        const int tstep = P.L / Lh;  // L / (Ns R) with Ns R = Lh
+#pragma unroll
+        for (int i = 0; i < E / R; ++i) {
+            const int j = tid + i * T;  // k == j in the last pass
+#pragma unroll
+            for (int q = 1; q < R; ++q) {
+                const C t = tw[q * j * tstep];
+#pragma unroll
+                for (int k = 0; k < NH; ++k) v[k][i * R + q] = cmul(v[k][i * R + q], t);
+            }
+#pragma unroll
+            for (int k = 0; k < NH; ++k) {
+                Dft<R, C>::run(&v[k][i * R]);
+#pragma unroll
+                for (int q = 0; q < R; ++q) {
+                    const int f = 2 * (j + q * stride) + h0 + k;
+                    const C x = v[k][i * R + q];
+                    const typename Cx<C>::R pw = x.x * x.x + x.y * x.y;
+                    acc[h0 + k][i * R + q] += (Acc)pw;
+                    const double2 wf = w[f];
+                    ar = fma(wf.x, (double)pw, ar);
+                    ai = fma(wf.y, (double)pw, ai);
+                }
+            }
+        }
+    }
+}
+
+// First pass: inputs come from a functor (HBM amplitudes or on-the-fly
+// phases); half 1 gets the pre-twiddle W_L^n = tw[n].  Stride Ns = 1, so the
+// pass itself needs no twiddles.
+template <typename C, int NH, int E, int R, typename Input>
+__device__ __forceinline__ void pass_first(const FftPlan &P, int h0, C (&v)[NH][E], C *buf,
+                                           const C *__restrict__ tw, int NF, const Input &in, int tid) {
+    if constexpr (E % R == 0) {
+        const int Lh = P.Lh, T = P.T;
+        const int stride = Lh / R;
+#pragma unroll
+        for (int i = 0; i < E / R; ++i) {
+            const int j = tid + i * T;
+#pragma unroll
+            for (int q = 0; q < R; ++q) {
+                const int n = j + q * stride;
+                C x = Cx<C>::make(0, 0);
+                if (n < NF) x = in(n);
+#pragma unroll
+                for (int k = 0; k < NH; ++k) v[k][i * R + q] = (h0 + k == 0 || n >= NF) ? x : cmul(x, tw[n]);
+            }
+#pragma unroll
+            for (int k = 0; k < NH; ++k) {
+                Dft<R, C>::run(&v[k][i * R]);
+#pragma unroll
+                for (int q = 0; q < R; ++q) buf[k * Lh + j * R + q] = v[k][i * R + q];
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// One sweep: the Stockham passes of NH halves starting at half h0.
+template <typename C, int NH, int E, typename Acc, typename Input>
+__device__ __forceinline__ void fft_sweep(const FftPlan &P, int h0, C (&v)[NH][E], Acc (&acc)[2][E], C *buf,
+                                          const C *__restrict__ tw, const double2 *__restrict__ w, int NF,
+                                          const Input &in, double &ar, double &ai, int tid) {
+    switch (P.radix[0]) {
+        case 2: pass_first<C, NH, E, 2>(P, h0, v, buf, tw, NF, in, tid); break;
+        case 4: pass_first<C, NH, E, 4>(P, h0, v, buf, tw, NF, in, tid); break;
+        case 8: pass_first<C, NH, E, 8>(P, h0, v, buf, tw, NF, in, tid); break;
+        default: break;
+    }
+    for (int p = 1; p < P.npass - 1; ++p) {
+        switch (P.radix[p]) {
+            case 2: pass_mid<C, NH, E, 2>(P, p, v, buf, tw, tid); break;
+            case 4: pass_mid<C, NH, E, 4>(P, p, v, buf, tw, tid); break;
+            case 8: pass_mid<C, NH, E, 8>(P, p, v, buf, tw, tid); break;
+            default: break;
+        }
+    }
+    switch (P.radix[P.npass - 1]) {
+        case 2: pass_last<C, NH, E, 2>(P, h0, v, acc, buf, tw, w, ar, ai, tid); break;
+        case 3: pass_last<C, NH, E, 3>(P, h0, v, acc, buf, tw, w, ar, ai, tid); break;
+        case 4: pass_last<C, NH, E, 4>(P, h0, v, acc, buf, tw, w, ar, ai, tid); break;
+        case 5: pass_last<C, NH, E, 5>(P, h0, v, acc, buf, tw, w, ar, ai, tid); break;
+        case 8: pass_last<C, NH, E, 8>(P, h0, v, acc, buf, tw, w, ar, ai, tid); break;
+        default: break;
+    }
+}
+
+// |X_f|^2 of one padded series accumulated into acc; a += sum_f w_f |X_f|^2.
+template <typename C, int NH, int E, typename Acc, typename Input>
+__device__ __forceinline__ void fft_power(const FftPlan &P, C (&v)[NH][E], Acc (&acc)[2][E], C *buf,
+                                          const C *__restrict__ tw, const double2 *__restrict__ w, int NF,
+                                          const Input &in, double &ar, double &ai, int tid) {
+    if constexpr (NH == 2) {
+        fft_sweep<C, 2, E>(P, 0, v, acc, buf, tw, w, NF, in, ar, ai, tid);
+    } else {
+        fft_sweep<C, 1, E>(P, 0, v, acc, buf, tw, w, NF, in, ar, ai, tid);
+        fft_sweep<C, 1, E>(P, 1, v, acc, buf, tw, w, NF, in, ar, ai, tid);
+    }
+}
+
+// Sum of (x, y) over the block; the result is valid in thread 0.
+__device__ __forceinline__ double2 block_sum2(double x, double y, double2 *red, int tid, int T) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        x += __shfl_xor(x, off, 64);
+        y += __shfl_xor(y, off, 64);
+    }
+    if ((tid & 63) == 0) red[tid >> 6] = make_double2(x, y);
+    __syncthreads();
+    double2 r = make_double2(0.0, 0.0);
+    if (tid == 0)
+        for (int k = 0; k < T / 64; ++k) {
+            r.x += red[k].x;
+            r.y += red[k].y;
+        }
+    __syncthreads();
+    return r;
+}
+
+// Adds the register power spectrum to this block's FP64 row and clears it.
+template <int E, typename Acc>
+__device__ __forceinline__ void flush_power(const FftPlan &P, Acc (&acc)[2][E], double *row, bool first, int tid) {
+    const int R = P.radix[P.npass - 1];
+    const int stride = P.Lh / R;
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        const int i = e / R, q = e % R;
+        const int f = tid + i * P.T + q * stride;
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const double add = (double)acc[h][e];
+            double *dst = row + 2 * f + h;
+            *dst = first ? add : *dst + add;
+            acc[h][e] = Acc(0);
+        }
+    }
+}
+
+// ------------------------------------------------------------ kernels -----
+
+// FP32 power spectra are folded into the FP64 row every kFlushEvery series
+// (bounding the FP32 accumulation to a few terms); FP64 ones only at the end.
+constexpr int kFlushEvery = 16;
+
+template <typename C>
+struct AmpInput {
+    const float2 *p;
+    size_t split_stride;
+    int split;
+    __device__ __forceinline__ C operator()(int n) const {
+        float2 s = p[n];
+        for (int k = 1; k < split; ++k) {
+            const float2 t = p[(size_t)k * split_stride + n];
+            s.x += t.x;
+            s.y += t.y;
+        }
+        return Cx<C>::make(s.x, s.y);
+    }
+};
+
+template <typename C>
+__device__ __forceinline__ const C *twiddles(const CorrArgs &a);
+template <>
+__device__ __forceinline__ const float2 *twiddles<float2>(const CorrArgs &a) {
+    return a.twL;
+}
+template <>
+__device__ __forceinline__ const double2 *twiddles<double2>(const CorrArgs &a) {
+    return a.twL64;
+}
+
+// "all": one block per (shell, chunk of vectors); the series of vector v is
+// its coherent amplitude A_v(t) from HBM (sum of the atom-range partials).
+// One block of the coherent correlation: shell s, vector chunk c.
+template <typename C, int NH, int E>
+__device__ __forceinline__ void corr_all_block(const CorrArgs &a, int s, int c, double2 *lds_d) {
+    using Acc = typename Cx<C>::R;
+    C *buf = reinterpret_cast<C *>(lds_d);
+    const FftPlan &P = a.plan;
+    // The block reduction of a_m runs after the last pass has read the
+    // buffer behind a barrier, and ends with a barrier before the next series
+    // writes it, so it can use the start of the FFT buffer.
+    double2 *red = reinterpret_cast<double2 *>(buf);
+    const C *tw = twiddles<C>(a);
+    const int tid = threadIdx.x;
+    const int m0 = (int)((long)a.NM * c / a.chunks), m1 = (int)((long)a.NM * (c + 1) / a.chunks);
+
+    C v[NH][E];
+    Acc acc[2][E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) acc[0][e] = acc[1][e] = Acc(0);
+    double sa_r = 0.0, sa_i = 0.0, sa2 = 0.0;
+    double *row = a.ppart + ((size_t)s * a.chunks + c) * P.L;
+    const bool fp32 = sizeof(Acc) == 4;
+    bool first = true;
+    int since = 0;
+    for (int m = m0; m < m1; ++m) {
+        const size_t vidx = (size_t)s * a.NM + m;
+        AmpInput<C> in{a.amp + vidx * a.NFp, a.split_stride, a.split};
+        double ar = 0.0, ai = 0.0;
+        // Opaque copy of the thread index: keeps the compiler from hoisting
+        // every pass's address arithmetic out of the series loop (which
+        // would not fit in registers).
+        int t = tid;
+        asm volatile("" : "+v"(t));
+        fft_power<C, NH, E>(P, v, acc, buf, tw, a.w, a.NF, in, ar, ai, t);
+        const double2 r = block_sum2(ar, ai, red, tid, P.T);
+        if (tid == 0) {
+            sa_r += r.x;
+            sa_i += r.y;
+            sa2 += r.x * r.x + r.y * r.y;
+        }
+        if (fp32 && ++since == kFlushEvery) {
+            flush_power<E>(P, acc, row, first, tid);
+            first = false;
+            since = 0;
+        }
+    }
+    if (since > 0 || first || !fp32) flush_power<E>(P, acc, row, first, tid);
+    if (tid == 0) {
+        double *ap = a.apart + ((size_t)s * a.chunks + c) * 4;
+        ap[0] = sa_r;
+        ap[1] = sa_i;
+        ap[2] = sa2;
+        ap[3] = 0.0;
+    }
+}
+
+// "self": the series of pair (atom j, vector m) is b_j exp(i q_m . r_j(t)),
+// evaluated on the fly in the first FFT pass from the SoA coordinates
+// (self_vectors_scatter_device.cpp:306-319).
+struct SelfInput {
+    const float *X, *Y, *Z;
+    double qx, qy, qz;
+    float b;
+    __device__ __forceinline__ float2 operator()(int n) const {
+        double p = qx * (double)X[n];
+        p = fma(qy, (double)Y[n], p);
+        p = fma(qz, (double)Z[n], p);
+        const float f = (float)__builtin_amdgcn_fract(p);
+        return make_float2(b * __builtin_amdgcn_cosf(f), b * __builtin_amdgcn_sinf(f));
+    }
+};
+
+// One block of the shell back-transform: shell s, frame tile `tile` (256
+// frames); Psh holds L doubles of LDS.  fqt(tau) = R(tau) / (L (NF - tau) NM)
+// with R = sum_f P_f e^{+2 pi i f tau / L} in FP64, P the sum of the chunk
+// partials; tile 0 also writes fq = sum a / NM and fq2 = sum |a|^2 / NM.
+__device__ __forceinline__ void final_block(const FinalArgs &a, int s, int tile, double *Psh) {
+    const int tid = threadIdx.x;
+    if (a.ppart != nullptr) {
+        const double *base = a.ppart + (size_t)s * a.chunks * a.L;
+        for (int f = tid; f < a.L; f += 256) {
+            double acc = 0.0;
+            for (int c = 0; c < a.chunks; ++c) acc += base[(size_t)c * a.L + f];
+            Psh[f] = acc;
+        }
+        __syncthreads();
+        const int tau = tile * 256 + tid;
+        if (tau < a.NF) {
+            double rr0 = 0.0, ri0 = 0.0, rr1 = 0.0, ri1 = 0.0;
+            int idx = 0;
+            int f = 0;
+            for (; f + 1 < a.L; f += 2) {
+                const double2 t0 = a.tw64[idx];
+                idx += tau;
+                if (idx >= a.L) idx -= a.L;
+                const double2 t1 = a.tw64[idx];
+                idx += tau;
+                if (idx >= a.L) idx -= a.L;
+                rr0 = fma(Psh[f], t0.x, rr0);
+                ri0 = fma(Psh[f], t0.y, ri0);
+                rr1 = fma(Psh[f + 1], t1.x, rr1);
+                ri1 = fma(Psh[f + 1], t1.y, ri1);
+            }
+            for (; f < a.L; ++f) {
+                const double2 t0 = a.tw64[idx];
+                idx += tau;
+                if (idx >= a.L) idx -= a.L;
+                rr0 = fma(Psh[f], t0.x, rr0);
+                ri0 = fma(Psh[f], t0.y, ri0);
+            }
+            const double scale = 1.0 / ((double)a.L * (double)(a.NF - tau) * (double)a.NM);
+            const double re = (rr0 + rr1) * scale, im = (ri0 + ri1) * scale;
+            a.fqt[((size_t)s * a.NF + tau) * 2] = re;
+            a.fqt[((size_t)s * a.NF + tau) * 2 + 1] = a.conj ? -im : im;
+        }
+    }
+    if (tile == 0 && tid == 0) {
+        double sr = 0.0, si = 0.0, s2 = 0.0;
+        const double *ap = a.apart + (size_t)s * a.chunks * 4;
+        for (int c = 0; c < a.chunks; ++c) {
+            sr += ap[4 * c];
+            si += ap[4 * c + 1];
+            s2 += ap[4 * c + 2];
+        }
+        const double factor = 1.0 / (double)a.NM;
+        a.fq[2 * s] = sr * factor;
+        a.fq[2 * s + 1] = (a.conj ? -si : si) * factor;
+        a.fq2[2 * s] = s2 * factor;
+        a.fq2[2 * s + 1] = 0.0;
+    }
+}
+
+}  // namespace sx

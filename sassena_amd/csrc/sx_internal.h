@@ -70,10 +70,14 @@ struct OrientArgs {
     const double *u;       // [NM][3]
     const float *b;        // [ns][NAp] if per_shell_b else [NAp]
     int NM, ns, split, per_shell_b;
+    int shape = 0;         // tile shape index (< kOrientShapes)
     float2 *amp;           // [split][ns*NM][NFp]
 };
+constexpr int kOrientShapes = 4;
 hipError_t launch_amp_oriented(const OrientArgs &a, hipStream_t stream);
-int oriented_base_blocks(size_t NFp, int NM, int ns);
+int oriented_base_blocks(size_t NFp, int NM, int ns, int shape);
+int orient_shape_vm(int shape);
+int orient_shape_vs(int shape);
 
 struct CorrArgs {
     FftPlan plan;

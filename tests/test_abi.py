@@ -28,7 +28,7 @@ def test_library_exports_every_declared_symbol():
 
 @pytest.mark.parametrize("NF,L", [(1, 512), (200, 512), (256, 512), (257, 1024), (1000, 2048), (1500, 3072),
                                   (2000, 4096), (2500, 5120), (3000, 6144), (4096, 8192), (5000, 10240),
-                                  (6000, 12288), (8192, 16384), (8193, 0)])
+                                  (6000, 12288), (8192, 16384), (8193, 20480), (10240, 20480), (10241, 0)])
 def test_pad_length_plan(NF, L):
     """The correlation length is >= 2NF (linear correlation for tau < NF)."""
     import sassena_amd

@@ -86,7 +86,7 @@ def test_self_against_oracle(sx, oracle_mod, dsp):
         check_close(fqt[s], fq[s], fq2[s], *r)
 
 
-@pytest.mark.parametrize("NF", [1, 2, 63, 64, 65, 255, 256, 257, 777, 1500, 2100, 3000, 4096, 5001])
+@pytest.mark.parametrize("NF", [1, 2, 63, 64, 65, 255, 256, 257, 777, 1500, 2100, 3000, 4096, 5001, 9000])
 def test_frame_counts_against_oracle(sx, oracle_mod, NF):
     """Every FFT length class (2^k, 3*2^k, 5*2^k), ragged frame tiles, NF=1."""
     NA = 37
@@ -169,7 +169,7 @@ def test_errors_are_reported(sx):
         with pytest.raises(sx.SxError):
             dev.set_factors(b[:5])
     with sx.ScatterDevice("all") as dev:
-        frames, b = small_trajectory(2, 9000, seed=1)
+        frames, b = small_trajectory(2, 10300, seed=1)
         dev.stage(frames, layout="frames")
         dev.set_factors(b)
         with pytest.raises(sx.SxError) as e:
