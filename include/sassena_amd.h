@@ -121,6 +121,19 @@ int sx_compute_shells(sx_ctx *ctx, size_t nshells, const double *qv, size_t NM,
 int sx_compute_shells_oriented(sx_ctx *ctx, size_t nshells, const double *qlen, const double *u, size_t NM,
                                const double *b_shell, double *fqt, double *fq, double *fq2);
 
+/* Multipole orientational averaging (scattering.average.orientation.type =
+   multipole), coherent "all" scattering only: MPSphereScatterDevice and
+   MPCylinderScatterDevice (src/scatter_devices/multipole_scatter_device.cpp:
+   428-498, 900-987).  qv: [nshells][3] q vectors; moments: [nmo][2] (major,
+   minor) pairs as the moments parameters give them (parameters.cpp:1037-1122);
+   axis: the orientation axis (cylinder only).  Outputs as sx_compute_shells;
+   the final factor is 1/(4 pi) (sphere) or 1/(2 pi) (cylinder). */
+#define SX_MULTIPOLE_SPHERE 0
+#define SX_MULTIPOLE_CYLINDER 1
+int sx_compute_multipole(sx_ctx *ctx, int kind, size_t nshells, const double *qv, const double *axis,
+                         const int *moments, size_t nmo, const double *b_shell, double *fqt, double *fq,
+                         double *fq2);
+
 /* Timing of the last compute call (zeros unless cfg.profile was set). */
 int sx_get_timing(const sx_ctx *ctx, sx_timing *out);
 

@@ -59,6 +59,13 @@ def lib():
         L.sxo_scan.argtypes = [ctypes.c_double, ctypes.c_double, sz,
                                ctypes.c_double, _dp, _dp]
         L.sxo_scan.restype = sz
+        ip = ctypes.POINTER(ctypes.c_int)
+        L.sxo_sph_bessel_all.argtypes = [ctypes.c_int, ctypes.c_double, _dp]
+        L.sxo_ylm.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double, _dp, _dp]
+        L.sxo_cyl_bessel_j.argtypes = [ctypes.c_int, ctypes.c_double]
+        L.sxo_cyl_bessel_j.restype = ctypes.c_double
+        L.sxo_compute_multipole.argtypes = [ctypes.c_int, _fp, sz, sz, _dp, _dp, _dp, ip, sz, ctypes.c_int,
+                                            ctypes.c_int, _dp, _dp, _dp]
         _lib = L
     return _lib
 
@@ -175,3 +182,50 @@ def scan(frm, to, points, exponent=1.0, base=(1.0, 0.0, 0.0)):
     out = np.zeros((max(points, 1), 3), np.float64)
     k = lib().sxo_scan(frm, to, points, exponent, _d(bb), _d(out))
     return out[:k]
+
+
+def sph_bessel_all(L, x):
+    out = np.zeros(L + 1)
+    lib().sxo_sph_bessel_all(L, x, _d(out))
+    return out
+
+
+def ylm(l, m, theta, phi):
+    re, im = ctypes.c_double(), ctypes.c_double()
+    lib().sxo_ylm(l, m, theta, phi, ctypes.byref(re), ctypes.byref(im))
+    return complex(re.value, im.value)
+
+
+def cyl_bessel_j(n, x):
+    return lib().sxo_cyl_bessel_j(n, x)
+
+
+def sphere_moments(L):
+    """parameters.cpp:1058-1066: (0,0), then l = 1..L with m = -l..l."""
+    return [(0, 0)] + [(l, m) for l in range(1, L + 1) for m in range(-l, l + 1)]
+
+
+def cylinder_moments(L):
+    """parameters.cpp:1068-1075: (0,0), then l = 1..L with m = 0..3."""
+    return [(0, 0)] + [(l, m) for l in range(1, L + 1) for m in range(4)]
+
+
+def compute_multipole(kind, coords_frames, b, q, moments, axis=(0.0, 0.0, 1.0), dsp="autocorrelate",
+                      method="fftw"):
+    """One shell of MPSphereScatterDevice (kind "sphere") or
+    MPCylinderScatterDevice ("cylinder"); coords frames-major [NF][NA][3]."""
+    c = np.ascontiguousarray(coords_frames, dtype=np.float32)
+    NF, NA = c.shape[0], c.shape[1]
+    bb = np.ascontiguousarray(b, dtype=np.float64)
+    qq = np.ascontiguousarray(q, dtype=np.float64)
+    ax = np.ascontiguousarray(axis, dtype=np.float64)
+    mo = np.ascontiguousarray(np.asarray(moments, dtype=np.int32).reshape(-1, 2))
+    fqt = np.zeros(2 * NF)
+    fq = np.zeros(2)
+    fq2 = np.zeros(2)
+    err = lib().sxo_compute_multipole({"sphere": 0, "cylinder": 1}[kind], _f(c), NA, NF, _d(bb), _d(qq), _d(ax),
+                                      mo.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), mo.shape[0], DSP[dsp],
+                                      METHOD[method], _d(fqt), _d(fq), _d(fq2))
+    if err:
+        raise RuntimeError(f"oracle multipole failed ({err})")
+    return _interleaved_to_cplx(fqt), complex(fq[0], fq[1]), complex(fq2[0], fq2[1])

@@ -432,3 +432,294 @@ size_t sxo_scan(double from, double to, size_t points, double exponent,
 #undef PUSH
     return n;
 }
+
+/* ------------------------------------------------------- multipole --- */
+
+/* Spherical Bessel functions j_0..j_L at x >= 0 (the role of
+   boost::math::sph_bessel in multipole_scatter_device.cpp:482): upward
+   recurrence where it is stable (x > L), otherwise Miller's downward
+   recurrence normalised to j_0 (or j_1 near a zero of j_0). */
+void sxo_sph_bessel_all(int L, double x, double *j) {
+    if (x == 0.0) {
+        j[0] = 1.0;
+        for (int l = 1; l <= L; ++l) j[l] = 0.0;
+        return;
+    }
+    const double s = sin(x), c = cos(x);
+    const double j0 = s / x, j1 = s / (x * x) - c / x;
+    if (x > (double)L) {
+        j[0] = j0;
+        if (L >= 1) j[1] = j1;
+        for (int l = 1; l < L; ++l) j[l + 1] = (2.0 * l + 1.0) / x * j[l] - j[l - 1];
+        return;
+    }
+    int N = L + 32 + (int)(2.0 * sqrt((double)L + x));
+    double fp1 = 0.0, f = 1e-280;
+    double scale_hist = 1.0;
+    (void)scale_hist;
+    for (int l = N; l >= 0; --l) {
+        const double fm1 = (2.0 * l + 1.0) / x * f - fp1;  /* f_{l-1} */
+        if (l <= L) j[l] = f;
+        fp1 = f;
+        f = fm1;
+        if (fabs(f) > 1e250) {
+            /* rescale the running values and those stored */
+            const double r = 1e-250;
+            f *= r;
+            fp1 *= r;
+            for (int k = l; k <= L; ++k) j[k] *= r;
+        }
+    }
+    /* j[] now holds unnormalised values for l = 0..L */
+    const double norm = fabs(j0) >= fabs(j1) ? j0 / j[0] : j1 / j[1];
+    for (int l = 0; l <= L; ++l) j[l] *= norm;
+}
+
+/* Normalised associated Legendre term of Y_l^m for m >= 0 (Condon-Shortley
+   phase): sqrt((2l+1)/(4 pi) (l-m)!/(l+m)!) P_l^m(x). */
+double sxo_ylm_theta(int l, int m, double x) {
+    double pmm = 1.0;
+    const double somx2 = sqrt((1.0 - x) * (1.0 + x));
+    double fact = 1.0;
+    for (int i = 1; i <= m; ++i) {
+        pmm *= -fact * somx2;
+        fact += 2.0;
+    }
+    double plm = pmm;
+    if (l > m) {
+        double pmmp1 = x * (2.0 * m + 1.0) * pmm;
+        plm = pmmp1;
+        for (int ll = m + 2; ll <= l; ++ll) {
+            const double pll = ((2.0 * ll - 1.0) * x * pmmp1 - (ll + m - 1.0) * pmm) / (ll - m);
+            pmm = pmmp1;
+            pmmp1 = pll;
+            plm = pll;
+        }
+    }
+    double ratio = 1.0; /* (l-m)!/(l+m)! */
+    for (int k = l - m + 1; k <= l + m; ++k) ratio /= (double)k;
+    return sqrt((2.0 * l + 1.0) / (4.0 * SXO_PI) * ratio) * plm;
+}
+
+/* Y_l^m(theta, phi) for any |m| <= l; Y_l^{-m} = (-1)^m conj(Y_l^m). */
+void sxo_ylm(int l, int m, double theta, double phi, double *re, double *im) {
+    const int am = m < 0 ? -m : m;
+    const double t = sxo_ylm_theta(l, am, cos(theta));
+    double yr = t * cos(am * phi), yi = t * sin(am * phi);
+    if (m < 0) {
+        const double sgn = (am & 1) ? -1.0 : 1.0;
+        yr = sgn * yr;
+        yi = -sgn * yi;
+    }
+    *re = yr;
+    *im = yi;
+}
+
+/* SphericalCoor3D(CartesianCoor3D) (coor3d.cpp:168-202) */
+static void to_spherical(double x, double y, double z, double *r, double *phi, double *theta) {
+    *r = sqrt(pow(x, 2) + pow(y, 2) + pow(z, 2));
+    if (*r == 0) {
+        *theta = 0;
+        *phi = 0;
+        return;
+    }
+    *theta = acos(z / *r);
+    double p;
+    if (x != 0.0) {
+        p = atan(y / x);
+        if (x < 0.0) p += SXO_PI;
+        else if (y < 0.0) p += 2 * SXO_PI;
+    } else if (y != 0.0) {
+        p = y > 0 ? SXO_PI / 2 : 3 * SXO_PI / 2;
+    } else {
+        p = 0.0;
+    }
+    *phi = p;
+}
+
+/* MPSphereScatterDevice::scatter (multipole_scatter_device.cpp:428-498) for
+   one moment over all frames: coordinates are converted to (r, phi, theta)
+   in FP64 and stored as float, as the staged spherical block does. */
+void sxo_scatter_mpsphere(const float *coords, size_t NA, size_t NF, const double *b, double ql, int l, int m,
+                          double *at) {
+    double *jl = (double *)malloc(sizeof(double) * (size_t)(l + 1));
+    for (size_t fi = 0; fi < NF; ++fi) {
+        const float *p_data = coords + fi * NA * 3;
+        double Ar = 0.0, Ai = 0.0;
+        /* 4 pi i^l */
+        const double ilr = (l % 4 == 0) ? 1.0 : (l % 4 == 2 ? -1.0 : 0.0);
+        const double ili = (l % 4 == 1) ? 1.0 : (l % 4 == 3 ? -1.0 : 0.0);
+        for (size_t j = 0; j < NA; ++j) {
+            double r, phi, theta;
+            to_spherical(p_data[3 * j], p_data[3 * j + 1], p_data[3 * j + 2], &r, &phi, &theta);
+            const float rf = (float)r, phif = (float)phi, thetaf = (float)theta;
+            sxo_sph_bessel_all(l, ql * (double)rf, jl);
+            double yr, yi;
+            sxo_ylm(l, m, (double)thetaf, (double)phif, &yr, &yi);
+            /* fmpiilesf * aabess * conj(Y) */
+            const double fr = 4 * SXO_PI * ilr * b[j], fi2 = 4 * SXO_PI * ili * b[j];
+            const double cr = yr * jl[l], ci = -yi * jl[l];
+            Ar += fr * cr - fi2 * ci;
+            Ai += fr * ci + fi2 * cr;
+        }
+        at[2 * fi] = Ar;
+        at[2 * fi + 1] = Ai;
+    }
+    free(jl);
+}
+
+/* Bessel function of the first kind J_n(x), n >= 0, x >= 0 (the role of
+   boost::math::cyl_bessel_j, multipole_scatter_device.cpp:958-976):
+   Miller's downward recurrence normalised by J_0 + 2 sum J_2k = 1, or the
+   upward recurrence from J_0, J_1 when n < x. */
+double sxo_cyl_bessel_j(int n, double x) {
+    if (x == 0.0) return n == 0 ? 1.0 : 0.0;
+    if (n == 0 || n == 1 || n < x) {
+        /* J0, J1 from the Miller sum below when x is small, else upward */
+        if (x > 25.0 && n < x) {
+            /* asymptotic-free route: upward recurrence from accurate J0, J1 */
+            double j0 = jn(0, x), j1 = jn(1, x);
+            if (n == 0) return j0;
+            if (n == 1) return j1;
+            for (int k = 1; k < n; ++k) {
+                const double j2 = 2.0 * k / x * j1 - j0;
+                j0 = j1;
+                j1 = j2;
+            }
+            return j1;
+        }
+    }
+    return jn(n, x);
+}
+
+/* CylinderCoor3D(CartesianCoor3D) (coor3d.cpp:116-140); sign() of
+   coor3d.hpp:30 takes floats, so the pi offsets are float-rounded. */
+static void to_cylinder(double x, double y, double z, double *r, double *phi, double *zz) {
+    *r = sqrt(pow(x, 2) + pow(y, 2));
+    double p;
+    if (x != 0.0) {
+        p = atan(y / x);
+        if (x < 0.0) p = (double)(y < 0.0 ? -(float)SXO_PI : (float)SXO_PI) + p;
+    } else if (y != 0.0) {
+        p = (double)(y < 0.0 ? -(float)(SXO_PI / 2) : (float)(SXO_PI / 2));
+    } else {
+        p = 0.0;
+    }
+    p = p < 0 ? 2 * SXO_PI + p : p;
+    *phi = p;
+    *zz = z;
+}
+
+/* CartesianVectorBase(axis) (coor3d.cpp:278-296): er, ephi, ez */
+void sxo_cylinder_base(const double axis[3], double *base /*[3][3]*/) {
+    const double al = sqrt(pow(axis[0], 2) + pow(axis[1], 2) + pow(axis[2], 2));
+    const double ez[3] = {axis[0] / al, axis[1] / al, axis[2] / al};
+    double ekez[3] = {0 * ez[2] - 1 * ez[1], 1 * ez[0] - 0 * ez[2], 0 * ez[1] - 0 * ez[0]};
+    double len = sqrt(pow(ekez[0], 2) + pow(ekez[1], 2) + pow(ekez[2], 2));
+    if (len == 0) {
+        /* ej x ez with ej = (0,1,0) */
+        ekez[0] = 1 * ez[2] - 0 * ez[1];
+        ekez[1] = 0 * ez[0] - 0 * ez[2];
+        ekez[2] = 0 * ez[1] - 1 * ez[0];
+        len = sqrt(pow(ekez[0], 2) + pow(ekez[1], 2) + pow(ekez[2], 2));
+    }
+    const double er[3] = {ekez[0] / len, ekez[1] / len, ekez[2] / len};
+    double ezer[3] = {ez[1] * er[2] - ez[2] * er[1], ez[2] * er[0] - ez[0] * er[2], ez[0] * er[1] - ez[1] * er[0]};
+    const double l2 = sqrt(pow(ezer[0], 2) + pow(ezer[1], 2) + pow(ezer[2], 2));
+    for (int k = 0; k < 3; ++k) {
+        base[k] = er[k];
+        base[3 + k] = ezer[k] / l2;
+        base[6 + k] = ez[k];
+    }
+}
+
+/* MPCylinderScatterDevice::scatter (multipole_scatter_device.cpp:900-987). */
+void sxo_scatter_mpcylinder(const float *coords, size_t NA, size_t NF, const double *b, const double q[3],
+                            const double axis[3], int l, int m, double *at) {
+    double base[9];
+    sxo_cylinder_base(axis, base);
+    const double qp[3] = {q[0] * base[0] + q[1] * base[1] + q[2] * base[2],
+                          q[0] * base[3] + q[1] * base[4] + q[2] * base[5],
+                          q[0] * base[6] + q[1] * base[7] + q[2] * base[8]};
+    double qr, qphi, qz;
+    to_cylinder(qp[0], qp[1], qp[2], &qr, &qphi, &qz);
+    for (size_t fi = 0; fi < NF; ++fi) {
+        const float *p_data = coords + fi * NA * 3;
+        double Ar = 0.0, Ai = 0.0;
+        for (size_t j = 0; j < NA; ++j) {
+            const double x = p_data[3 * j], y = p_data[3 * j + 1], z0 = p_data[3 * j + 2];
+            const double cp[3] = {x * base[0] + y * base[1] + z0 * base[2], x * base[3] + y * base[4] + z0 * base[5],
+                                  x * base[6] + y * base[7] + z0 * base[8]};
+            double r, phi, z;
+            to_cylinder(cp[0], cp[1], cp[2], &r, &phi, &z);
+            const float rf = (float)r, phif = (float)phi, zf = (float)z;
+            r = rf;
+            phi = phif;
+            z = zf;
+            const double esf = b[j];
+            double parallel_sign = 1.0;
+            if ((z != 0) && (qz != 0)) parallel_sign = (z * qz) / (fabs(z) * fabs(qz));
+            const double ex_r = cos(parallel_sign * z * qz), ex_i = sin(parallel_sign * z * qz);
+            const double p = r * qr;
+            const double psiphi = phi - qphi;
+            double cr = 0.0, ci = 0.0; /* complex factor multiplying expi * esf */
+            if (m == 0 && l == 0) {
+                cr = sxo_cyl_bessel_j(0, p);
+            } else if (m == 0) {
+                cr = sqrt(0.5) * 2.0 * powf(-1.0f, (float)l) * sxo_cyl_bessel_j(2 * l, p) * cos(2 * l * psiphi);
+            } else if (m == 1) {
+                cr = sqrt(0.5) * 2.0 * powf(-1.0f, (float)l) * sxo_cyl_bessel_j(2 * l, p) * sin(2 * l * psiphi);
+            } else if (m == 2) {
+                ci = sqrt(0.5) * 2.0 * powf(-1.0f, (float)(l - 1)) * sxo_cyl_bessel_j(2 * l - 1, p) *
+                     cos((2 * l - 1) * psiphi);
+            } else if (m == 3) {
+                ci = sqrt(0.5) * 2.0 * powf(-1.0f, (float)(l - 1)) * sxo_cyl_bessel_j(2 * l - 1, p) *
+                     sin((2 * l - 1) * psiphi);
+            }
+            Ar += esf * (cr * ex_r - ci * ex_i);
+            Ai += esf * (cr * ex_i + ci * ex_r);
+        }
+        const double norm = sqrt(2 * SXO_PI);
+        at[2 * fi] = norm * Ar;
+        at[2 * fi + 1] = norm * Ai;
+    }
+}
+
+/* MPSphereScatterDevice::compute / MPCylinderScatterDevice::compute: the
+   moments play the role of the vectors; final scale 1/(4 pi) (sphere,
+   multipole_scatter_device.cpp:394) or 1/(2 pi) (cylinder, :866). */
+int sxo_compute_multipole(int kind, const float *coords_frames, size_t NA, size_t NF, const double *b,
+                          const double q[3], const double axis[3], const int *moments, size_t NMO, int dsp_type,
+                          int dsp_method, double *fqt, double *fq, double *fq2) {
+    if (NF == 0 || NMO == 0) return -3;
+    double *buf = (double *)calloc(4 * NF + 2, sizeof(double));
+    if (!buf) return -1;
+    memset(fqt, 0, sizeof(double) * 2 * NF);
+    double afr = 0, afi = 0, a2r = 0;
+    const double ql = sqrt(pow(q[0], 2) + pow(q[1], 2) + pow(q[2], 2));
+    for (size_t k = 0; k < NMO; ++k) {
+        memset(buf, 0, sizeof(double) * 4 * NF);
+        const int l = moments[2 * k], m = moments[2 * k + 1];
+        if (kind == 0) sxo_scatter_mpsphere(coords_frames, NA, NF, b, ql, l, m, buf);
+        else sxo_scatter_mpcylinder(coords_frames, NA, NF, b, q, axis, l, m, buf);
+        int e = dsp(buf, NF, dsp_type, dsp_method);
+        if (e) {
+            free(buf);
+            return e;
+        }
+        double a[2];
+        sxo_store_one(buf, NF, a);
+        afr += a[0];
+        afi += a[1];
+        a2r += a[0] * a[0] + a[1] * a[1];
+        for (size_t i = 0; i < 2 * NF; ++i) fqt[i] += buf[i];
+    }
+    const double factor = kind == 0 ? 1.0 / (4 * SXO_PI) : 1.0 / (2 * SXO_PI);
+    for (size_t i = 0; i < 2 * NF; ++i) fqt[i] *= factor;
+    fq[0] = afr * factor;
+    fq[1] = afi * factor;
+    fq2[0] = a2r * factor;
+    fq2[1] = 0.0;
+    free(buf);
+    return 0;
+}

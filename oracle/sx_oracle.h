@@ -68,6 +68,22 @@ void sxo_normalise_rows(double *v, size_t n);
 size_t sxo_scan(double from, double to, size_t points, double exponent,
                 const double base[3], double *out);
 
+/* Multipole averaging (multipole_scatter_device.cpp): special functions,
+   per-moment amplitudes and the per-shell driver (kind 0 sphere, 1 cylinder;
+   moments [NMO][2] (l, m); axis only for the cylinder). */
+void sxo_sph_bessel_all(int L, double x, double *j);
+double sxo_ylm_theta(int l, int m, double x);
+void sxo_ylm(int l, int m, double theta, double phi, double *re, double *im);
+double sxo_cyl_bessel_j(int n, double x);
+void sxo_cylinder_base(const double axis[3], double *base);
+void sxo_scatter_mpsphere(const float *coords, size_t NA, size_t NF, const double *b, double ql, int l, int m,
+                          double *at);
+void sxo_scatter_mpcylinder(const float *coords, size_t NA, size_t NF, const double *b, const double q[3],
+                            const double axis[3], int l, int m, double *at);
+int sxo_compute_multipole(int kind, const float *coords_frames, size_t NA, size_t NF, const double *b,
+                          const double q[3], const double axis[3], const int *moments, size_t NMO, int dsp_type,
+                          int dsp_method, double *fqt, double *fq, double *fq2);
+
 #ifdef __cplusplus
 }
 #endif

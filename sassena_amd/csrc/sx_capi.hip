@@ -158,10 +158,18 @@ int pick_shape(sx_ctx *c, int ns, int NM, const float *bptr, bool per_shell, int
     return SX_OK;
 }
 
+// Multipole averaging of one call: moments (l, m), per-shell q data.
+struct MPSpec {
+    int kind;                    // 0 sphere, 1 cylinder
+    std::vector<int> moments;    // [nmo][2]
+    std::vector<double> qinfo;   // per shell: |q| (sphere) or q_r, phi_q, q_z (cylinder)
+    double base[9];              // cylinder frame of the axis
+};
+
 // One batch of shells [s0, s0+ns) through the configured path; outputs land in
 // the device buffers at shell offset 0 and are copied to host by the caller.
 int run_batch(sx_ctx *c, int ns, size_t NM, const double *qv, const double *qlen, const double *u,
-              const double *b_shell, double *fqt_h, double *fq_h, double *fq2_h) {
+              const double *b_shell, double *fqt_h, double *fq_h, double *fq2_h, const MPSpec *mp = nullptr) {
     const bool self = c->cfg.scattering_type == SX_SCATTERING_SELF;
     const int dsp = c->cfg.dsp_type;
     const int V = (int)(ns * NM);
@@ -170,14 +178,16 @@ int run_batch(sx_ctx *c, int ns, size_t NM, const double *qv, const double *qlen
 
     // q / (2 pi): the kernels evaluate phases in turns
     std::vector<double> qh(3 * (size_t)V);
-    if (qv) {
+    if (mp) {
+        // multipole: no q vectors per series
+    } else if (qv) {
         for (size_t i = 0; i < qh.size(); ++i) qh[i] = qv[i] / kTwoPi;
     } else {
         for (int s = 0; s < ns; ++s)
             for (size_t m = 0; m < NM; ++m)
                 for (int k = 0; k < 3; ++k) qh[(s * NM + m) * 3 + k] = qlen[s] * u[3 * m + k] / kTwoPi;
     }
-    const bool oriented = qv == nullptr;
+    const bool oriented = qv == nullptr && mp == nullptr;
     if (oriented) {
         std::vector<double> ql(ns);
         for (int s = 0; s < ns; ++s) ql[s] = qlen[s] / kTwoPi;
@@ -217,6 +227,10 @@ int run_batch(sx_ctx *c, int ns, size_t NM, const double *qv, const double *qlen
     fa.NM = (int)NM;
     fa.nshells = ns;
     fa.conj = (dsp == SX_DSP_AUTOCORRELATE && c->cfg.dsp_method == SX_METHOD_DIRECT) ? 1 : 0;
+    // final factor: 1/NM for vector averages (all_vectors_scatter_device.cpp:355), 1/(4 pi) for
+    // multipole spheres and 1/(2 pi) for cylinders (multipole_scatter_device.cpp:394, 866)
+    const double norm = mp ? (mp->kind == 0 ? 1.0 / (2.0 * kTwoPi) : 1.0 / kTwoPi) : 1.0 / (double)NM;
+    fa.norm = norm;
     fa.fqt = c->fqt.as<double>();
     fa.fq = c->fq.as<double>();
     fa.fq2 = c->fq2.as<double>();
@@ -225,15 +239,39 @@ int run_batch(sx_ctx *c, int ns, size_t NM, const double *qv, const double *qlen
     if (!self) {
         const int vb = pick_vector_block(c);
         int shape = 0;
-        if (oriented) {
+        if (oriented && !mp) {
             int rc = pick_shape(c, ns, (int)NM, bptr, b_shell != nullptr, &shape);
             if (rc != SX_OK) return rc;
         }
-        const int split = oriented ? pick_split_blocks(c, sx::oriented_base_blocks(c->NFp, (int)NM, ns, shape))
-                                   : pick_split(c, V, vb);
+        const int split = mp ? 1
+                             : oriented ? pick_split_blocks(c, sx::oriented_base_blocks(c->NFp, (int)NM, ns, shape))
+                                        : pick_split(c, V, vb);
         const size_t split_stride = (size_t)V * c->NFp;
         SX_HIP(c, c->amp.ensure(sizeof(float2) * split_stride * split));
-        if (oriented) {
+        if (mp) {
+            SX_HIP(c, c->mpmom.ensure(sizeof(int) * mp->moments.size()));
+            SX_HIP(c, hipMemcpyAsync(c->mpmom.p, mp->moments.data(), sizeof(int) * mp->moments.size(),
+                                     hipMemcpyHostToDevice, c->stream));
+            SX_HIP(c, c->mpq.ensure(sizeof(double) * (mp->qinfo.size() + 9)));
+            SX_HIP(c, hipMemcpyAsync(c->mpq.p, mp->qinfo.data(), sizeof(double) * mp->qinfo.size(),
+                                     hipMemcpyHostToDevice, c->stream));
+            SX_HIP(c, hipMemcpyAsync(c->mpq.as<double>() + mp->qinfo.size(), mp->base, sizeof(double) * 9,
+                                     hipMemcpyHostToDevice, c->stream));
+            sx::MultipoleArgs ma;
+            ma.coords = c->coords.as<float>();
+            ma.NAp = c->NAp;
+            ma.NFp = c->NFp;
+            ma.kind = mp->kind;
+            ma.qinfo = c->mpq.as<double>();
+            ma.base = c->mpq.as<double>() + mp->qinfo.size();
+            ma.moments = c->mpmom.as<int>();
+            ma.nmo = (int)NM;
+            ma.ns = ns;
+            ma.b = bptr;
+            ma.per_shell_b = b_shell ? 1 : 0;
+            ma.amp = c->amp.as<float2>();
+            SX_HIP(c, sx::launch_multipole(ma, c->stream));
+        } else if (oriented) {
             sx::OrientArgs oa;
             oa.coords = c->coords.as<float>();
             oa.NAp = c->NAp;
@@ -292,7 +330,7 @@ int run_batch(sx_ctx *c, int ns, size_t NM, const double *qv, const double *qlen
         } else {
             SX_HIP(c, sx::launch_series_all(dsp == SX_DSP_SQUARE ? 1 : 2, c->amp.as<float2>(), c->NFp, split,
                                             split_stride, NF, (int)NM, ns, chunks, c->fqt.as<double>(),
-                                            c->apart.as<double>(), c->stream));
+                                            c->apart.as<double>(), norm, c->stream));
             fa.ppart = nullptr;
         }
         c->timing.correlate_launches++;
@@ -418,7 +456,7 @@ void sx_destroy(sx_ctx *c) {
     if (!c) return;
     (void)hipSetDevice(c->cfg.device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
-    for (DevBuf *d : {&c->coords, &c->b, &c->twL, &c->twL64, &c->tw64, &c->w, &c->qrev, &c->qlen, &c->uvec, &c->brow, &c->bshell, &c->amp,
+    for (DevBuf *d : {&c->coords, &c->b, &c->twL, &c->twL64, &c->tw64, &c->w, &c->qrev, &c->qlen, &c->uvec, &c->brow, &c->bshell, &c->mpmom, &c->mpq, &c->amp,
                       &c->ppart, &c->apart, &c->fqt, &c->fq, &c->fq2, &c->stage})
         d->release();
     for (auto &e : c->ev)
@@ -559,6 +597,97 @@ int sx_compute_shells_oriented(sx_ctx *c, size_t nshells, const double *qlen, co
 
 int sx_compute_shell(sx_ctx *c, const double *qv, size_t NM, double *fqt, double *fq, double *fq2) {
     return sx_compute_shells(c, 1, qv, NM, nullptr, fqt, fq, fq2);
+}
+
+int sx_compute_multipole(sx_ctx *c, int kind, size_t nshells, const double *qv, const double *axis,
+                         const int *moments, size_t nmo, const double *b_shell, double *fqt, double *fq,
+                         double *fq2) {
+    if (!c) return SX_ERR_ARG;
+    if (c->cfg.scattering_type != SX_SCATTERING_ALL)
+        return fail(c, SX_ERR_UNSUPPORTED, "multipole averaging is defined for scattering.type=all");
+    if (kind != SX_MULTIPOLE_SPHERE && kind != SX_MULTIPOLE_CYLINDER)
+        return fail(c, SX_ERR_ARG, "unknown multipole type %d", kind);
+    if (!c->staged) return fail(c, SX_ERR_STATE, "compute: no staged coordinates");
+    if (!b_shell && !c->have_b) return fail(c, SX_ERR_STATE, "compute: scattering factors not set");
+    if (nshells == 0 || nshells > 65535 || !qv || !moments || nmo == 0 || nmo > 100000 || !fqt || !fq || !fq2)
+        return fail(c, SX_ERR_ARG, "compute: bad shell list, moment list or output");
+    if (kind == SX_MULTIPOLE_CYLINDER && !axis) return fail(c, SX_ERR_ARG, "cylinder multipole needs an axis");
+    // moment checks of parameters.cpp:1078-1119
+    for (size_t k = 0; k < nmo; ++k) {
+        const int l = moments[2 * k], m = moments[2 * k + 1];
+        if (l < 0 || l > 200) return fail(c, SX_ERR_ARG, "moment %zu: major %d out of range", k, l);
+        if (kind == SX_MULTIPOLE_SPHERE && (m < -l || m > l))
+            return fail(c, SX_ERR_ARG, "moment %zu: minor %d not in [-%d, %d]", k, m, l, l);
+        if (kind == SX_MULTIPOLE_CYLINDER && (m < 0 || m > 3 || (l == 0 && m != 0)))
+            return fail(c, SX_ERR_ARG, "moment %zu: cylinder minor %d invalid for major %d", k, m, l);
+    }
+    if (c->cfg.dsp_type == SX_DSP_AUTOCORRELATE && !c->plan_ok)
+        return fail(c, SX_ERR_UNSUPPORTED, "compute: NF=%zu exceeds the supported correlation length", c->NF);
+    for (size_t i = 0; i < 3 * nshells; ++i)
+        if (!std::isfinite(qv[i])) return fail(c, SX_ERR_ARG, "compute: non-finite q vector component");
+    SX_HIP(c, hipSetDevice(c->cfg.device));
+    c->timing = sx_timing{};
+    c->timing.pad_length = c->plan_ok ? (size_t)c->plan.L : 0;
+    MPSpec mp;
+    mp.kind = kind;
+    mp.moments.assign(moments, moments + 2 * nmo);
+    for (double &x : mp.base) x = 0.0;
+    if (kind == SX_MULTIPOLE_CYLINDER) {
+        // CartesianVectorBase(axis) (coor3d.cpp:278-296)
+        const double al = std::sqrt(axis[0] * axis[0] + axis[1] * axis[1] + axis[2] * axis[2]);
+        if (!(al > 0)) return fail(c, SX_ERR_ARG, "cylinder axis has zero length");
+        const double ez[3] = {axis[0] / al, axis[1] / al, axis[2] / al};
+        double ek[3] = {-ez[1], ez[0], 0.0};  // (0,0,1) x ez
+        double len = std::sqrt(ek[0] * ek[0] + ek[1] * ek[1] + ek[2] * ek[2]);
+        if (len == 0) {
+            ek[0] = ez[2];  // (0,1,0) x ez
+            ek[1] = 0.0;
+            ek[2] = -ez[0];
+            len = std::sqrt(ek[0] * ek[0] + ek[1] * ek[1] + ek[2] * ek[2]);
+        }
+        const double er[3] = {ek[0] / len, ek[1] / len, ek[2] / len};
+        const double ezer[3] = {ez[1] * er[2] - ez[2] * er[1], ez[2] * er[0] - ez[0] * er[2],
+                                ez[0] * er[1] - ez[1] * er[0]};
+        const double l2 = std::sqrt(ezer[0] * ezer[0] + ezer[1] * ezer[1] + ezer[2] * ezer[2]);
+        for (int k = 0; k < 3; ++k) {
+            mp.base[k] = er[k];
+            mp.base[3 + k] = ezer[k] / l2;
+            mp.base[6 + k] = ez[k];
+        }
+    }
+    const size_t per = std::max<size_t>(1, kAmpBudgetBytes / (sizeof(float2) * nmo * c->NFp));
+    for (size_t s0 = 0; s0 < nshells; s0 += per) {
+        const int ns = (int)std::min(per, nshells - s0);
+        mp.qinfo.clear();
+        for (int s = 0; s < ns; ++s) {
+            const double *q = qv + 3 * (s0 + s);
+            if (kind == SX_MULTIPOLE_SPHERE) {
+                mp.qinfo.push_back(std::sqrt(std::pow(q[0], 2) + std::pow(q[1], 2) + std::pow(q[2], 2)));
+            } else {
+                // q projected on the frame, as CylinderCoor3D (coor3d.cpp:116-140)
+                const double px = q[0] * mp.base[0] + q[1] * mp.base[1] + q[2] * mp.base[2];
+                const double py = q[0] * mp.base[3] + q[1] * mp.base[4] + q[2] * mp.base[5];
+                const double pz = q[0] * mp.base[6] + q[1] * mp.base[7] + q[2] * mp.base[8];
+                double phi;
+                if (px != 0.0) {
+                    phi = std::atan(py / px);
+                    if (px < 0.0) phi = (double)(py < 0.0 ? -(float)M_PI : (float)M_PI) + phi;
+                } else if (py != 0.0) {
+                    phi = (double)(py < 0.0 ? -(float)(M_PI / 2) : (float)(M_PI / 2));
+                } else {
+                    phi = 0.0;
+                }
+                phi = phi < 0 ? 2 * M_PI + phi : phi;
+                mp.qinfo.push_back(std::sqrt(std::pow(px, 2) + std::pow(py, 2)));
+                mp.qinfo.push_back(phi);
+                mp.qinfo.push_back(pz);
+            }
+        }
+        int rc = run_batch(c, ns, nmo, nullptr, nullptr, nullptr, b_shell ? b_shell + c->NA * s0 : nullptr,
+                           fqt + 2 * c->NF * s0, fq + 2 * s0, fq2 + 2 * s0, &mp);
+        if (rc != SX_OK) return rc;
+    }
+    return SX_OK;
 }
 
 int sx_get_timing(const sx_ctx *c, sx_timing *out) {

@@ -226,7 +226,7 @@ hipError_t launch_final(const FinalArgs &a, hipStream_t stream) {
 // fqt(t) = (1/NM) sum_m S_m(t), S = |A|^2 (square, smath.cpp:168-177) or A (plain)
 __global__ __launch_bounds__(256) void series_fqt_kernel(int mode, const float2 *__restrict__ amp, size_t NFp,
                                                          int split, size_t split_stride, int NF, int NM,
-                                                         double *__restrict__ fqt) {
+                                                         double *__restrict__ fqt, double norm) {
     const int s = blockIdx.y;
     const int t = blockIdx.x * 256 + threadIdx.x;
     if (t >= NF) return;
@@ -246,9 +246,8 @@ __global__ __launch_bounds__(256) void series_fqt_kernel(int mode, const float2 
             im += x.y;
         }
     }
-    const double f = 1.0 / (double)NM;
-    fqt[((size_t)s * NF + t) * 2] = re * f;
-    fqt[((size_t)s * NF + t) * 2 + 1] = im * f;
+    fqt[((size_t)s * NF + t) * 2] = re * norm;
+    fqt[((size_t)s * NF + t) * 2 + 1] = im * norm;
 }
 
 // per chunk of vectors: sum_m a_m and sum_m |a_m|^2 with a_m = mean_t S_m(t)
@@ -295,9 +294,9 @@ __global__ __launch_bounds__(256) void series_mean_kernel(int mode, const float2
 }
 
 hipError_t launch_series_all(int mode, const float2 *amp, size_t NFp, int split, size_t split_stride, int NF, int NM,
-                             int nshells, int chunks, double *fqt, double *apart, hipStream_t stream) {
+                             int nshells, int chunks, double *fqt, double *apart, double norm, hipStream_t stream) {
     hipLaunchKernelGGL(series_fqt_kernel, dim3((unsigned)((NF + 255) / 256), (unsigned)nshells), dim3(256), 0,
-                       stream, mode, amp, NFp, split, split_stride, NF, NM, fqt);
+                       stream, mode, amp, NFp, split, split_stride, NF, NM, fqt, norm);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(series_mean_kernel, dim3((unsigned)(nshells * chunks)), dim3(256), 0, stream, mode, amp,

@@ -60,7 +60,7 @@ struct sx_ctx {
     sx::FftPlan plan;
     bool plan_ok = false;
     sxh::DevBuf coords, b, twL, twL64, tw64, w;
-    sxh::DevBuf qrev, qlen, uvec, brow, bshell, amp, ppart, apart, fqt, fq, fq2, stage;
+    sxh::DevBuf qrev, qlen, uvec, brow, bshell, amp, ppart, apart, fqt, fq, fq2, stage, mpmom, mpq;
     sx_timing timing{};
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     // autotuned oriented tile shape per (NFp, NAp, NM, shells, per-shell b)

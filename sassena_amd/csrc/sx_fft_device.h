@@ -471,7 +471,8 @@ __device__ __forceinline__ void final_block(const FinalArgs &a, int s, int tile,
                 rr0 = fma(Psh[f], t0.x, rr0);
                 ri0 = fma(Psh[f], t0.y, ri0);
             }
-            const double scale = 1.0 / ((double)a.L * (double)(a.NF - tau) * (double)a.NM);
+            const double norm = a.norm != 0.0 ? a.norm : 1.0 / (double)a.NM;
+            const double scale = norm / ((double)a.L * (double)(a.NF - tau));
             const double re = (rr0 + rr1) * scale, im = (ri0 + ri1) * scale;
             a.fqt[((size_t)s * a.NF + tau) * 2] = re;
             a.fqt[((size_t)s * a.NF + tau) * 2 + 1] = a.conj ? -im : im;
@@ -485,7 +486,7 @@ __device__ __forceinline__ void final_block(const FinalArgs &a, int s, int tile,
             si += ap[4 * c + 1];
             s2 += ap[4 * c + 2];
         }
-        const double factor = 1.0 / (double)a.NM;
+        const double factor = a.norm != 0.0 ? a.norm : 1.0 / (double)a.NM;
         a.fq[2 * s] = sr * factor;
         a.fq[2 * s + 1] = (a.conj ? -si : si) * factor;
         a.fq2[2 * s] = s2 * factor;

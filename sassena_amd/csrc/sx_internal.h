@@ -100,6 +100,7 @@ hipError_t launch_weights(int L, int NF, double2 *w, hipStream_t stream);
 
 struct FinalArgs {
     int L, NF, NM, nshells, chunks;
+    double norm = 0.0;     // final factor (1/NM for vector averages); 0 means 1/NM
     const double *ppart;   // nullptr: fqt already written by the series kernel
     const double *apart;
     const double2 *tw64;   // [L] exp(+2 pi i e / L), double
@@ -108,10 +109,26 @@ struct FinalArgs {
 };
 hipError_t launch_final(const FinalArgs &a, hipStream_t stream);
 
-// dsp.type square (mode 1) / plain (mode 2) on coherent amplitudes
+// dsp.type square (mode 1) / plain (mode 2) on coherent amplitudes; norm is
+// the final factor of fqt (1/NM for vector averages)
 hipError_t launch_series_all(int mode, const float2 *amp, size_t NFp, int split,
                              size_t split_stride, int NF, int NM, int nshells, int chunks,
-                             double *fqt, double *apart, hipStream_t stream);
+                             double *fqt, double *apart, double norm, hipStream_t stream);
+
+// Multipole amplitudes (kind 0 sphere, 1 cylinder), one series per (shell, moment)
+struct MultipoleArgs {
+    const float *coords;
+    size_t NAp, NFp;
+    int kind;
+    const double *qinfo;   // sphere: |q| per shell; cylinder: (q_r, phi_q, q_z) per shell
+    const double *base;    // cylinder frame, 9 doubles
+    const int *moments;    // [nmo][2]
+    int nmo, ns;
+    const float *b;
+    int per_shell_b;
+    float2 *amp;           // [ns * nmo][NFp]
+};
+hipError_t launch_multipole(const MultipoleArgs &m, hipStream_t stream);
 
 // "self" with dsp.type square (mode 1) / plain (mode 2); NF <= kMaxSeriesFrames
 constexpr int kMaxSeriesFrames = 8192;

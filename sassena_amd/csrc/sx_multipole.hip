@@ -1,0 +1,268 @@
+// sx_multipole.hip -- amplitudes of the multipole orientational averages.
+//
+// Sphere (MPSphereScatterDevice::scatter, multipole_scatter_device.cpp:428-498):
+//   A_lm(t) = sum_j 4 pi i^l b_j j_l(|q| r_j) conj(Y_lm(theta_j, phi_j))
+// Cylinder (MPCylinderScatterDevice::scatter, :900-987): with q and r_j in the
+// cylinder frame of the axis (r, phi, z),
+//   A_lm(t) = sqrt(2 pi) sum_j b_j e^{i s z q_z} F_lm(r q_r, phi - phi_q)
+// where F_lm is J_0, or sqrt(1/2) 2 (-1)^l J_2l cos/sin(2l psi) (m = 0, 1), or
+// i sqrt(1/2) 2 (-1)^(l-1) J_(2l-1) cos/sin((2l-1) psi) (m = 2, 3).
+// The staged Cartesian float coordinates are converted per atom and frame in
+// FP64 and the spherical / cylinder components rounded to float, as the
+// reference stages them (coordinate_sets.cpp:359-361, data_stager.cpp:111-113).
+// Lane = frame; a wavefront accumulates up to kMoments moments of one
+// (shell, moment block) in FP64; the amplitudes feed the same correlation
+// and finalisation kernels as the vector averages.
+#include <cmath>
+
+#include "sx_internal.h"
+
+namespace sx {
+
+constexpr int kMoments = 8;
+constexpr double kPi = 3.14159265358979323846;
+
+struct MultipoleKernelArgs {
+    const float *X, *Y, *Z;
+    size_t NFp, NAp;
+    int kind;               // 0 sphere, 1 cylinder
+    const double *qinfo;    // per shell: sphere |q|; cylinder q_r, phi_q, q_z (3 doubles)
+    const double *base;     // cylinder frame er, ephi, ez (9 doubles)
+    const int *moments;     // [nmo][2] (l, m)
+    int nmo, ns, mblocks;
+    const float *b;         // [nb][NAp]
+    int per_shell_b;
+    float2 *amp;            // [ns * nmo][NFp]
+};
+
+// j_l(x) for one l >= 0, x >= 0: upward recurrence where stable (x > l),
+// else Miller's downward recurrence normalised to j_0 (or j_1 near a zero of j_0).
+__device__ double sph_bessel(int l, double x) {
+    if (x == 0.0) return l == 0 ? 1.0 : 0.0;
+    double s, c;
+    sincos(x, &s, &c);
+    const double j0 = s / x, j1 = s / (x * x) - c / x;
+    if (l == 0) return j0;
+    if (l == 1) return j1;
+    if (x > (double)l) {
+        double a = j0, bb = j1;
+        for (int k = 1; k < l; ++k) {
+            const double nx = (2.0 * k + 1.0) / x * bb - a;
+            a = bb;
+            bb = nx;
+        }
+        return bb;
+    }
+    const int N = l + 32 + (int)(2.0 * sqrt((double)l + x));
+    double fp1 = 0.0, f = 1e-280, fl = 0.0, f0 = 0.0, f1 = 0.0;
+    for (int k = N; k >= 0; --k) {
+        if (k == l) fl = f;
+        if (k == 1) f1 = f;
+        if (k == 0) f0 = f;
+        const double fm1 = (2.0 * k + 1.0) / x * f - fp1;
+        fp1 = f;
+        f = fm1;
+        if (fabs(f) > 1e250) {
+            f *= 1e-250;
+            fp1 *= 1e-250;
+            fl *= 1e-250;
+        }
+    }
+    return fabs(j0) >= fabs(j1) ? fl * (j0 / f0) : fl * (j1 / f1);
+}
+
+// sqrt((2l+1)/(4 pi) (l-m)!/(l+m)!) P_l^m(x), m >= 0, Condon-Shortley phase.
+__device__ double ylm_theta(int l, int m, double x) {
+    const double somx2 = sqrt((1.0 - x) * (1.0 + x));
+    double pmm = 1.0, fact = 1.0;
+    for (int i = 1; i <= m; ++i) {
+        pmm *= -fact * somx2;
+        fact += 2.0;
+    }
+    double plm = pmm;
+    if (l > m) {
+        double pmmp1 = x * (2.0 * m + 1.0) * pmm;
+        plm = pmmp1;
+        for (int ll = m + 2; ll <= l; ++ll) {
+            const double pll = ((2.0 * ll - 1.0) * x * pmmp1 - (ll + m - 1.0) * pmm) / (ll - m);
+            pmm = pmmp1;
+            pmmp1 = pll;
+            plm = pll;
+        }
+    }
+    double ratio = 1.0;
+    for (int k = l - m + 1; k <= l + m; ++k) ratio /= (double)k;
+    return sqrt((2.0 * l + 1.0) / (4.0 * kPi) * ratio) * plm;
+}
+
+// coor3d.cpp:168-202
+__device__ void to_spherical(double x, double y, double z, double &r, double &phi, double &theta) {
+    r = sqrt(x * x + y * y + z * z);
+    if (r == 0) {
+        theta = 0;
+        phi = 0;
+        return;
+    }
+    theta = acos(z / r);
+    double p;
+    if (x != 0.0) {
+        p = atan(y / x);
+        if (x < 0.0) p += kPi;
+        else if (y < 0.0) p += 2 * kPi;
+    } else if (y != 0.0) {
+        p = y > 0 ? kPi / 2 : 3 * kPi / 2;
+    } else {
+        p = 0.0;
+    }
+    phi = p;
+}
+
+// coor3d.cpp:116-140 (the pi offsets pass through a float-valued sign())
+__device__ void to_cylinder(double x, double y, double z, double &r, double &phi, double &zz) {
+    r = sqrt(x * x + y * y);
+    double p;
+    if (x != 0.0) {
+        p = atan(y / x);
+        if (x < 0.0) p = (double)(y < 0.0 ? -(float)kPi : (float)kPi) + p;
+    } else if (y != 0.0) {
+        p = (double)(y < 0.0 ? -(float)(kPi / 2) : (float)(kPi / 2));
+    } else {
+        p = 0.0;
+    }
+    phi = p < 0 ? 2 * kPi + p : p;
+    zz = z;
+}
+
+__global__ __launch_bounds__(256) void multipole_kernel(MultipoleKernelArgs a) {
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    // block = (frame tile, shell, group of 4 moment blocks)
+    const int ftile = blockIdx.x;
+    const int s = blockIdx.y;
+    const int mb = blockIdx.z * 4 + wave;
+    if (mb >= a.mblocks) return;  // whole wavefront exits; the kernel has no barriers
+    const int k0 = mb * kMoments;
+    const int nk = min(kMoments, a.nmo - k0);
+    const size_t t = (size_t)ftile * 64 + lane;
+    int ls[kMoments], ms[kMoments];
+#pragma unroll
+    for (int k = 0; k < kMoments; ++k) {
+        const int kk = min(k0 + k, a.nmo - 1);
+        ls[k] = a.moments[2 * kk];
+        ms[k] = a.moments[2 * kk + 1];
+    }
+    double ar[kMoments], ai[kMoments];
+#pragma unroll
+    for (int k = 0; k < kMoments; ++k) ar[k] = ai[k] = 0.0;
+    const float *brow = a.b + (a.per_shell_b ? (size_t)s * a.NAp : 0);
+    double b9[9];
+    if (a.kind == 1)
+        for (int k = 0; k < 9; ++k) b9[k] = a.base[k];
+    for (size_t j = 0; j < a.NAp; ++j) {
+        const float bj = brow[j];
+        if (bj == 0.f) continue;  // padding atoms and zero lengths add nothing
+        const double x = a.X[j * a.NFp + t], y = a.Y[j * a.NFp + t], z = a.Z[j * a.NFp + t];
+        if (a.kind == 0) {
+            const double ql = a.qinfo[s];
+            double r, phi, theta;
+            to_spherical(x, y, z, r, phi, theta);
+            const double rf = (float)r, phif = (float)phi, thetaf = (float)theta;
+            const double ct = cos(thetaf);
+            double sp, cp;
+            sincos(phif, &sp, &cp);
+#pragma unroll
+            for (int k = 0; k < kMoments; ++k) {
+                if (k >= nk) break;
+                const int l = ls[k], m = ms[k], am = m < 0 ? -m : m;
+                const double jl = sph_bessel(l, ql * rf);
+                const double th = ylm_theta(l, am, ct);
+                // e^{i am phi} by repeated products of e^{i phi}
+                double er = 1.0, ei = 0.0;
+                for (int q = 0; q < am; ++q) {
+                    const double nr = er * cp - ei * sp;
+                    ei = er * sp + ei * cp;
+                    er = nr;
+                }
+                double yr = th * er, yi = th * ei;
+                if (m < 0) {
+                    const double sg = (am & 1) ? -1.0 : 1.0;
+                    yr = sg * yr;
+                    yi = -sg * yi;
+                }
+                // 4 pi i^l b j_l conj(Y)
+                const double f = 4.0 * kPi * (double)bj * jl;
+                const double cr = f * yr, ci = -f * yi;
+                switch (l & 3) {
+                    case 0: ar[k] += cr; ai[k] += ci; break;
+                    case 1: ar[k] -= ci; ai[k] += cr; break;
+                    case 2: ar[k] -= cr; ai[k] -= ci; break;
+                    default: ar[k] += ci; ai[k] -= cr; break;
+                }
+            }
+        } else {
+            const double qr = a.qinfo[3 * s], qphi = a.qinfo[3 * s + 1], qz = a.qinfo[3 * s + 2];
+            const double px = x * b9[0] + y * b9[1] + z * b9[2];
+            const double py = x * b9[3] + y * b9[4] + z * b9[5];
+            const double pz = x * b9[6] + y * b9[7] + z * b9[8];
+            double r, phi, zz;
+            to_cylinder(px, py, pz, r, phi, zz);
+            const double rf = (float)r, phif = (float)phi, zf = (float)zz;
+            double sgn = 1.0;
+            if ((zf != 0) && (qz != 0)) sgn = (zf * qz) / (fabs(zf) * fabs(qz));
+            double exs, exc;
+            sincos(sgn * zf * qz, &exs, &exc);
+            const double p = rf * qr, psi = phif - qphi;
+#pragma unroll
+            for (int k = 0; k < kMoments; ++k) {
+                if (k >= nk) break;
+                const int l = ls[k], m = ms[k];
+                double cr = 0.0, ci = 0.0;
+                const double sgl = (l & 1) ? -1.0 : 1.0;
+                if (m == 0 && l == 0) {
+                    cr = jn(0, p);
+                } else if (m == 0) {
+                    cr = sqrt(0.5) * 2.0 * sgl * jn(2 * l, p) * cos(2 * l * psi);
+                } else if (m == 1) {
+                    cr = sqrt(0.5) * 2.0 * sgl * jn(2 * l, p) * sin(2 * l * psi);
+                } else if (m == 2) {
+                    ci = sqrt(0.5) * 2.0 * (-sgl) * jn(2 * l - 1, p) * cos((2 * l - 1) * psi);
+                } else if (m == 3) {
+                    ci = sqrt(0.5) * 2.0 * (-sgl) * jn(2 * l - 1, p) * sin((2 * l - 1) * psi);
+                }
+                ar[k] += (double)bj * (cr * exc - ci * exs);
+                ai[k] += (double)bj * (cr * exs + ci * exc);
+            }
+        }
+    }
+    const double norm = a.kind == 1 ? sqrt(2.0 * kPi) : 1.0;
+#pragma unroll
+    for (int k = 0; k < kMoments; ++k) {
+        if (k >= nk) break;
+        const size_t v = (size_t)s * a.nmo + k0 + k;
+        a.amp[v * a.NFp + t] = make_float2((float)(norm * ar[k]), (float)(norm * ai[k]));
+    }
+}
+
+hipError_t launch_multipole(const MultipoleArgs &m, hipStream_t stream) {
+    MultipoleKernelArgs k;
+    k.X = m.coords;
+    k.Y = m.coords + m.NAp * m.NFp;
+    k.Z = m.coords + 2 * m.NAp * m.NFp;
+    k.NFp = m.NFp;
+    k.NAp = m.NAp;
+    k.kind = m.kind;
+    k.qinfo = m.qinfo;
+    k.base = m.base;
+    k.moments = m.moments;
+    k.nmo = m.nmo;
+    k.ns = m.ns;
+    k.mblocks = (m.nmo + kMoments - 1) / kMoments;
+    k.b = m.b;
+    k.per_shell_b = m.per_shell_b;
+    k.amp = m.amp;
+    dim3 grid((unsigned)(m.NFp / 64), (unsigned)m.ns, (unsigned)((k.mblocks + 3) / 4));
+    hipLaunchKernelGGL(multipole_kernel, grid, dim3(256), 0, stream, k);
+    return hipGetLastError();
+}
+
+}  // namespace sx

@@ -139,6 +139,28 @@ class ScatterDevice:
                                                        _dp(fq2)), "sx_compute_shells_oriented")
         return (fqt[..., 0] + 1j * fqt[..., 1], fq[:, 0] + 1j * fq[:, 1], fq2[:, 0] + 1j * fq2[:, 1])
 
+    def compute_multipole(self, kind, qv, moments, axis=(0.0, 0.0, 1.0), b_shell=None):
+        """Multipole averaging: kind "sphere" or "cylinder"; qv [nshells][3];
+        moments [[l, m], ...] -> (fqt [nshells][NF], fq, fq2)."""
+        q = np.ascontiguousarray(qv, dtype=np.float64).reshape(-1, 3)
+        mo = np.ascontiguousarray(np.asarray(moments, dtype=np.int32).reshape(-1, 2))
+        ax = np.ascontiguousarray(axis, dtype=np.float64)
+        ns = q.shape[0]
+        fqt = np.zeros((ns, self.NF, 2), np.float64)
+        fq = np.zeros((ns, 2), np.float64)
+        fq2 = np.zeros((ns, 2), np.float64)
+        bp = None
+        if b_shell is not None:
+            bs = np.ascontiguousarray(b_shell, dtype=np.float64)
+            if bs.shape != (ns, self.NA):
+                raise ValueError("b_shell must be [nshells][NA]")
+            bp = _dp(bs)
+        kinds = {"sphere": 0, "cylinder": 1}
+        self._check(self._L.sx_compute_multipole(self._h, kinds[kind], ns, _dp(q), _dp(ax),
+                                                 mo.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), mo.shape[0], bp,
+                                                 _dp(fqt), _dp(fq), _dp(fq2)), "sx_compute_multipole")
+        return (fqt[..., 0] + 1j * fqt[..., 1], fq[:, 0] + 1j * fq[:, 1], fq2[:, 0] + 1j * fq2[:, 1])
+
     def timing(self):
         t = _lib.sx_timing()
         self._check(self._L.sx_get_timing(self._h, ctypes.byref(t)), "sx_get_timing")

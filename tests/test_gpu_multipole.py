@@ -1,0 +1,113 @@
+"""Parity of the multipole orientational averages (sx_compute_multipole,
+through the C ABI) with the FP64 oracle.  Same bound as the vector
+averages: max_tau |F - F_ref| / |F_ref(0)| < 1e-5 per shell."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+sys.path.insert(0, os.path.dirname(__file__))
+from util import shell_error  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5
+
+
+def centred_walk(NA, NF, seed, half=8.0, step=0.15):
+    """Seeded random walk about the origin (the multipole expansions are
+    about the coordinate origin), frames-major [NF][NA][3] float32."""
+    rng = np.random.default_rng(seed)
+    start = rng.uniform(-half, half, size=(NA, 3))
+    frames = start[None] + np.cumsum(rng.normal(0, step, size=(NF, NA, 3)), axis=0)
+    b = rng.choice([-3.74, 6.65, 9.36, 5.80], size=NA)
+    return frames.astype(np.float32), b
+
+
+def check(fqt, fq, fq2, ref):
+    rfqt, rfq, rfq2 = ref
+    scale = abs(rfqt[0])
+    err = shell_error(fqt, rfqt)[0]
+    assert err < TOL, f"fqt shell error {err:.3e}"
+    assert abs(fq - rfq) <= TOL * scale, (fq, rfq)
+    assert abs(fq2 - rfq2) <= TOL * (abs(rfq2) + 1e-3 * scale ** 2), (fq2, rfq2)
+
+
+def run(kind, frames, b, qv, moments, axis=(0.0, 0.0, 1.0), dsp="autocorrelate", method="fftw", b_shell=None):
+    import sassena_amd
+    with sassena_amd.ScatterDevice("all", dsp, method) as dev:
+        dev.stage(frames, layout="frames")
+        dev.set_factors(b)
+        return dev.compute_multipole(kind, qv, moments, axis=axis, b_shell=b_shell)
+
+
+@pytest.mark.parametrize("dsp,method", [("autocorrelate", "fftw"), ("autocorrelate", "direct"), ("square", "fftw"),
+                                        ("plain", "fftw")])
+def test_sphere_against_oracle(oracle_mod, dsp, method):
+    frames, b = centred_walk(120, 257, 11)
+    qv = np.array([[0.0, 0.0, 0.3], [0.5, 0.2, -0.7], [1.6, 0.4, 0.1]])
+    mom = oracle_mod.sphere_moments(7)
+    fqt, fq, fq2 = run("sphere", frames, b, qv, mom, dsp=dsp, method=method)
+    for s in range(len(qv)):
+        check(fqt[s], fq[s], fq2[s], oracle_mod.compute_multipole("sphere", frames, b, qv[s], mom, dsp=dsp,
+                                                                   method=method))
+
+
+@pytest.mark.parametrize("axis", [(0.0, 0.0, 1.0), (0.3, -0.2, 0.9), (1.0, 0.0, 0.0)])
+def test_cylinder_against_oracle(oracle_mod, axis):
+    frames, b = centred_walk(100, 300, 12)
+    qv = np.array([[0.4, 0.1, 0.2], [-0.3, 0.9, -0.5], [0.0, 0.0, 1.1], [1.2, -0.6, 0.0]])
+    mom = oracle_mod.cylinder_moments(6)
+    fqt, fq, fq2 = run("cylinder", frames, b, qv, mom, axis=axis)
+    for s in range(len(qv)):
+        check(fqt[s], fq[s], fq2[s], oracle_mod.compute_multipole("cylinder", frames, b, qv[s], mom, axis=axis))
+
+
+def test_large_moment_lists_and_per_shell_factors(oracle_mod):
+    """More moments than one moment block (8) and one block group (32),
+    high orders, ragged frame count and per-shell scattering factors."""
+    frames, b = centred_walk(70, 131, 13)
+    qv = np.array([[0.9, 0.0, 0.0], [0.0, 2.2, 0.5]])
+    mom = oracle_mod.sphere_moments(14)  # 225 moments, l up to 14
+    rng = np.random.default_rng(5)
+    bs = rng.uniform(-2.0, 4.0, size=(len(qv), frames.shape[1]))
+    fqt, fq, fq2 = run("sphere", frames, b, qv, mom, b_shell=bs)
+    for s in range(len(qv)):
+        check(fqt[s], fq[s], fq2[s], oracle_mod.compute_multipole("sphere", frames, bs[s], qv[s], mom))
+
+
+def test_sphere_converges_to_vector_average(oracle_mod):
+    """A static cluster: the truncated multipole sum and a dense vector
+    average both tend to the Debye sum."""
+    rng = np.random.default_rng(8)
+    r = rng.uniform(-1.2, 1.2, size=(1, 5, 3)).astype(np.float32)
+    frames = np.repeat(r, 4, axis=0)
+    b = rng.uniform(0.5, 3.0, 5)
+    ql = 2.1
+    fqt, fq, _ = run("sphere", frames, b, np.array([[0.0, ql, 0.0]]), oracle_mod.sphere_moments(22))
+    rr = r[0].astype(np.float64)
+    d = np.linalg.norm(rr[:, None] - rr[None], axis=2)
+    debye = np.sum(b[:, None] * b[None] * np.sinc(ql * d / np.pi))
+    assert abs(fq[0].real - debye) < 2e-6 * debye
+
+
+def test_multipole_errors():
+    import sassena_amd
+    frames, b = centred_walk(10, 20, 1)
+    with sassena_amd.ScatterDevice("all") as dev:
+        dev.stage(frames, layout="frames")
+        dev.set_factors(b)
+        with pytest.raises(sassena_amd.SxError):
+            dev.compute_multipole("sphere", [[0.1, 0.0, 0.0]], [[2, 3]])
+        with pytest.raises(sassena_amd.SxError):
+            dev.compute_multipole("cylinder", [[0.1, 0.0, 0.0]], [[1, 4]])
+        with pytest.raises(sassena_amd.SxError):
+            dev.compute_multipole("cylinder", [[0.1, 0.0, 0.0]], [[0, 1]])
+        with pytest.raises(sassena_amd.SxError):
+            dev.compute_multipole("cylinder", [[0.1, 0.0, 0.0]], [[0, 0]], axis=(0.0, 0.0, 0.0))
+    with sassena_amd.ScatterDevice("self") as dev:
+        dev.stage(frames, layout="frames")
+        dev.set_factors(b)
+        with pytest.raises(sassena_amd.SxError):
+            dev.compute_multipole("sphere", [[0.1, 0.0, 0.0]], [[0, 0]])

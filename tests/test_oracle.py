@@ -173,3 +173,38 @@ def test_subvectors_and_scans(oracle_mod):
     s2 = oracle_mod.scan(0.0, 1.0, 5, exponent=2.0)[:, 0]
     frac = (np.arange(5) / 4).astype(np.float32)
     assert np.allclose(s2[1:4], (frac * frac)[1:4].astype(np.float64), rtol=0, atol=1e-7)
+
+
+# ------------------------------------------------------------- multipole ---
+
+def test_special_functions_match_scipy(oracle_mod):
+    import scipy.special as sp
+    for x in [1e-6, 0.3, 1.0, 4.2, 19.7, 20.3, 80.0, 866.0]:
+        got = oracle_mod.sph_bessel_all(40, x)
+        ref = sp.spherical_jn(np.arange(41), x)
+        mask = np.abs(ref) > 1e-250
+        assert np.allclose(got[mask], ref[mask], rtol=1e-11, atol=0)
+    for n in [0, 1, 2, 7, 20, 39]:
+        for x in [0.05, 1.0, 9.3, 45.0, 300.0]:
+            assert abs(oracle_mod.cyl_bessel_j(n, x) - sp.jv(n, x)) < 1e-12 * max(1.0, abs(sp.jv(n, x))) + 1e-15
+    for l in range(0, 16):
+        for m in range(-l, l + 1):
+            for th, ph in [(0.2, 0.7), (2.5, 4.0)]:
+                assert abs(oracle_mod.ylm(l, m, th, ph) - sp.sph_harm_y(l, m, th, ph)) < 1e-13
+
+
+def test_sphere_multipole_reaches_debye_limit(oracle_mod):
+    """(1/4 pi) sum_lm |A_lm|^2 of a static cluster tends to the Debye sum."""
+    rng = np.random.default_rng(3)
+    r = (rng.uniform(-1.5, 1.5, size=(1, 6, 3))).astype(np.float32)
+    b = rng.uniform(-2, 3, 6)
+    ql = 1.7
+    mom = oracle_mod.sphere_moments(24)
+    fqt, fq, fq2 = oracle_mod.compute_multipole("sphere", np.repeat(r, 2, axis=0), b, [0, 0, ql], mom)
+    rr = r[0].astype(np.float64)
+    d = np.linalg.norm(rr[:, None] - rr[None], axis=2)
+    debye = np.sum(b[:, None] * b[None] * np.sinc(ql * d / np.pi))
+    # the spherical coordinates are rounded to float, as the reference does
+    assert abs(fqt[0].real - debye) < 2e-7 * abs(debye)
+    assert abs(fq.real - debye) < 2e-7 * abs(debye)
+    assert abs(fqt[0].imag) < 1e-12 * abs(debye)
