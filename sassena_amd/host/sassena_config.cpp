@@ -289,7 +289,14 @@ void read_config_text(const std::string &text, const std::string &basedir, Param
                 v.seed = (unsigned long)o->get("vectors/seed", (long)v.seed);
                 v.filepath = resolve(basedir, o->get("vectors/file", v.filepath));
                 v.create();
-            } else if (sc.orientation_type != "none" && sc.orientation_type != "multipole") {
+            } else if (sc.orientation_type == "multipole") {
+                sc.multipole_type = o->get("multipole/type", sc.multipole_type);
+                MultipoleMoments &mm = sc.moments;
+                mm.type = o->get("multipole/moments/type", mm.type);
+                mm.resolution = o->get("multipole/moments/resolution", mm.resolution);
+                mm.filepath = resolve(basedir, o->get("multipole/moments/file", mm.filepath));
+                mm.create(sc.multipole_type);
+            } else if (sc.orientation_type != "none") {
                 throw terminate_request("Orientation averaging type not understood: " + sc.orientation_type);
             }
         }

@@ -390,15 +390,7 @@ void AbstractScatterDevice::write() {
 void AbstractScatterDevice::compute() {
     if (current_vector_ >= block_first_ && current_vector_ < block_first_ + block_count_) return;
     const size_t first = current_vector_;
-    std::vector<std::vector<CartesianCoor3D>> subs;
-    subs.push_back(init_subvectors(vectors_[first]));
-    const size_t NM = subs[0].size();
-    while (subs.size() < params_.batch_shells && first + subs.size() < vectors_.size()) {
-        std::vector<CartesianCoor3D> s = init_subvectors(vectors_[first + subs.size()]);
-        if (s.size() != NM) break;
-        subs.push_back(std::move(s));
-    }
-    const size_t ns = subs.size();
+    const size_t ns = block_size(first);
     // scattering factors: one set when they do not depend on |q|, else per shell
     std::vector<double> bshell;
     const bool per_shell = scatterfactors_.depends_on_q();
@@ -413,31 +405,7 @@ void AbstractScatterDevice::compute() {
         check(sx_set_factors(ctx_, b.data(), NA), "sx_set_factors");
     }
     std::vector<double> fqt(ns * NF * 2), fq(ns * 2), fq2(ns * 2);
-    if (oriented()) {
-        // q_{s,m} = |q_s| u_m with the shared orientation set
-        const auto &u = params_.scattering.vectors.vectors;
-        std::vector<double> uu(3 * u.size()), ql(ns);
-        for (size_t m = 0; m < u.size(); ++m) {
-            uu[3 * m] = u[m].x;
-            uu[3 * m + 1] = u[m].y;
-            uu[3 * m + 2] = u[m].z;
-        }
-        for (size_t s = 0; s < ns; ++s) ql[s] = vectors_[first + s].length();
-        check(sx_compute_shells_oriented(ctx_, ns, ql.data(), uu.data(), NM, per_shell ? bshell.data() : nullptr,
-                                         fqt.data(), fq.data(), fq2.data()),
-              "sx_compute_shells_oriented");
-    } else {
-        std::vector<double> qv(ns * NM * 3);
-        for (size_t s = 0; s < ns; ++s)
-            for (size_t m = 0; m < NM; ++m) {
-                qv[(s * NM + m) * 3] = subs[s][m].x;
-                qv[(s * NM + m) * 3 + 1] = subs[s][m].y;
-                qv[(s * NM + m) * 3 + 2] = subs[s][m].z;
-            }
-        check(sx_compute_shells(ctx_, ns, qv.data(), NM, per_shell ? bshell.data() : nullptr, fqt.data(), fq.data(),
-                                fq2.data()),
-              "sx_compute_shells");
-    }
+    compute_block(first, ns, per_shell ? bshell.data() : nullptr, fqt.data(), fq.data(), fq2.data());
     block_first_ = first;
     block_count_ = ns;
     block_fqt_.assign(ns, std::vector<std::complex<double>>(NF));
@@ -516,6 +484,130 @@ void SelfVectorsScatterDevice::stage_data() {
     timer_.stop("st:device");
 }
 
+size_t AbstractVectorsScatterDevice::block_size(size_t first) const {
+    // consecutive shells with the same number of subvectors share one call
+    const size_t NM = init_subvectors(vectors_[first]).size();
+    size_t ns = 1;
+    while (ns < params_.batch_shells && first + ns < vectors_.size() &&
+           init_subvectors(vectors_[first + ns]).size() == NM)
+        ++ns;
+    return ns;
+}
+
+void AbstractVectorsScatterDevice::compute_block(size_t first, size_t ns, const double *bshell, double *fqt,
+                                                 double *fq, double *fq2) {
+    if (oriented()) {
+        // q_{s,m} = |q_s| u_m with the shared orientation set
+        const auto &u = params_.scattering.vectors.vectors;
+        std::vector<double> uu(3 * u.size()), ql(ns);
+        for (size_t m = 0; m < u.size(); ++m) {
+            uu[3 * m] = u[m].x;
+            uu[3 * m + 1] = u[m].y;
+            uu[3 * m + 2] = u[m].z;
+        }
+        for (size_t s = 0; s < ns; ++s) ql[s] = vectors_[first + s].length();
+        check(sx_compute_shells_oriented(ctx_, ns, ql.data(), uu.data(), u.size(), bshell, fqt, fq, fq2),
+              "sx_compute_shells_oriented");
+        return;
+    }
+    const size_t NM = init_subvectors(vectors_[first]).size();
+    std::vector<double> qv(ns * NM * 3);
+    for (size_t s = 0; s < ns; ++s) {
+        const std::vector<CartesianCoor3D> sub = init_subvectors(vectors_[first + s]);
+        for (size_t m = 0; m < NM; ++m) {
+            qv[(s * NM + m) * 3] = sub[m].x;
+            qv[(s * NM + m) * 3 + 1] = sub[m].y;
+            qv[(s * NM + m) * 3 + 2] = sub[m].z;
+        }
+    }
+    check(sx_compute_shells(ctx_, ns, qv.data(), NM, bshell, fqt, fq, fq2), "sx_compute_shells");
+}
+
+void MultipoleMoments::create(const std::string &multipole_type) {
+    moments.clear();
+    if (type == "file") {
+        std::ifstream in(filepath);
+        if (!in) throw terminate_request("cannot open multipole moments file: " + filepath);
+        long major, minor;
+        while (in >> major >> minor) moments.emplace_back(major, minor);
+    } else if (type == "resolution") {
+        if (multipole_type == "sphere") {
+            moments.emplace_back(0L, 0L);
+            for (long l = 1; l <= resolution; ++l)
+                for (long m = -l; m <= l; ++m) moments.emplace_back(l, m);
+        } else if (multipole_type == "cylinder") {
+            moments.emplace_back(0L, 0L);
+            for (long l = 1; l <= resolution; ++l)
+                for (long m = 0; m < 4; ++m) moments.emplace_back(l, m);
+        } else {
+            throw terminate_request("Type not understood: scattering.average.orientation.multipole.type=" +
+                                    multipole_type);
+        }
+    } else {
+        throw terminate_request("Type not understood: scattering.average.orientation.multipole.moments.type=" + type);
+    }
+    for (const auto &mm : moments) {
+        const std::string id = "major=" + std::to_string(mm.first) + ", minor=" + std::to_string(mm.second);
+        if (mm.first < 0) throw terminate_request("Major multipole moment must be >= 0! " + id);
+        if (multipole_type == "cylinder") {
+            if (mm.second < 0 || mm.second > 3)
+                throw terminate_request("Minor multipole moment must be between 0 and 3! " + id);
+            if (mm.first == 0 && mm.second != 0)
+                throw terminate_request("Minor multipole moment must be 0 for Major 0! " + id);
+        } else if (multipole_type == "sphere") {
+            if (std::labs(mm.second) > mm.first)
+                throw terminate_request("Minor multipole moment must be between -Major and +Major! " + id);
+        }
+    }
+}
+
+MultipoleScatterDevice::MultipoleScatterDevice(const Params &params, ICoordinateSource &src, ScatterFactors &sf,
+                                               ISignalWriter &writer, std::vector<CartesianCoor3D> qvectors,
+                                               int kind)
+    : AbstractScatterDevice(params, src, sf, writer, std::move(qvectors)), kind_(kind) {
+    const auto &mm = params_.scattering.moments.moments;
+    if (mm.empty()) throw terminate_request("multipole orientational averaging needs at least one moment");
+    for (const auto &p : mm) {
+        moments_.push_back((int)p.first);
+        moments_.push_back((int)p.second);
+    }
+}
+
+void MultipoleScatterDevice::stage_data() {
+    DataStagerByFrame stager(src_, timer_);
+    std::vector<float> block = stager.stage();
+    timer_.start("st:device");
+    check(sx_stage(ctx_, block.data(), NA, NF, SX_LAYOUT_FRAMES_AOS), "sx_stage");
+    timer_.stop("st:device");
+}
+
+size_t MultipoleScatterDevice::block_size(size_t first) const {
+    return std::min(params_.batch_shells, vectors_.size() - first);
+}
+
+void MultipoleScatterDevice::compute_block(size_t first, size_t ns, const double *bshell, double *fqt, double *fq,
+                                           double *fq2) {
+    std::vector<double> qv(3 * ns);
+    for (size_t s = 0; s < ns; ++s) {
+        qv[3 * s] = vectors_[first + s].x;
+        qv[3 * s + 1] = vectors_[first + s].y;
+        qv[3 * s + 2] = vectors_[first + s].z;
+    }
+    const CartesianCoor3D &ax = params_.scattering.axis;
+    const double axis[3] = {ax.x, ax.y, ax.z};
+    check(sx_compute_multipole(ctx_, kind_, ns, qv.data(), axis, moments_.data(), moments_.size() / 2, bshell, fqt,
+                               fq, fq2),
+          "sx_compute_multipole");
+}
+
+MPSphereScatterDevice::MPSphereScatterDevice(const Params &params, ICoordinateSource &src, ScatterFactors &sf,
+                                             ISignalWriter &writer, std::vector<CartesianCoor3D> qvectors)
+    : MultipoleScatterDevice(params, src, sf, writer, std::move(qvectors), SX_MULTIPOLE_SPHERE) {}
+
+MPCylinderScatterDevice::MPCylinderScatterDevice(const Params &params, ICoordinateSource &src, ScatterFactors &sf,
+                                                 ISignalWriter &writer, std::vector<CartesianCoor3D> qvectors)
+    : MultipoleScatterDevice(params, src, sf, writer, std::move(qvectors), SX_MULTIPOLE_CYLINDER) {}
+
 std::unique_ptr<IScatterDevice> ScatterDeviceFactory::create(const Params &params, ICoordinateSource &src,
                                                              ScatterFactors &sf, ISignalWriter &writer,
                                                              std::vector<CartesianCoor3D> qvectors) {
@@ -524,8 +616,14 @@ std::unique_ptr<IScatterDevice> ScatterDeviceFactory::create(const Params &param
     if (sp.type == "all") {
         if (sp.orientation_type == "vectors" || sp.orientation_type == "none")
             return std::make_unique<AllVectorsScatterDevice>(params, src, sf, writer, qvectors);
-        if (sp.orientation_type == "multipole")
-            throw terminate_request("scattering.average.orientation.type=multipole is not provided by this build");
+        if (sp.orientation_type == "multipole") {
+            if (sp.multipole_type == "sphere")
+                return std::make_unique<MPSphereScatterDevice>(params, src, sf, writer, qvectors);
+            if (sp.multipole_type == "cylinder")
+                return std::make_unique<MPCylinderScatterDevice>(params, src, sf, writer, qvectors);
+            throw terminate_request("scattering.average.orientation.multipole.type not understood: " +
+                                    sp.multipole_type);
+        }
         throw terminate_request("Orientation averaging type not understood: " + sp.orientation_type);
     }
     throw terminate_request("scattering.type not understood: " + sp.type);

@@ -88,6 +88,17 @@ struct OrientationVectors {
     void create();
 };
 
+// ScatteringAverageOrientationMultipoleMomentsParameters (parameters.hpp:621-648,
+// parameters.cpp:1037-1122): (major, minor) pairs read from a file or generated
+// up to a maximum major, then checked against the multipole type.
+struct MultipoleMoments {
+    std::string type;  // file | resolution (no default, as the reference)
+    long resolution = 20;
+    std::string filepath = "moments.txt";
+    std::vector<std::pair<long, long>> moments;
+    void create(const std::string &multipole_type);
+};
+
 struct ScatteringParameters {
     std::string type = "all";              // all | self
     double background_factor = 0.0;
@@ -96,6 +107,8 @@ struct ScatteringParameters {
     std::string orientation_type = "none";   // none | vectors | multipole
     CartesianCoor3D axis{0, 0, 1};
     OrientationVectors vectors;
+    std::string multipole_type = "sphere";  // sphere | cylinder
+    MultipoleMoments moments;
     std::vector<CartesianCoor3D> qvectors;   // scattering.vectors (single | scans | file)
     std::vector<ScanParameters> scans;
     void create_from_scans();                // parameters.cpp:1125-1189
@@ -261,9 +274,11 @@ class AbstractScatterDevice : public IScatterDevice {
     void write();
     void next();
     void check(int rc, const char *what);
-    // subvectors of shell q (q_m, m < NM) as AbstractVectorsScatterDevice::init_subvectors
-    virtual std::vector<CartesianCoor3D> init_subvectors(const CartesianCoor3D &q) const = 0;
-    virtual bool oriented() const = 0;  // subvectors are |q| u_m for a shared set u
+    // number of shells from `first` that one device call evaluates
+    virtual size_t block_size(size_t first) const = 0;
+    // one device call over shells [first, first + ns); bshell [ns][NA] or null
+    virtual void compute_block(size_t first, size_t ns, const double *bshell, double *fqt, double *fq,
+                               double *fq2) = 0;
 
     Params params_;
     ICoordinateSource &src_;
@@ -286,8 +301,11 @@ class AbstractVectorsScatterDevice : public AbstractScatterDevice {
     using AbstractScatterDevice::AbstractScatterDevice;
 
    protected:
-    std::vector<CartesianCoor3D> init_subvectors(const CartesianCoor3D &q) const override;
-    bool oriented() const override;
+    // subvectors of shell q (q_m, m < NM) as AbstractVectorsScatterDevice::init_subvectors
+    std::vector<CartesianCoor3D> init_subvectors(const CartesianCoor3D &q) const;
+    bool oriented() const;  // subvectors are |q| u_m for a shared set u
+    size_t block_size(size_t first) const override;
+    void compute_block(size_t first, size_t ns, const double *bshell, double *fqt, double *fq, double *fq2) override;
 };
 
 class AllVectorsScatterDevice : public AbstractVectorsScatterDevice {
@@ -306,6 +324,34 @@ class SelfVectorsScatterDevice : public AbstractVectorsScatterDevice {
 
    protected:
     void stage_data() override;
+};
+
+// MPSphereScatterDevice / MPCylinderScatterDevice (multipole_scatter_device.cpp):
+// the orientational average as a sum over multipole moments of the staged
+// frames; q shells are batched like the vector devices.
+class MultipoleScatterDevice : public AbstractScatterDevice {
+   public:
+    MultipoleScatterDevice(const Params &params, ICoordinateSource &src, ScatterFactors &sf, ISignalWriter &writer,
+                           std::vector<CartesianCoor3D> qvectors, int kind);
+
+   protected:
+    void stage_data() override;
+    size_t block_size(size_t first) const override;
+    void compute_block(size_t first, size_t ns, const double *bshell, double *fqt, double *fq, double *fq2) override;
+    int kind_;  // SX_MULTIPOLE_SPHERE or SX_MULTIPOLE_CYLINDER
+    std::vector<int> moments_;
+};
+
+class MPSphereScatterDevice : public MultipoleScatterDevice {
+   public:
+    MPSphereScatterDevice(const Params &params, ICoordinateSource &src, ScatterFactors &sf, ISignalWriter &writer,
+                          std::vector<CartesianCoor3D> qvectors);
+};
+
+class MPCylinderScatterDevice : public MultipoleScatterDevice {
+   public:
+    MPCylinderScatterDevice(const Params &params, ICoordinateSource &src, ScatterFactors &sf,
+                            ISignalWriter &writer, std::vector<CartesianCoor3D> qvectors);
 };
 
 // ScatterDeviceFactory::create (scatter_device_factory.cpp:23-210)

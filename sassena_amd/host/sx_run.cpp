@@ -33,6 +33,8 @@ static void print_vectors(const Params &p) {
     for (const auto &q : p.scattering.qvectors) std::printf("%.17g %.17g %.17g\n", q.x, q.y, q.z);
     std::printf("orientations %zu\n", p.scattering.vectors.vectors.size());
     for (const auto &u : p.scattering.vectors.vectors) std::printf("%.17g %.17g %.17g\n", u.x, u.y, u.z);
+    std::printf("moments %zu\n", p.scattering.moments.moments.size());
+    for (const auto &m : p.scattering.moments.moments) std::printf("%ld %ld\n", m.first, m.second);
 }
 
 int main(int argc, char **argv) {

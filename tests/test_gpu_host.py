@@ -31,7 +31,12 @@ def setup_case(tmp_path, NA=150, NF=120, NM=16, seed=5):
 
 def config(kind, qfile, orientation="vectors", extra=""):
     orient = {"vectors": "<type>vectors</type><vectors><type>file</type><file>orient.txt</file></vectors>",
-              "none": "<type>none</type>"}[orientation]
+              "none": "<type>none</type>",
+              "sphere": "<type>multipole</type><multipole><type>sphere</type><moments><type>resolution</type>"
+                        "<resolution>5</resolution></moments></multipole>",
+              "cylinder": "<type>multipole</type><axis><x>0.2</x><y>0.1</y><z>1</z></axis><multipole>"
+                          "<type>cylinder</type><moments><type>resolution</type><resolution>4</resolution>"
+                          "</moments></multipole>"}[orientation]
     return f"""<?xml version="1.0"?>
 <root>
   <sample>
@@ -95,3 +100,23 @@ def test_sx_run_scans_without_orientations(tmp_path, oracle_mod):
     q, fqt, fq, fq2 = read_signal(tmp_path)
     assert q.shape == (7, 3)
     check_shells(oracle_mod, "all", frames, b, None, q, fqt, fq, fq2)
+
+
+@pytest.mark.parametrize("mp", ["sphere", "cylinder"])
+def test_sx_run_multipole(tmp_path, oracle_mod, mp):
+    """scattering.average.orientation.type=multipole through the factory."""
+    frames, b, _ = setup_case(tmp_path, NA=60, NF=90)
+    frames = frames - np.float32(12.5)   # about the origin of the expansion
+    write_dcd(tmp_path / "traj.dcd", frames, unit_cell=True)
+    qs = np.array([[0.3, 0.0, 0.0], [0.0, 0.8, 0.3], [0.5, -0.5, 0.4]])
+    (tmp_path / "q.txt").write_text("\n".join(" ".join(f"{v:.17g}" for v in r) for r in qs))
+    (tmp_path / "m.xml").write_text(config("all", "q.txt", orientation=mp))
+    sx_run("--config", str(tmp_path / "m.xml"), cwd=tmp_path)
+    q, fqt, fq, fq2 = read_signal(tmp_path)
+    assert np.array_equal(q, qs)
+    mom = oracle_mod.sphere_moments(5) if mp == "sphere" else oracle_mod.cylinder_moments(4)
+    for i in range(len(qs)):
+        r = oracle_mod.compute_multipole(mp, frames, b, qs[i], mom, axis=(0.2, 0.1, 1.0))
+        assert shell_error(fqt[i], r[0])[0] < 1e-5
+        assert abs(fq[i] - r[1]) < 1e-5 * abs(r[0][0])
+        assert abs(fq2[i] - r[2]) < 1e-5 * (abs(r[2]) + 1e-3 * abs(r[0][0]) ** 2)

@@ -133,3 +133,41 @@ def test_pdb_structure(tmp_path):
         "ATOM      4 1HB  ALA A   1       1.000   2.000   3.000  1.00  0.00\n")
     out = sx_run("--structure", str(tmp_path / "s.pdb")).stdout.split()
     assert out == ["N", "C", "O", "H"]
+
+
+def moments(tmp_path, xml):
+    cfg = tmp_path / "scatter.xml"
+    cfg.write_text(xml)
+    res = sx_run("--config", str(cfg), "--vectors", check=False)
+    if res.returncode != 0:
+        return res
+    lines = res.stdout.split("\n")
+    k = next(i for i, l in enumerate(lines) if l.startswith("moments"))
+    n = int(lines[k].split()[1])
+    return [tuple(int(v) for v in l.split()) for l in lines[k + 1:k + 1 + n]]
+
+
+MP = ("<root><scattering><vectors><type>single</type><single><x>0.5</x><y>0</y><z>0</z></single></vectors>"
+      "<average><orientation><type>multipole</type><multipole><type>{t}</type><moments>{m}</moments>"
+      "</multipole></orientation></average></scattering></root>")
+
+
+def test_multipole_moment_generation(tmp_path, oracle_mod):
+    """parameters.cpp:1037-1122: generated lists and file lists with checks."""
+    got = moments(tmp_path, MP.format(t="sphere", m="<type>resolution</type><resolution>4</resolution>"))
+    assert got == oracle_mod.sphere_moments(4) and len(got) == 25
+    got = moments(tmp_path, MP.format(t="cylinder", m="<type>resolution</type><resolution>3</resolution>"))
+    assert got == oracle_mod.cylinder_moments(3) and len(got) == 13
+    (tmp_path / "mm.txt").write_text("0 0\n3 -2\n5 5\n")
+    got = moments(tmp_path, MP.format(t="sphere", m="<type>file</type><file>mm.txt</file>"))
+    assert got == [(0, 0), (3, -2), (5, 5)]
+    # invalid lists and types are terminate requests
+    for t, text, msg in [("sphere", "2 3\n", "between -Major and +Major"), ("cylinder", "1 4\n", "between 0 and 3"),
+                         ("cylinder", "0 2\n", "must be 0 for Major 0"), ("sphere", "-1 0\n", ">= 0")]:
+        (tmp_path / "bad.txt").write_text(text)
+        res = moments(tmp_path, MP.format(t=t, m="<type>file</type><file>bad.txt</file>"))
+        assert res.returncode == 1 and msg in res.stderr, (t, text, res.stderr)
+    res = moments(tmp_path, MP.format(t="sphere", m="<resolution>4</resolution>"))
+    assert res.returncode == 1 and "moments.type" in res.stderr
+    res = moments(tmp_path, MP.format(t="cone", m="<type>resolution</type>"))
+    assert res.returncode == 1 and "multipole.type" in res.stderr
