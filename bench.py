@@ -58,6 +58,77 @@ def cpu_baseline(T, threads):
                       f"FP64 libm sin/cos, {threads} threads on {cpu_model()}"}
 
 
+def fft_flops(L):
+    """Algorithmic flops of one zero-padded series: two complex FFTs of L/2
+    points (the even and odd bins of the length-L transform), 5 n log2 n each."""
+    import math
+    n = L // 2
+    return 2 * 5.0 * n * math.log2(n)
+
+
+def bench_c3(args):
+    """Incoherent "self": 50,000 atoms (60% H, 25.27 fm; 40% C/O, zero
+    incoherent length), Ornstein-Uhlenbeck walks (sigma 0.2 A, tau 50
+    frames) over 5,000 frames, 20 shells over 0.2-3.0 A^-1 x 100 vectors,
+    FP32 FFT autocorrelation of length 10,240 per (atom, vector), FP64 shell
+    averaging.  One step = all 20 shells."""
+    from sassena_amd import ScatterDevice, synth
+    T = synth.case("c3")
+    dev = ScatterDevice("self", "autocorrelate", "fftw", profile=True)
+    dev.stage(T.xyz, layout="soa")
+    dev.set_factors(T.b)
+    step = lambda: dev.compute_oriented(T.qlen, T.orient)
+    for _ in range(args.warmup):
+        step()
+    k_ms = dev_ms = 0.0
+    launches = 0
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+        tm = dev.timing()
+        k_ms += tm["amplitude_ms"]
+        dev_ms += tm["total_ms"]
+        launches += tm["correlate_launches"]
+        L = tm["pad_length"]
+    wall = time.perf_counter() - t0
+    dev.close()
+    terms = T.terms() * args.steps
+    # zero-length atoms are skipped by the kernel; count the series it transforms
+    nz = int(np.count_nonzero(T.b))
+    series_per_launch = nz * T.NM * T.nshells * args.steps / launches
+    k_avg_s = k_ms / launches * 1e-3
+    achieved = fft_flops(L) * series_per_launch / k_avg_s
+    out = {
+        "metric": "atom*frame*q-vector terms/s (incoherent self F(q,t), per-pair FFT autocorrelation)",
+        "value": terms / wall, "unit": "terms/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32 FFT / f64 phase and shell sums", "data": "synthetic",
+        "config": {"workload": "incoherent self, 50000 atoms x 5000 frames, 20 shells x 100 vectors, pad 10240",
+                   "NA": T.NA, "NF": T.NF, "shells": T.nshells, "vectors": T.NM, "pad": L,
+                   "nonzero_atoms": nz, "terms_per_step": T.terms()},
+        "kernel_ms_per_step": {"corr_self": k_ms / args.steps, "device_total": dev_ms / args.steps},
+        "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": 157.3, "unit": "TFLOP/s",
+                     "frac": achieved / 157.3e12, "traffic": None, "kernel": "corr_self_kernel",
+                     "note": "FP32 FFT flops 2 x 5 (L/2) log2(L/2) per (atom, vector) series over the fused "
+                             "amplitude + FFT kernel time; peak is the published FP32 vector rate"},
+    }
+    if not args.no_cpu:
+        import oracle
+        threads = min(16, os.cpu_count() or 1)
+        idx = np.flatnonzero(T.b)[:threads * 4]
+        atoms = np.ascontiguousarray(np.transpose(T.xyz[:, idx, :], (1, 2, 0)))
+        qv = T.qvectors(T.nshells - 1)[:8]
+        t1 = time.perf_counter()
+        oracle.compute("self", atoms, T.b[idx], qv, "autocorrelate", "fftw", nthreads=threads)
+        dt = time.perf_counter() - t1
+        n = len(idx) * len(qv) * T.NF
+        out["cpu_baseline"] = {"value": n / dt, "unit": "terms/s", "cores": threads, "kind": "port",
+                               "sample": f"1 shell x {len(qv)} vectors x {len(idx)} atoms x {T.NF} frames "
+                                         f"({n:.2e} terms) in {dt:.1f} s, FP64 FFT of {L}, {threads} threads "
+                                         f"on {cpu_model()}"}
+    print(json.dumps(out), flush=True)
+
+
 def bench_c4(args):
     """64 trajectories (atoms log-uniform 500-50,000, frames 256-4,096, box
     (NA/0.1)^(1/3) A, sigma 0.1 walk, 10-60 shells over 0.1-4.0 A^-1 with
@@ -146,8 +217,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--vector-block", type=int, default=0)
     ap.add_argument("--atom-split", type=int, default=0)
-    ap.add_argument("--workload", default="c2", choices=["c2", "c4", "c5"],
-                    help="c2: the headline coherent case; c4: 64 uneven trajectories in one batched graph; "
+    ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c4", "c5"],
+                    help="c2: the headline coherent case; c3: incoherent self with per-pair FFTs; "
+                         "c4: 64 uneven trajectories in one batched graph; "
                          "c5: learned control variate against the 1,000-orientation average")
     ap.add_argument("--c5-scale", type=float, default=1.0, help="shrink the c5 atoms and frames (tests)")
     ap.add_argument("--c4-scale", type=float, default=1.0, help="shrink the c4 trajectories (tests)")
@@ -175,6 +247,8 @@ def main():
 
     from sassena_amd import ScatterDevice, synth
 
+    if args.workload == "c3":
+        return bench_c3(args)
     if args.workload == "c4":
         return bench_c4(args)
     if args.workload == "c5":

@@ -242,6 +242,8 @@ int build_plan(sx_batch *b, const std::vector<int> &ks, int shape, int stride, P
         a.chunks = tw.chunks;
         a.twL = c->twL.as<float2>();
         a.twL64 = c->twL64.as<double2>();
+        a.twp = c->twp.as<float2>();
+        a.twp64 = c->twp64.as<double2>();
         a.w = c->w.as<double2>();
         a.ppart = tw.ppart.as<double>();
         a.apart = tw.apart.as<double>();

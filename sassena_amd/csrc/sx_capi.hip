@@ -94,7 +94,25 @@ int prepare_tables(sx_ctx *c) {
     SX_HIP(c, c->w.ensure(sizeof(double2) * L));
     SX_HIP(c, hipMemcpyAsync(c->twL.p, tw.data(), sizeof(float2) * L, hipMemcpyHostToDevice, c->stream));
     SX_HIP(c, hipMemcpyAsync(c->tw64.p, tw64.data(), sizeof(double2) * L, hipMemcpyHostToDevice, c->stream));
-    SX_HIP(c, sx::launch_weights(L, (int)c->NF, c->w.as<double2>(), c->stream));
+    SX_HIP(c, sx::launch_weights(c->plan, (int)c->NF, c->w.as<double2>(), c->stream));
+    // packed pass twiddles: exp(-2 pi i q kk / (Ns R)) at toff[p] + kk R + q
+    const sx::FftPlan &P = c->plan;
+    std::vector<float2> tp(P.ntw);
+    std::vector<double2> tp64(P.ntw);
+    for (int p = 1; p < P.npass; ++p) {
+        const int Ns = P.ns[p], R = P.radix[p];
+        for (int kk = 0; kk < Ns; ++kk)
+            for (int q = 0; q < R; ++q) {
+                const double ang = kTwoPi * (double)(q * kk) / (double)(Ns * R);
+                const double cs = std::cos(ang), sn = std::sin(ang);
+                tp[P.toff[p] + kk * R + q] = make_float2((float)cs, (float)-sn);
+                tp64[P.toff[p] + kk * R + q] = make_double2(cs, -sn);
+            }
+    }
+    SX_HIP(c, c->twp.ensure(sizeof(float2) * P.ntw));
+    SX_HIP(c, c->twp64.ensure(sizeof(double2) * P.ntw));
+    SX_HIP(c, hipMemcpyAsync(c->twp.p, tp.data(), sizeof(float2) * P.ntw, hipMemcpyHostToDevice, c->stream));
+    SX_HIP(c, hipMemcpyAsync(c->twp64.p, tp64.data(), sizeof(double2) * P.ntw, hipMemcpyHostToDevice, c->stream));
     SX_HIP(c, hipStreamSynchronize(c->stream));
     return SX_OK;
 }
@@ -318,6 +336,8 @@ int run_batch(sx_ctx *c, int ns, size_t NM, const double *qv, const double *qlen
             ca.chunks = chunks;
             ca.twL = c->twL.as<float2>();
             ca.twL64 = c->twL64.as<double2>();
+            ca.twp = c->twp.as<float2>();
+            ca.twp64 = c->twp64.as<double2>();
             ca.w = c->w.as<double2>();
             ca.ppart = c->ppart.as<double>();
             ca.apart = c->apart.as<double>();
@@ -360,9 +380,27 @@ int run_batch(sx_ctx *c, int ns, size_t NM, const double *qv, const double *qlen
         fa.apart = c->apart.as<double>();
         fa.ppart = nullptr;
     } else {
+        // atoms whose length is zero in every shell of the call add nothing
+        // to any sum; the transforms run over the others only
+        c->aidx_h.clear();
+        for (size_t j = 0; j < c->NA; ++j) {
+            bool nz = false;
+            if (b_shell) {
+                for (int s = 0; s < ns && !nz; ++s) nz = (float)b_shell[(size_t)s * c->NA + j] != 0.f;
+            } else {
+                nz = c->b_h[j] != 0.f;  // the kernels read the float factors
+            }
+            if (nz) c->aidx_h.push_back((int)j);
+        }
+        const bool all_atoms = c->aidx_h.size() == c->NA;
+        if (!all_atoms && !c->aidx_h.empty()) {
+            SX_HIP(c, c->aidx.ensure(sizeof(int) * c->aidx_h.size()));
+            SX_HIP(c, hipMemcpyAsync(c->aidx.p, c->aidx_h.data(), sizeof(int) * c->aidx_h.size(),
+                                     hipMemcpyHostToDevice, c->stream));
+        }
         if (prof) SX_HIP(c, hipEventRecord(c->ev[1], c->stream));
-        const long pairs = (long)c->NA * (long)NM;
-        int chunks = (int)std::min<long>(pairs, std::max<long>(1, (2048 + ns - 1) / ns));
+        const long pairs = (long)c->aidx_h.size() * (long)NM;
+        int chunks = (int)std::max<long>(1, std::min<long>(pairs, std::max<long>(1, (2048 + ns - 1) / ns)));
         const sx::FftPlan &P = c->plan;
         SX_HIP(c, c->apart.ensure(sizeof(double) * 4 * (size_t)ns * chunks));
         SX_HIP(c, c->ppart.ensure(sizeof(double) * (size_t)ns * chunks * P.L));
@@ -373,17 +411,19 @@ int run_batch(sx_ctx *c, int ns, size_t NM, const double *qv, const double *qlen
         ca.nshells = ns;
         ca.chunks = chunks;
         ca.twL = c->twL.as<float2>();
+        ca.twp = c->twp.as<float2>();
         ca.w = c->w.as<double2>();
         ca.ppart = c->ppart.as<double>();
         ca.apart = c->apart.as<double>();
         ca.coords = c->coords.as<float>();
         ca.NAp = c->NAp;
         ca.NFp = c->NFp;
-        ca.NA = (int)c->NA;
+        ca.NA = (int)c->aidx_h.size();
+        ca.aidx = all_atoms ? nullptr : c->aidx.as<int>();
         ca.qrev = c->qrev.as<double>();
         ca.b = bptr;
         ca.brow_per_shell = b_shell ? 1 : 0;
-        SX_HIP(c, sx::launch_corr_self(ca, c->stream));
+        SX_HIP(c, sx::launch_corr_self(ca, c->cfg.fft_halves == 2 ? 2 : 1, c->stream));
         c->timing.correlate_launches++;
         fa.chunks = chunks;
         fa.apart = c->apart.as<double>();
@@ -433,6 +473,7 @@ int sx_create(const sx_config *cfg, sx_ctx **out) {
     if (cfg->scattering_type != SX_SCATTERING_ALL && cfg->scattering_type != SX_SCATTERING_SELF) return SX_ERR_ARG;
     if (cfg->dsp_type < SX_DSP_AUTOCORRELATE || cfg->dsp_type > SX_DSP_PLAIN) return SX_ERR_ARG;
     if (cfg->dsp_method != SX_METHOD_FFT && cfg->dsp_method != SX_METHOD_DIRECT) return SX_ERR_ARG;
+    if (cfg->fft_halves < 0 || cfg->fft_halves > 2) return SX_ERR_ARG;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || cfg->device < 0 || cfg->device >= ndev) return SX_ERR_HIP;
     if (hipSetDevice(cfg->device) != hipSuccess) return SX_ERR_HIP;
@@ -456,7 +497,7 @@ void sx_destroy(sx_ctx *c) {
     if (!c) return;
     (void)hipSetDevice(c->cfg.device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
-    for (DevBuf *d : {&c->coords, &c->b, &c->twL, &c->twL64, &c->tw64, &c->w, &c->qrev, &c->qlen, &c->uvec, &c->brow, &c->bshell, &c->mpmom, &c->mpq, &c->amp,
+    for (DevBuf *d : {&c->coords, &c->b, &c->twL, &c->twL64, &c->twp, &c->twp64, &c->tw64, &c->w, &c->aidx, &c->qrev, &c->qlen, &c->uvec, &c->brow, &c->bshell, &c->mpmom, &c->mpq, &c->amp,
                       &c->ppart, &c->apart, &c->fqt, &c->fq, &c->fq2, &c->stage})
         d->release();
     for (auto &e : c->ev)
@@ -529,6 +570,7 @@ int sx_set_factors(sx_ctx *c, const double *b, size_t NA) {
     SX_HIP(c, c->b.ensure(sizeof(float) * c->NAp));
     SX_HIP(c, hipMemcpyAsync(c->b.p, bh.data(), sizeof(float) * c->NAp, hipMemcpyHostToDevice, c->stream));
     SX_HIP(c, hipStreamSynchronize(c->stream));
+    c->b_h = std::move(bh);
     c->have_b = true;
     return SX_OK;
 }

@@ -61,6 +61,12 @@ bool make_fft_plan(size_t NF, FftPlan *plan) {
             ns *= p.radix[i];
         }
         if (ns != c.Lh || np < 2) return false;
+        int off = 0;
+        for (int i = 1; i < np; ++i) {
+            p.toff[i] = off;
+            off += p.ns[i] * p.radix[i];
+        }
+        p.ntw = off;
         *plan = p;
         return true;
     }
@@ -83,7 +89,7 @@ __global__ __launch_bounds__(512) void corr_self_kernel(CorrArgs a) {
     double2 *red = reinterpret_cast<double2 *>(buf);  // reused after the last pass, as in corr_all_block
     const int tid = threadIdx.x;
     const int s = blockIdx.x / a.chunks, c = blockIdx.x % a.chunks;
-    const long npairs = (long)a.NA * a.NM;
+    const long npairs = (long)a.NA * a.NM;  // NA counts the atoms listed in aidx
     const long p0 = npairs * c / a.chunks, p1 = npairs * (c + 1) / a.chunks;
 
     C v[NH][E];
@@ -96,7 +102,8 @@ __global__ __launch_bounds__(512) void corr_self_kernel(CorrArgs a) {
     bool first = true;
     int since = 0;
     for (long pr = p0; pr < p1; ++pr) {
-        const int j = (int)(pr / a.NM), m = (int)(pr % a.NM);
+        const int ja = (int)(pr / a.NM), m = (int)(pr % a.NM);
+        const int j = a.aidx ? a.aidx[ja] : ja;
         const double *q = a.qrev + ((size_t)s * a.NM + m) * 3;
         SelfInput in;
         in.X = a.coords + (size_t)j * a.NFp;
@@ -109,7 +116,7 @@ __global__ __launch_bounds__(512) void corr_self_kernel(CorrArgs a) {
         double ar = 0.0, ai = 0.0;
         int t = tid;
         asm volatile("" : "+v"(t));
-        fft_power<C, NH, E>(P, v, acc, buf, a.twL, a.w, a.NF, in, ar, ai, t);
+        fft_power<C, NH, E>(P, v, acc, buf, a.twp, a.twL, a.w, a.NF, in, ar, ai, t);
         const double2 r = block_sum2(ar, ai, red, tid, P.T);
         if (tid == 0) {
             sa_r += r.x;
@@ -168,16 +175,20 @@ hipError_t launch_corr_all(const CorrArgs &a, hipStream_t stream) {
                                    corr_all_kernel<double2, 1, 24>, a, stream);
 }
 
-hipError_t launch_corr_self(const CorrArgs &a, hipStream_t stream) {
-    return launch_corr<float2, 2>(corr_self_kernel<float2, 2, 4>, corr_self_kernel<float2, 2, 8>,
-                                  corr_self_kernel<float2, 2, 16>, corr_self_kernel<float2, 2, 20>,
-                                  corr_self_kernel<float2, 2, 24>, a, stream);
+hipError_t launch_corr_self(const CorrArgs &a, int halves, hipStream_t stream) {
+    if (halves == 2)
+        return launch_corr<float2, 2>(corr_self_kernel<float2, 2, 4>, corr_self_kernel<float2, 2, 8>,
+                                      corr_self_kernel<float2, 2, 16>, corr_self_kernel<float2, 2, 20>,
+                                      corr_self_kernel<float2, 2, 24>, a, stream);
+    return launch_corr<float2, 1>(corr_self_kernel<float2, 1, 4>, corr_self_kernel<float2, 1, 8>,
+                                  corr_self_kernel<float2, 1, 16>, corr_self_kernel<float2, 1, 20>,
+                                  corr_self_kernel<float2, 1, 24>, a, stream);
 }
 
 // ------------------------------------------------------ FP64 transforms ----
 
 // w_f = sum_{tau<NF} c_tau e^{+2 pi i f tau / L}, c_tau = 1 / (NF L (NF - tau))
-__global__ __launch_bounds__(256) void weights_kernel(int L, int NF, double2 *__restrict__ w) {
+__global__ __launch_bounds__(256) void weights_kernel(int L, int NF, int R, double2 *__restrict__ w) {
     const int f = blockIdx.x * 256 + threadIdx.x;
     if (f >= L) return;
     double wr = 0.0, wi = 0.0;
@@ -192,11 +203,14 @@ __global__ __launch_bounds__(256) void weights_kernel(int L, int NF, double2 *__
         idx += f;
         if (idx >= L) idx -= L;
     }
-    w[f] = make_double2(wr, wi);
+    const int stride = L / (2 * R);  // Lh / R
+    const int fh = f >> 1, q = fh / stride, j = fh - q * stride;
+    w[((f & 1) * R + q) * stride + j] = make_double2(wr, wi);
 }
 
-hipError_t launch_weights(int L, int NF, double2 *w, hipStream_t stream) {
-    hipLaunchKernelGGL(weights_kernel, dim3((unsigned)((L + 255) / 256)), dim3(256), 0, stream, L, NF, w);
+hipError_t launch_weights(const FftPlan &P, int NF, double2 *w, hipStream_t stream) {
+    hipLaunchKernelGGL(weights_kernel, dim3((unsigned)((P.L + 255) / 256)), dim3(256), 0, stream, P.L, NF,
+                       P.radix[P.npass - 1], w);
     return hipGetLastError();
 }
 

@@ -133,7 +133,7 @@ struct Dft<5, C> {
 // Middle pass p: read radix-R groups from LDS, twiddle, butterfly, write back.
 template <typename C, int NH, int E, int R>
 __device__ __forceinline__ void pass_mid(const FftPlan &P, int p, C (&v)[NH][E], C *buf,
-                                         const C *__restrict__ tw, int tid) {
+                                         const C *__restrict__ twp, int tid) {
     if constexpr (E % R == 0) {
         const int Lh = P.Lh, T = P.T, Ns = P.ns[p];
         const int stride = Lh / R;
@@ -146,14 +146,14 @@ __device__ __forceinline__ void pass_mid(const FftPlan &P, int p, C (&v)[NH][E],
                 for (int k = 0; k < NH; ++k) v[k][i * R + q] = buf[k * Lh + j + q * stride];
         }
         __syncthreads();
-        const int tstep = P.L / (Ns * R);
 #pragma unroll
         for (int i = 0; i < E / R; ++i) {
             const int j = tid + i * T;
             const int kk = j & (Ns - 1);
+            const C *tk = twp + P.toff[p] + kk * R;
 #pragma unroll
             for (int q = 1; q < R; ++q) {
-                const C w = tw[q * kk * tstep];
+                const C w = tk[q];
 #pragma unroll
                 for (int k = 0; k < NH; ++k) v[k][i * R + q] = cmul(v[k][i * R + q], w);
             }
@@ -174,7 +174,7 @@ __device__ __forceinline__ void pass_mid(const FftPlan &P, int p, C (&v)[NH][E],
 // full-length bin is f = 2 f' + h.
 template <typename C, int NH, int E, int R, typename Acc>
 __device__ __forceinline__ void pass_last(const FftPlan &P, int h0, C (&v)[NH][E], Acc (&acc)[2][E],
-                                          const C *buf, const C *__restrict__ tw,
+                                          const C *buf, const C *__restrict__ twp,
                                           const double2 *__restrict__ w, double &ar, double &ai, int tid) {
     if constexpr (E % R == 0) {
         const int Lh = P.Lh, T = P.T;
@@ -188,13 +188,13 @@ __device__ __forceinline__ void pass_last(const FftPlan &P, int h0, C (&v)[NH][E
                 for (int k = 0; k < NH; ++k) v[k][i * R + q] = buf[k * Lh + j + q * stride];
         }
         __syncthreads();  // the buffer may now be overwritten by the next sweep
-        const int tstep = P.L / Lh;  // L / (Ns R) with Ns R = Lh
 #pragma unroll
         for (int i = 0; i < E / R; ++i) {
             const int j = tid + i * T;  // k == j in the last pass
+            const C *tk = twp + P.toff[P.npass - 1] + j * R;
 #pragma unroll
             for (int q = 1; q < R; ++q) {
-                const C t = tw[q * j * tstep];
+                const C t = tk[q];
 #pragma unroll
                 for (int k = 0; k < NH; ++k) v[k][i * R + q] = cmul(v[k][i * R + q], t);
             }
@@ -203,11 +203,11 @@ __device__ __forceinline__ void pass_last(const FftPlan &P, int h0, C (&v)[NH][E
                 Dft<R, C>::run(&v[k][i * R]);
 #pragma unroll
                 for (int q = 0; q < R; ++q) {
-                    const int f = 2 * (j + q * stride) + h0 + k;
                     const C x = v[k][i * R + q];
                     const typename Cx<C>::R pw = x.x * x.x + x.y * x.y;
                     acc[h0 + k][i * R + q] += (Acc)pw;
-                    const double2 wf = w[f];
+                    // bin f = 2 (j + q stride) + h0 + k, weights in visiting order
+                    const double2 wf = w[((h0 + k) * R + q) * stride + j];
                     ar = fma(wf.x, (double)pw, ar);
                     ai = fma(wf.y, (double)pw, ai);
                 }
@@ -217,11 +217,11 @@ __device__ __forceinline__ void pass_last(const FftPlan &P, int h0, C (&v)[NH][E
 }
 
 // First pass: inputs come from a functor (HBM amplitudes or on-the-fly
-// phases); half 1 gets the pre-twiddle W_L^n = tw[n].  Stride Ns = 1, so the
+// phases); half 1 gets the pre-twiddle W_L^n = pre[n].  Stride Ns = 1, so the
 // pass itself needs no twiddles.
 template <typename C, int NH, int E, int R, typename Input>
 __device__ __forceinline__ void pass_first(const FftPlan &P, int h0, C (&v)[NH][E], C *buf,
-                                           const C *__restrict__ tw, int NF, const Input &in, int tid) {
+                                           const C *__restrict__ pre, int NF, const Input &in, int tid) {
     if constexpr (E % R == 0) {
         const int Lh = P.Lh, T = P.T;
         const int stride = Lh / R;
@@ -234,7 +234,7 @@ __device__ __forceinline__ void pass_first(const FftPlan &P, int h0, C (&v)[NH][
                 C x = Cx<C>::make(0, 0);
                 if (n < NF) x = in(n);
 #pragma unroll
-                for (int k = 0; k < NH; ++k) v[k][i * R + q] = (h0 + k == 0 || n >= NF) ? x : cmul(x, tw[n]);
+                for (int k = 0; k < NH; ++k) v[k][i * R + q] = (h0 + k == 0 || n >= NF) ? x : cmul(x, pre[n]);
             }
 #pragma unroll
             for (int k = 0; k < NH; ++k) {
@@ -250,12 +250,13 @@ __device__ __forceinline__ void pass_first(const FftPlan &P, int h0, C (&v)[NH][
 // One sweep: the Stockham passes of NH halves starting at half h0.
 template <typename C, int NH, int E, typename Acc, typename Input>
 __device__ __forceinline__ void fft_sweep(const FftPlan &P, int h0, C (&v)[NH][E], Acc (&acc)[2][E], C *buf,
-                                          const C *__restrict__ tw, const double2 *__restrict__ w, int NF,
-                                          const Input &in, double &ar, double &ai, int tid) {
+                                          const C *__restrict__ tw, const C *__restrict__ pre,
+                                          const double2 *__restrict__ w, int NF, const Input &in, double &ar,
+                                          double &ai, int tid) {
     switch (P.radix[0]) {
-        case 2: pass_first<C, NH, E, 2>(P, h0, v, buf, tw, NF, in, tid); break;
-        case 4: pass_first<C, NH, E, 4>(P, h0, v, buf, tw, NF, in, tid); break;
-        case 8: pass_first<C, NH, E, 8>(P, h0, v, buf, tw, NF, in, tid); break;
+        case 2: pass_first<C, NH, E, 2>(P, h0, v, buf, pre, NF, in, tid); break;
+        case 4: pass_first<C, NH, E, 4>(P, h0, v, buf, pre, NF, in, tid); break;
+        case 8: pass_first<C, NH, E, 8>(P, h0, v, buf, pre, NF, in, tid); break;
         default: break;
     }
     for (int p = 1; p < P.npass - 1; ++p) {
@@ -277,15 +278,17 @@ __device__ __forceinline__ void fft_sweep(const FftPlan &P, int h0, C (&v)[NH][E
 }
 
 // |X_f|^2 of one padded series accumulated into acc; a += sum_f w_f |X_f|^2.
+// tw: packed pass twiddles; pre: exp(-2 pi i n / L) for the odd bins.
 template <typename C, int NH, int E, typename Acc, typename Input>
 __device__ __forceinline__ void fft_power(const FftPlan &P, C (&v)[NH][E], Acc (&acc)[2][E], C *buf,
-                                          const C *__restrict__ tw, const double2 *__restrict__ w, int NF,
-                                          const Input &in, double &ar, double &ai, int tid) {
+                                          const C *__restrict__ tw, const C *__restrict__ pre,
+                                          const double2 *__restrict__ w, int NF, const Input &in, double &ar,
+                                          double &ai, int tid) {
     if constexpr (NH == 2) {
-        fft_sweep<C, 2, E>(P, 0, v, acc, buf, tw, w, NF, in, ar, ai, tid);
+        fft_sweep<C, 2, E>(P, 0, v, acc, buf, tw, pre, w, NF, in, ar, ai, tid);
     } else {
-        fft_sweep<C, 1, E>(P, 0, v, acc, buf, tw, w, NF, in, ar, ai, tid);
-        fft_sweep<C, 1, E>(P, 1, v, acc, buf, tw, w, NF, in, ar, ai, tid);
+        fft_sweep<C, 1, E>(P, 0, v, acc, buf, tw, pre, w, NF, in, ar, ai, tid);
+        fft_sweep<C, 1, E>(P, 1, v, acc, buf, tw, pre, w, NF, in, ar, ai, tid);
     }
 }
 
@@ -350,13 +353,23 @@ struct AmpInput {
 };
 
 template <typename C>
-__device__ __forceinline__ const C *twiddles(const CorrArgs &a);
+__device__ __forceinline__ const C *pass_twiddles(const CorrArgs &a);
 template <>
-__device__ __forceinline__ const float2 *twiddles<float2>(const CorrArgs &a) {
+__device__ __forceinline__ const float2 *pass_twiddles<float2>(const CorrArgs &a) {
+    return a.twp;
+}
+template <>
+__device__ __forceinline__ const double2 *pass_twiddles<double2>(const CorrArgs &a) {
+    return a.twp64;
+}
+template <typename C>
+__device__ __forceinline__ const C *pre_twiddles(const CorrArgs &a);
+template <>
+__device__ __forceinline__ const float2 *pre_twiddles<float2>(const CorrArgs &a) {
     return a.twL;
 }
 template <>
-__device__ __forceinline__ const double2 *twiddles<double2>(const CorrArgs &a) {
+__device__ __forceinline__ const double2 *pre_twiddles<double2>(const CorrArgs &a) {
     return a.twL64;
 }
 
@@ -372,7 +385,8 @@ __device__ __forceinline__ void corr_all_block(const CorrArgs &a, int s, int c, 
     // buffer behind a barrier, and ends with a barrier before the next series
     // writes it, so it can use the start of the FFT buffer.
     double2 *red = reinterpret_cast<double2 *>(buf);
-    const C *tw = twiddles<C>(a);
+    const C *tw = pass_twiddles<C>(a);
+    const C *pre = pre_twiddles<C>(a);
     const int tid = threadIdx.x;
     const int m0 = (int)((long)a.NM * c / a.chunks), m1 = (int)((long)a.NM * (c + 1) / a.chunks);
 
@@ -394,7 +408,7 @@ __device__ __forceinline__ void corr_all_block(const CorrArgs &a, int s, int c, 
         // would not fit in registers).
         int t = tid;
         asm volatile("" : "+v"(t));
-        fft_power<C, NH, E>(P, v, acc, buf, tw, a.w, a.NF, in, ar, ai, t);
+        fft_power<C, NH, E>(P, v, acc, buf, tw, pre, a.w, a.NF, in, ar, ai, t);
         const double2 r = block_sum2(ar, ai, red, tid, P.T);
         if (tid == 0) {
             sa_r += r.x;

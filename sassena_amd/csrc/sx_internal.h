@@ -35,6 +35,11 @@ struct FftPlan {
     int npass = 0;
     int radix[kMaxPasses] = {};
     int ns[kMaxPasses] = {};
+    // Packed twiddles of pass p >= 1 start at toff[p]: entry toff[p] + kk*R + q
+    // is exp(-2 pi i q kk / (Ns R)), kk < Ns, q < R (one contiguous run per
+    // thread and pass).  ntw is the table length.
+    int toff[kMaxPasses] = {};
+    int ntw = 0;
 };
 
 // Returns false if NF cannot be served by a supported FFT length.
@@ -82,9 +87,12 @@ int orient_shape_vs(int shape);
 struct CorrArgs {
     FftPlan plan;
     int NF, NM, nshells, chunks;
-    const float2 *twL;     // [L] exp(-2 pi i e / L), float
+    const float2 *twL;     // [L] exp(-2 pi i e / L), float (pre-twiddle of the odd bins)
     const double2 *twL64;  // [L] exp(-2 pi i e / L), double
-    const double2 *w;      // [L] weights of a = mean_tau C(tau)
+    const float2 *twp;     // [plan.ntw] packed pass twiddles, float
+    const double2 *twp64;  // [plan.ntw] packed pass twiddles, double
+    const double2 *w;      // [L] weights of a = mean_tau C(tau), in last-pass order (launch_weights)
+    const int *aidx;       // "self": atoms with a nonzero length (nullptr: all NA atoms)
     double *ppart;         // [nshells][chunks][L]
     double *apart;         // [nshells][chunks][4]
     // "all" inputs
@@ -94,9 +102,14 @@ struct CorrArgs {
     const double *qrev; const float *b; int brow_per_shell;
 };
 hipError_t launch_corr_all(const CorrArgs &a, hipStream_t stream);
-hipError_t launch_corr_self(const CorrArgs &a, hipStream_t stream);
+// halves: 1 transforms the two halves of a padded series one after the other
+// (half the LDS, twice the resident blocks), 2 both at once.
+hipError_t launch_corr_self(const CorrArgs &a, int halves, hipStream_t stream);
 
-hipError_t launch_weights(int L, int NF, double2 *w, hipStream_t stream);
+// w_f = (1/NF) sum_tau e^{+2 pi i f tau / L} / (L (NF - tau)), stored at
+// ((f & 1) R + q) Lh/R + j for f/2 = j + q Lh/R, R the last radix of the plan:
+// the order in which the last FFT pass visits the bins.
+hipError_t launch_weights(const FftPlan &P, int NF, double2 *w, hipStream_t stream);
 
 struct FinalArgs {
     int L, NF, NM, nshells, chunks;
