@@ -226,10 +226,73 @@ static CartesianCoor3D read_vec(const XMLElement *e, CartesianCoor3D def) {
     return {e->get("x", def.x), e->get("y", def.y), e->get("z", def.z)};
 }
 
+Database read_database_xml_text(const std::string &text) {
+    std::unique_ptr<XMLElement> root = parse_xml(text);
+    struct Entry {
+        bool has_sf = false;
+        ElementEntry e;
+    };
+    std::vector<std::string> names;
+    std::vector<Entry> entries;
+    auto slot = [&](const std::string &name) -> Entry & {
+        for (size_t i = 0; i < names.size(); ++i)
+            if (names[i] == name) return entries[i];
+        names.push_back(name);
+        entries.emplace_back();
+        return entries.back();
+    };
+    auto read_function = [](const XMLElement *el, DatabaseFunction &f) {
+        f.function_type = (int)el->get("type", 0L);
+        f.constants.clear();
+        for (const XMLElement *p : el->find_all("param")) {
+            char *end = nullptr;
+            const double v = std::strtod(p->text.c_str(), &end);
+            if (end == p->text.c_str()) throw terminate_request("database: bad <param> value '" + p->text + "'");
+            f.constants.push_back(v);
+        }
+    };
+    const XMLElement *sf = root->name == "scatterfactors" ? root.get() : root->find("scatterfactors");
+    if (!sf) throw terminate_request("Need scattering factors.");
+    for (const XMLElement *el : sf->find_all("element")) {
+        Entry &e = slot(el->get("name", std::string()));
+        read_function(el, e.e.sfactor);
+        e.has_sf = true;
+    }
+    if (const XMLElement *sz = root->find("sizes"))
+        for (const XMLElement *el : sz->find_all("element")) read_function(el, slot(el->get("name", std::string())).e.volume);
+    if (const XMLElement *ex = root->find("exclusionfactors"))
+        for (const XMLElement *el : ex->find_all("element"))
+            read_function(el, slot(el->get("name", std::string())).e.exclusion);
+    Database db;
+    for (size_t i = 0; i < names.size(); ++i) {
+        if (names[i].empty()) throw terminate_request("database: <element> without <name>");
+        if (!entries[i].has_sf) continue;  // only elements with a scattering factor can be looked up
+        db.add(names[i], entries[i].e);
+    }
+    return db;
+}
+
+Database read_database_xml(const std::string &path) {
+    std::ifstream in(path);
+    if (!in) throw terminate_request("cannot open database file: " + path);
+    std::stringstream ss;
+    ss << in.rdbuf();
+    return read_database_xml_text(ss.str());
+}
+
 void read_config_text(const std::string &text, const std::string &basedir, Params &params, RunFiles &files) {
     std::unique_ptr<XMLElement> root = parse_xml(text);
     const XMLElement &x = *root;
     ScatteringParameters &sc = params.scattering;
+
+    // database (parameters.cpp:775-800): type file, format xml
+    if (const XMLElement *d = x.find("database")) {
+        const std::string type = d->get("type", std::string("file"));
+        const std::string format = d->get("format", std::string("xml"));
+        if (type != "file") throw terminate_request("database.type not understood: " + type);
+        if (format != "xml") throw terminate_request("database.format not understood: " + format);
+        files.database_file = resolve(basedir, d->get("file", std::string("db.xml")));
+    }
 
     // sample (structure and trajectory of the target)
     files.structure_file = resolve(basedir, x.get("sample/structure/file", std::string()));

@@ -38,6 +38,7 @@ struct RunFiles {
     std::string structure_file, structure_format = "elements";  // sample.structure
     std::string frames_file, frames_format = "raw";             // sample.framesets.frameset
     std::string signal_file = "signal.h5";                      // scattering.signal.file
+    std::string database_file;  // database.file when given; else db.xml beside the configuration if present
     bool fqt = true, fq0 = true, fq = true, fq2 = true;
 };
 
@@ -45,5 +46,12 @@ struct RunFiles {
 // q vectors and orientation vectors as the reference does) and files.
 void read_config(const std::string &path, Params &params, RunFiles &files);
 void read_config_text(const std::string &text, const std::string &basedir, Params &params, RunFiles &files);
+
+// Database::read_xml (database.cpp:31-145): elements of //scatterfactors
+// (required), //sizes and //exclusionfactors, each <element> with <name>,
+// <type> and a list of <param>.  Masses and PDB name patterns are not used
+// on this path and are skipped.
+Database read_database_xml(const std::string &path);
+Database read_database_xml_text(const std::string &text);
 
 }  // namespace sassena_amd

@@ -270,7 +270,16 @@ HDF5SignalWriter::HDF5SignalWriter(const std::string &file, size_t NF, const Run
     hsize_t rawsize = rawconfig.size();
     if (rawsize) H5LTmake_dataset_char(meta, "rawconfig", 1, &rawsize, rawconfig.data());
     H5LTmake_dataset_string(meta, "config", "sassena_amd scattering-device hot path");
-    H5LTmake_dataset_string(meta, "database", "neutron coherent scattering lengths (fm)");
+    // meta/database holds the raw database file, as meta/rawconfig the configuration
+    std::string dbtext = "built-in neutron coherent scattering lengths (fm)";
+    if (!flags.database_file.empty()) {
+        std::ifstream in(flags.database_file);
+        std::stringstream ss;
+        ss << in.rdbuf();
+        dbtext = ss.str();
+    }
+    hsize_t dbsize = dbtext.size();
+    if (dbsize) H5LTmake_dataset_char(meta, "database", 1, &dbsize, dbtext.data());
     H5Gclose(meta);
     const hsize_t chunk = 10000;  // limits.signal.chunksize default (parameters.cpp:615)
     hsize_t three = 3, two = 2, nf2[2] = {NF, 2};

@@ -9,6 +9,8 @@
 //   sx_run --convert-trajectory in fmt out.raw   re-read a trajectory
 //   sx_run --write-test-signal file NF nq  exercise a signal writer
 //   sx_run --structure file.pdb            print the elements of a PDB
+//   sx_run --database db.xml Q BG E1 E2 ...  print the scattering factors of the listed
+//                                            elements at |q| = Q, background factor BG
 //
 // Structure formats: "elements" (one element symbol per line, the target
 // atoms in order).  Trajectory formats: "raw" (SXRAW1 header, uint64 NA, NF,
@@ -69,10 +71,22 @@ int main(int argc, char **argv) {
             w->flush();
             return 0;
         }
+        if (argc >= 5 && !std::strcmp(argv[1], "--database")) {
+            Database db = read_database_xml(argv[2]);
+            const double q = std::stod(argv[3]), bg = std::stod(argv[4]);
+            std::vector<size_t> ids;
+            for (int i = 5; i < argc; ++i) ids.push_back(db.id(argv[i]));
+            ScatterFactors sf(db, ids, bg);
+            for (double b : sf.update(q)) std::printf("%.17g\n", b);
+            return 0;
+        }
         if (argc == 3 && !std::strcmp(argv[1], "--structure")) {
             for (const auto &e : read_structure(argv[2], "pdb")) std::printf("%s\n", e.c_str());
             return 0;
         }
+    } catch (const terminate_request &e) {
+        std::fprintf(stderr, "sx_run: terminate request: %s\n", e.what());
+        return 1;
     } catch (const std::exception &e) {
         std::fprintf(stderr, "sx_run: %s\n", e.what());
         return 1;
@@ -99,7 +113,16 @@ int main(int argc, char **argv) {
         if (src->atoms() != elements.size())
             throw terminate_request("structure has " + std::to_string(elements.size()) + " atoms, trajectory " +
                                     std::to_string(src->atoms()));
-        Database db = Database::neutron_coherent();
+        // the database named in the configuration, else db.xml beside it,
+        // else the built-in neutron coherent lengths
+        std::string dbfile = files.database_file;
+        if (dbfile.empty()) {
+            const size_t slash = config.find_last_of('/');
+            const std::string beside = (slash == std::string::npos ? std::string() : config.substr(0, slash + 1)) + "db.xml";
+            if (std::ifstream(beside).good()) dbfile = beside;
+        }
+        Database db = dbfile.empty() ? Database::neutron_coherent() : read_database_xml(dbfile);
+        files.database_file = dbfile;
         std::vector<size_t> ids;
         for (const auto &e : elements) ids.push_back(db.id(e));
         ScatterFactors sf(db, ids, params.scattering.background_factor);

@@ -171,3 +171,53 @@ def test_multipole_moment_generation(tmp_path, oracle_mod):
     assert res.returncode == 1 and "moments.type" in res.stderr
     res = moments(tmp_path, MP.format(t="cone", m="<type>resolution</type>"))
     assert res.returncode == 1 and "multipole.type" in res.stderr
+
+
+DB = """<?xml version="1.0"?>
+<root>
+  <masses><element><name>C</name><param>12.011</param></element></masses>
+  <sizes>
+    <element><name>C</name><type>1</type><param>1.7</param></element>
+    <element><name>H</name><type>0</type><param>5.0</param></element>
+  </sizes>
+  <exclusionfactors>
+    <element><name>C</name><type>1</type><param>0.334</param></element>
+    <element><name>H</name><type>0</type><param>0.0</param></element>
+  </exclusionfactors>
+  <scatterfactors>
+    <element><name>H</name><type>0</type><param>-3.7390</param></element>
+    <element><name>C</name><type>2</type>
+      <param>2.31</param><param>20.8439</param><param>1.02</param><param>10.2075</param>
+      <param>1.5886</param><param>0.5687</param><param>0.865</param><param>51.6512</param><param>0.2156</param>
+    </element>
+  </scatterfactors>
+</root>"""
+
+
+def test_database_xml(tmp_path):
+    """database.cpp:31-145 layout; type 0 is a constant length, type 2 the
+    four-Gaussian form whose q=0 value is the electron count (C: 6); the
+    background term is factor x exclusion(kappa V, q) (scatter_factors.cpp:56-78)."""
+    (tmp_path / "db.xml").write_text(DB)
+    out = sx_run("--database", str(tmp_path / "db.xml"), "0", "0", "H", "C", "H").stdout.split()
+    h, c, h2 = (float(v) for v in out)
+    assert h == -3.739 and h2 == h
+    assert abs(c - 6.0) < 1e-3
+    # at q = 0 with background 0.5: the carbon sphere volume (4/3 pi 1.7^3,
+    # radius cubed in float as the reference) times 0.334 is subtracted
+    out = sx_run("--database", str(tmp_path / "db.xml"), "0", "0.5", "C", "H").stdout.split()
+    vol = 4.0 / 3.0 * np.pi * float(np.float32(np.float32(1.7) ** 3))
+    assert abs(float(out[0]) - (c - 0.5 * 0.334 * vol)) < 1e-5
+    assert float(out[1]) == h
+    # q dependence of the Gaussian form
+    q = 1.3
+    a = [2.31, 20.8439, 1.02, 10.2075, 1.5886, 0.5687, 0.865, 51.6512, 0.2156]
+    ref = sum(a[2 * i] * np.exp(-a[2 * i + 1] * q * q) for i in range(4)) + a[8]
+    out = sx_run("--database", str(tmp_path / "db.xml"), str(q), "0", "C").stdout.split()
+    assert abs(float(out[0]) - ref) < 1e-12 * abs(ref)
+    # errors: unknown element, missing scattering factors
+    res = sx_run("--database", str(tmp_path / "db.xml"), "0", "0", "Xe", check=False)
+    assert res.returncode == 1 and "not in database" in res.stderr
+    (tmp_path / "bad.xml").write_text("<root><sizes/></root>")
+    res = sx_run("--database", str(tmp_path / "bad.xml"), "0", "0", "H", check=False)
+    assert res.returncode == 1 and "Need scattering factors" in res.stderr
