@@ -423,7 +423,7 @@ int run_batch(sx_ctx *c, int ns, size_t NM, const double *qv, const double *qlen
         ca.qrev = c->qrev.as<double>();
         ca.b = bptr;
         ca.brow_per_shell = b_shell ? 1 : 0;
-        SX_HIP(c, sx::launch_corr_self(ca, c->cfg.fft_halves == 2 ? 2 : 1, c->stream));
+        SX_HIP(c, sx::launch_corr_self(ca, c->cfg.fft_halves == 1 ? 1 : 2, c->stream));
         c->timing.correlate_launches++;
         fa.chunks = chunks;
         fa.apart = c->apart.as<double>();

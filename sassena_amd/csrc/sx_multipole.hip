@@ -71,6 +71,41 @@ __device__ double sph_bessel(int l, double x) {
     return fabs(j0) >= fabs(j1) ? fl * (j0 / f0) : fl * (j1 / f1);
 }
 
+// J_n(x) for n >= 0, x >= 0: upward recurrence from J_0, J_1 where x > n,
+// else Miller's downward recurrence normalised by J_0 + 2 sum_k J_2k = 1.
+__device__ double cyl_bessel(int n, double x) {
+    if (n == 0) return j0(x);
+    if (n == 1) return j1(x);
+    if (x == 0.0) return 0.0;
+    if (x > (double)n) {
+        double a = j0(x), bb = j1(x);
+        for (int k = 1; k < n; ++k) {
+            const double nx = (2.0 * k) / x * bb - a;
+            a = bb;
+            bb = nx;
+        }
+        return bb;
+    }
+    int N = n + 32 + (int)(2.0 * sqrt((double)n + x));
+    N += N & 1;  // even start keeps the normalisation sum on even orders
+    double fp1 = 0.0, f = 1e-280, fn = 0.0, norm = 0.0;
+    for (int k = N; k > 0; --k) {
+        const double fm1 = (2.0 * k) / x * f - fp1;
+        fp1 = f;
+        f = fm1;  // f = J_{k-1} (unnormalised)
+        if (k - 1 == n) fn = f;
+        if (((k - 1) & 1) == 0 && k - 1 > 0) norm += 2.0 * f;
+        if (fabs(f) > 1e250) {
+            f *= 1e-250;
+            fp1 *= 1e-250;
+            fn *= 1e-250;
+            norm *= 1e-250;
+        }
+    }
+    norm += f;  // J_0
+    return fn / norm;
+}
+
 // sqrt((2l+1)/(4 pi) (l-m)!/(l+m)!) P_l^m(x), m >= 0, Condon-Shortley phase.
 __device__ double ylm_theta(int l, int m, double x) {
     const double somx2 = sqrt((1.0 - x) * (1.0 + x));
@@ -219,15 +254,15 @@ __global__ __launch_bounds__(256) void multipole_kernel(MultipoleKernelArgs a) {
                 double cr = 0.0, ci = 0.0;
                 const double sgl = (l & 1) ? -1.0 : 1.0;
                 if (m == 0 && l == 0) {
-                    cr = jn(0, p);
+                    cr = cyl_bessel(0, p);
                 } else if (m == 0) {
-                    cr = sqrt(0.5) * 2.0 * sgl * jn(2 * l, p) * cos(2 * l * psi);
+                    cr = sqrt(0.5) * 2.0 * sgl * cyl_bessel(2 * l, p) * cos(2 * l * psi);
                 } else if (m == 1) {
-                    cr = sqrt(0.5) * 2.0 * sgl * jn(2 * l, p) * sin(2 * l * psi);
+                    cr = sqrt(0.5) * 2.0 * sgl * cyl_bessel(2 * l, p) * sin(2 * l * psi);
                 } else if (m == 2) {
-                    ci = sqrt(0.5) * 2.0 * (-sgl) * jn(2 * l - 1, p) * cos((2 * l - 1) * psi);
+                    ci = sqrt(0.5) * 2.0 * (-sgl) * cyl_bessel(2 * l - 1, p) * cos((2 * l - 1) * psi);
                 } else if (m == 3) {
-                    ci = sqrt(0.5) * 2.0 * (-sgl) * jn(2 * l - 1, p) * sin((2 * l - 1) * psi);
+                    ci = sqrt(0.5) * 2.0 * (-sgl) * cyl_bessel(2 * l - 1, p) * sin((2 * l - 1) * psi);
                 }
                 ar[k] += (double)bj * (cr * exc - ci * exs);
                 ai[k] += (double)bj * (cr * exs + ci * exc);
