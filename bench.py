@@ -33,6 +33,11 @@ TRANS_PEAK = 256 * 64 * 2.4e9 / 4.0   # transcendental ops/s
 AMP_TRAFFIC_BYTES = 2.35e9 + 0.131e9
 
 
+def log(msg):
+    """Progress of the long workloads, on stderr."""
+    print(msg, file=sys.stderr, flush=True)
+
+
 def cpu_model():
     try:
         with open("/proc/cpuinfo") as f:
@@ -178,8 +183,10 @@ def bench_c4(args):
 def bench_c5(args):
     """100,000 atoms in a 100 A box, sigma 0.1 walk over 10,000 frames, 60
     shells over 0.05-5.0 A^-1, 1,000 orientations: the network is trained on
-    frames [0, 2000) (1,000- and 16-orientation estimates there) and corrects
-    the 16-orientation estimate on frames [2000, 10000), judged against the
+    frames [0, 2000) (1,000- and 16-orientation estimates there), validated
+    out of sample per shell, and corrects the 16-orientation estimate on
+    frames [2000, 10000) where validation meets the tolerance; other shells
+    fall back to the plain 1,000-orientation average.  Judged against the
     plain 1,000-orientation average on those frames."""
     from sassena_amd import cv, synth
     rng = np.random.default_rng(505)
@@ -192,19 +199,24 @@ def bench_c5(args):
     orient = synth.orientations(rng, 1000)
     qlen = synth.linear_shells(0.05, 5.0, 60)
     gen_s = time.perf_counter() - t0
-    r = cv.run(xyz, b, qlen, orient, ntrain_frames=NF // 5, steps=3000, batch=4096)
+    log(f"c5: generated {NA} atoms x {NF} frames in {gen_s:.1f} s")
+    r = cv.run(xyz, b, qlen, orient, ntrain_frames=NF // 5, tol=args.c5_tol, steps=3000, batch=4096,
+               progress=lambda stage, sec: log(f"c5: {stage}: {sec:.2f} s"))
     tm = r["timing_s"]
-    cv_eval = tm["small_eval"] + tm["predict"]
+    cv_eval = tm["small_eval"] + tm["predict"] + tm["fallback"]
     cv_total = cv_eval + tm["many_train"] + tm["small_train"] + tm["train"]
     out = {
-        "metric": "F(q,t) shell error of the learned control variate vs the 1000-orientation average, and wall time",
+        "metric": "wall-time ratio of the plain 1000-orientation average to the learned control variate "
+                  "(with fallback) on the evaluated frames, at the stated shell error",
         "value": tm["many_eval"] / cv_eval, "unit": "x less wall time on the evaluated frames",
         "n_gpus": 1, "higher_is_better": True, "data": "synthetic",
-        "config": {"workload": "learned control variate", "NA": NA, "NF": NF, "shells": 60, "orientations": 1000,
-                   "small_orientations": 16, "train_frames": NF // 5, "generation_s": gen_s},
-        "timing_s": tm, "speedup_including_training": tm["many_eval"] / cv_total,
-        "error_small": r["error_small"], "error_cv": r["error_cv"], "bias_cv": r["bias_cv"],
-        "train_loss": r["train_loss"],
+        "config": {"workload": "learned control variate, 100000 atoms x 10000 frames, 60 shells, "
+                               "1000 vs 16 orientations", "NA": NA, "NF": NF, "shells": 60, "orientations": 1000,
+                   "small_orientations": 16, "train_frames": NF // 5, "tol": args.c5_tol, "generation_s": gen_s},
+        "cv_shells": r["cv_shells"], "timing_s": tm, "speedup_including_training": tm["many_eval"] / cv_total,
+        "error_small": r["error_small"], "error_net_all_shells": r["error_net"], "error_cv": r["error_cv"],
+        "bias_net": r["bias_net"], "train_loss": r["train_loss"],
+        "speedup_net_all_shells": tm["many_eval"] / (tm["small_eval"] + tm["predict"]),
     }
     print(json.dumps(out), flush=True)
 
@@ -223,6 +235,9 @@ def main():
                          "c5: learned control variate against the 1,000-orientation average")
     ap.add_argument("--c5-scale", type=float, default=1.0, help="shrink the c5 atoms and frames (tests)")
     ap.add_argument("--c4-scale", type=float, default=1.0, help="shrink the c4 trajectories (tests)")
+    ap.add_argument("--c5-tol", type=float, default=1e-3,
+                    help="c5: shell error (mean over lags, relative to F(q,0)) a shell must meet in validation "
+                         "to use the control variate")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (counter calibration) and exit")
     ap.add_argument("--generic", action="store_true",

@@ -135,11 +135,22 @@ template <typename C, int NH, int E, int R>
 __device__ __forceinline__ void pass_mid(const FftPlan &P, int p, C (&v)[NH][E], C *buf,
                                          const C *__restrict__ twp, int tid) {
     if constexpr (E % R == 0) {
-        const int Lh = P.Lh, T = P.T, Ns = P.ns[p];
-        const int stride = Lh / R;
+        const unsigned Lh = P.Lh, T = P.T, Ns = P.ns[p];
+        const unsigned stride = Lh / R;
+        const unsigned ut = (unsigned)tid;
+        // With Ns <= T (both powers of two) every group of this thread has the
+        // same twiddle index kk = tid mod Ns: load its R-1 twiddles once, and
+        // before the barrier, so the load latency hides behind it.
+        const bool shared = Ns <= T;
+        C ws[R];
+        if (shared) {
+            const C *tk = twp + P.toff[p] + (ut & (Ns - 1)) * R;
+#pragma unroll
+            for (int q = 1; q < R; ++q) ws[q] = tk[q];
+        }
 #pragma unroll
         for (int i = 0; i < E / R; ++i) {
-            const int j = tid + i * T;
+            const unsigned j = ut + i * T;
 #pragma unroll
             for (int q = 0; q < R; ++q)
 #pragma unroll
@@ -148,16 +159,23 @@ __device__ __forceinline__ void pass_mid(const FftPlan &P, int p, C (&v)[NH][E],
         __syncthreads();
 #pragma unroll
         for (int i = 0; i < E / R; ++i) {
-            const int j = tid + i * T;
-            const int kk = j & (Ns - 1);
-            const C *tk = twp + P.toff[p] + kk * R;
+            const unsigned j = ut + i * T;
+            const unsigned kk = j & (Ns - 1);
+            if (shared) {
 #pragma unroll
-            for (int q = 1; q < R; ++q) {
-                const C w = tk[q];
+                for (int q = 1; q < R; ++q)
 #pragma unroll
-                for (int k = 0; k < NH; ++k) v[k][i * R + q] = cmul(v[k][i * R + q], w);
+                    for (int k = 0; k < NH; ++k) v[k][i * R + q] = cmul(v[k][i * R + q], ws[q]);
+            } else {
+                const C *tk = twp + P.toff[p] + kk * R;
+#pragma unroll
+                for (int q = 1; q < R; ++q) {
+                    const C w = tk[q];
+#pragma unroll
+                    for (int k = 0; k < NH; ++k) v[k][i * R + q] = cmul(v[k][i * R + q], w);
+                }
             }
-            const int o = (j - kk) * R + kk;
+            const unsigned o = (j - kk) * R + kk;
 #pragma unroll
             for (int k = 0; k < NH; ++k) {
                 Dft<R, C>::run(&v[k][i * R]);
@@ -177,11 +195,12 @@ __device__ __forceinline__ void pass_last(const FftPlan &P, int h0, C (&v)[NH][E
                                           const C *buf, const C *__restrict__ twp,
                                           const double2 *__restrict__ w, double &ar, double &ai, int tid) {
     if constexpr (E % R == 0) {
-        const int Lh = P.Lh, T = P.T;
-        const int stride = Lh / R;  // == Ns of this pass
+        const unsigned Lh = P.Lh, T = P.T;
+        const unsigned stride = Lh / R;  // == Ns of this pass
+        const unsigned ut = (unsigned)tid;
 #pragma unroll
         for (int i = 0; i < E / R; ++i) {
-            const int j = tid + i * T;
+            const unsigned j = ut + i * T;
 #pragma unroll
             for (int q = 0; q < R; ++q)
 #pragma unroll
@@ -190,7 +209,7 @@ __device__ __forceinline__ void pass_last(const FftPlan &P, int h0, C (&v)[NH][E
         __syncthreads();  // the buffer may now be overwritten by the next sweep
 #pragma unroll
         for (int i = 0; i < E / R; ++i) {
-            const int j = tid + i * T;  // k == j in the last pass
+            const unsigned j = ut + i * T;  // k == j in the last pass
             const C *tk = twp + P.toff[P.npass - 1] + j * R;
 #pragma unroll
             for (int q = 1; q < R; ++q) {
@@ -207,7 +226,7 @@ __device__ __forceinline__ void pass_last(const FftPlan &P, int h0, C (&v)[NH][E
                     const typename Cx<C>::R pw = x.x * x.x + x.y * x.y;
                     acc[h0 + k][i * R + q] += (Acc)pw;
                     // bin f = 2 (j + q stride) + h0 + k, weights in visiting order
-                    const double2 wf = w[((h0 + k) * R + q) * stride + j];
+                    const double2 wf = w[((unsigned)(h0 + k) * R + q) * stride + j];
                     ar = fma(wf.x, (double)pw, ar);
                     ai = fma(wf.y, (double)pw, ai);
                 }
@@ -223,18 +242,19 @@ template <typename C, int NH, int E, int R, typename Input>
 __device__ __forceinline__ void pass_first(const FftPlan &P, int h0, C (&v)[NH][E], C *buf,
                                            const C *__restrict__ pre, int NF, const Input &in, int tid) {
     if constexpr (E % R == 0) {
-        const int Lh = P.Lh, T = P.T;
-        const int stride = Lh / R;
+        const unsigned Lh = P.Lh, T = P.T;
+        const unsigned stride = Lh / R;
+        const unsigned unf = (unsigned)NF;
 #pragma unroll
         for (int i = 0; i < E / R; ++i) {
-            const int j = tid + i * T;
+            const unsigned j = (unsigned)tid + i * T;
 #pragma unroll
             for (int q = 0; q < R; ++q) {
-                const int n = j + q * stride;
+                const unsigned n = j + q * stride;
                 C x = Cx<C>::make(0, 0);
-                if (n < NF) x = in(n);
+                if (n < unf) x = in(n);
 #pragma unroll
-                for (int k = 0; k < NH; ++k) v[k][i * R + q] = (h0 + k == 0 || n >= NF) ? x : cmul(x, pre[n]);
+                for (int k = 0; k < NH; ++k) v[k][i * R + q] = (h0 + k == 0 || n >= unf) ? x : cmul(x, pre[n]);
             }
 #pragma unroll
             for (int k = 0; k < NH; ++k) {
@@ -341,7 +361,7 @@ struct AmpInput {
     const float2 *p;
     size_t split_stride;
     int split;
-    __device__ __forceinline__ C operator()(int n) const {
+    __device__ __forceinline__ C operator()(unsigned n) const {
         float2 s = p[n];
         for (int k = 1; k < split; ++k) {
             const float2 t = p[(size_t)k * split_stride + n];
@@ -438,7 +458,7 @@ struct SelfInput {
     const float *X, *Y, *Z;
     double qx, qy, qz;
     float b;
-    __device__ __forceinline__ float2 operator()(int n) const {
+    __device__ __forceinline__ float2 operator()(unsigned n) const {
         double p = qx * (double)X[n];
         p = fma(qy, (double)Y[n], p);
         p = fma(qz, (double)Z[n], p);

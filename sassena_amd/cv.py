@@ -1,29 +1,29 @@
-"""Learned control variate for the orientational average.
+"""Learned control variate for the orientational average, with the plain
+many-orientation average as fallback.
 
 The plain estimate of a coherent shell average uses NM orientations (1,000 in
-the large case); its cost is linear in NM.  Here a small network, trained on
-the GPU by libsassena_amd.so's own MFMA kernels, maps cheap features of a
-shell and lag -- |q|, log lag, the 16-orientation estimate F16(q, tau) and
-its neighbouring shells -- to the many-orientation average.  It is trained on
-the first part of the trajectory, where both estimates are computed.
+the large case); its cost is linear in NM.  A cheap estimate F16 uses the
+first 16 of them.  Where the orientational sampling error of F16 persists in
+time -- the anisotropy of a crystal, of the simulation box at small |q| --
+the correction D = F_many - F16 measured on early frames predicts the
+correction on later ones, and a small network learns how much of it carries
+over, per shell and lag.
 
-On the remaining frames the network output g is used as a control variate of
-the 16-orientation estimate,
+Training (the first ``ntrain_frames`` frames, split in halves A and B; both
+estimates are computed on each half):
 
-    F_cv(q, tau) = F16(q, tau) + c_q (g(q, tau) - F16(q, tau)),
+    x = features(F16_B, D_A),  y = D_B / s_q
 
-with the coefficient c_q in [0, 1] fitted per shell on held-out training
-frames (the network is trained on the first half of the training frames and
-c_q is the least-squares coefficient on the second half).  c_q = 0 returns
-the plain 16-orientation estimate, so the network only acts where it is
-shown to reduce the error on data it was not trained on.  The plain
-many-orientation average stays the fallback and is what the result is judged
-against (``evaluate``).
+where s_q is the shell scale |F_many(q, 0)| of the training halves.  The
+network is trained by libsassena_amd.so's own MFMA kernels.  It is then
+validated out of sample in the reverse direction (features of F16_A and D_B,
+against D_A): a shell whose validation error exceeds ``tol`` falls back to
+the plain many-orientation average on the evaluation frames.
 
-All values are normalised by nb = sum_j b_j^2 (the large-|q| limit of
-F(q, 0) for uncorrelated positions), which is known exactly from the
-scattering factors.  Lags beyond the training range are presented to the
-network at the last trained lag.
+Evaluation (the remaining frames): F_cv = F16 + s_q g(F16_E, D_B) on the
+shells that passed, the plain average elsewhere.  ``run`` also computes the
+plain many-orientation average on every shell of the evaluation frames, the
+judge of the error, and times each stage.
 """
 from __future__ import annotations
 
@@ -37,27 +37,26 @@ from .mlp import MLP
 NFEAT = 16
 
 
-def features(qlen, f16, nb, tmax, qmax):
-    """Rows (shell, lag) of the network input, zero padded to 16 columns:
-    |q|/qmax, log1p(tc)/log1p(tmax), tc/tmax (tc = lag clamped to tmax - 1),
-    Re F16/nb, Im F16/nb, Re F16(q,0)/nb, the mean Re F16/nb of the two
-    neighbouring shells, (|q|/qmax)^2 log1p(tc)/log1p(tmax)."""
+def features(qlen, f16, dprev, scale, nb, tmax, qmax):
+    """Rows (shell, lag), zero padded to 16 columns: |q|/qmax, log1p(tc)/
+    log1p(tmax), tc/tmax (tc = lag clamped to tmax - 1), Re and Im F16/s,
+    Re F16(q,0)/s, Re and Im D_prev(q, tc)/s, Re D_prev(q, 0)/s and
+    log10(s/nb)/4, with s the shell scale."""
     ns, NF = f16.shape
-    tc = np.minimum(np.arange(NF, dtype=np.float64), tmax - 1.0)
-    q = np.repeat(np.asarray(qlen, np.float64) / qmax, NF)
-    lt = np.tile(np.log1p(tc) / np.log1p(tmax), ns)
-    re = f16.real / nb
-    up = re[np.minimum(np.arange(ns) + 1, ns - 1)]
-    dn = re[np.maximum(np.arange(ns) - 1, 0)]
+    tcl = np.minimum(np.arange(NF), int(tmax) - 1)
+    s = np.asarray(scale, np.float64)[:, None]
+    dp = dprev[:, tcl] / s
     x = np.zeros((ns * NF, NFEAT), np.float32)
-    x[:, 0] = q
-    x[:, 1] = lt
-    x[:, 2] = np.tile(tc / tmax, ns)
-    x[:, 3] = re.ravel()
-    x[:, 4] = (f16.imag / nb).ravel()
-    x[:, 5] = np.repeat(re[:, 0], NF)
-    x[:, 6] = (0.5 * (up + dn)).ravel()
-    x[:, 7] = q * q * lt
+    x[:, 0] = np.repeat(np.asarray(qlen, np.float64) / qmax, NF)
+    x[:, 1] = np.tile(np.log1p(tcl) / np.log1p(tmax), ns)
+    x[:, 2] = np.tile(tcl / tmax, ns)
+    x[:, 3] = (f16.real / s).ravel()
+    x[:, 4] = (f16.imag / s).ravel()
+    x[:, 5] = np.repeat(f16[:, 0].real / s[:, 0], NF)
+    x[:, 6] = dp.real.ravel()
+    x[:, 7] = dp.imag.ravel()
+    x[:, 8] = np.repeat(dprev[:, 0].real / s[:, 0], NF)
+    x[:, 9] = np.repeat(np.log10(np.maximum(s[:, 0], 1e-300) / nb) / 4.0, NF)
     return x
 
 
@@ -67,80 +66,91 @@ def shell_errors(f, ref):
     return d.max(axis=1), d.mean(axis=1)
 
 
-def cv_coefficients(g, f16, fmany):
-    """Least-squares c_q of fmany - f16 ~ c_q (g - f16) per shell (real
-    parts), clipped to [0, 1]."""
-    d = g - f16.real
-    num = np.sum(d * (fmany.real - f16.real), axis=1)
-    den = np.sum(d * d, axis=1)
-    c = np.where(den > 0, num / np.where(den > 0, den, 1.0), 0.0)
-    return np.clip(c, 0.0, 1.0)
-
-
-def run(xyz, b, qlen, u, ntrain_frames, n_small=16, steps=3000, batch=4096, lr=2e-3, seed=1, device=0):
+def run(xyz, b, qlen, u, ntrain_frames, n_small=16, tol=1e-3, steps=3000, batch=4096, lr=2e-3, seed=1,
+        device=0, progress=None):
     """The full pipeline on one trajectory (xyz [3][NA][NF] float32).
 
-    Returns a dict with the estimates on the evaluation frames (the frames
-    after ntrain_frames), the many-orientation reference there, timings
-    (seconds) of each stage, the per-shell coefficients and the errors of
-    both estimates."""
+    Returns the estimates on the evaluation frames (the frames after
+    ntrain_frames) -- f_cv (learned control variate with fallback), f_net
+    (control variate on every shell), f_small, f_many (the judge) -- the
+    shells that use the control variate, timings (seconds) of each stage and
+    the errors.  ``progress(stage, seconds)`` is called after each stage."""
+    note = progress or (lambda stage, sec: None)
     qlen = np.asarray(qlen, np.float64)
     u = np.asarray(u, np.float64)
     nb = float(np.sum(np.asarray(b, np.float64) ** 2))
-    NF = xyz.shape[2]
     qmax = float(np.max(qlen))
     half = ntrain_frames // 2
     seg_a = np.ascontiguousarray(xyz[:, :, :half])
-    seg_b = np.ascontiguousarray(xyz[:, :, half:ntrain_frames])
+    seg_b = np.ascontiguousarray(xyz[:, :, half:2 * half])
     evalf = np.ascontiguousarray(xyz[:, :, ntrain_frames:])
     tmax = float(half)
     usmall = u[:n_small]
-    tm = {}
+    tm = {"many_train": 0.0, "small_train": 0.0}
 
-    def shells(seg, orient):
+    def shells(seg, orient, sel=None):
+        q = qlen if sel is None else qlen[sel]
         with ScatterDevice("all", device=device) as dev:
             dev.stage(seg, layout="soa")
             dev.set_factors(b)
             t0 = time.perf_counter()
-            fqt, _, _ = dev.compute_oriented(qlen, orient)
+            fqt, _, _ = dev.compute_oriented(q, orient)
             return fqt, time.perf_counter() - t0
 
-    # training data: both estimates on the first half of the training frames
-    f_many_a, t1 = shells(seg_a, u)
-    f_small_a, t2 = shells(seg_a, usmall)
-    x_a = features(qlen, f_small_a, nb, tmax, qmax)
-    y_a = (f_many_a.real / nb).ravel().astype(np.float32)
-    net = MLP(device=device, seed=seed)
-    t0 = time.perf_counter()
-    loss = net.train(x_a, y_a, steps=steps, batch=batch, lr=lr, seed=seed)
-    tm["train"] = time.perf_counter() - t0
-    # control-variate coefficients on the held-out second half
-    f_many_b, t3 = shells(seg_b, u)
-    f_small_b, t4 = shells(seg_b, usmall)
-    g_b = nb * net.predict(features(qlen, f_small_b, nb, tmax, qmax)).reshape(f_small_b.shape).astype(np.float64)
-    coef = cv_coefficients(g_b, f_small_b, f_many_b)
-    tm["many_train"] = t1 + t3
-    tm["small_train"] = t2 + t4
+    f_many = {}
+    f_small = {}
+    for name, seg in (("A", seg_a), ("B", seg_b)):
+        f_many[name], t = shells(seg, u)
+        tm["many_train"] += t
+        note(f"many-orientation average, training half {name}", t)
+        f_small[name], t = shells(seg, usmall)
+        tm["small_train"] += t
+    d_a = f_many["A"] - f_small["A"]
+    d_b = f_many["B"] - f_small["B"]
+    scale = 0.5 * (np.abs(f_many["A"][:, 0]) + np.abs(f_many["B"][:, 0]))
+    scale = np.where(scale > 0, scale, nb)
 
-    # evaluation frames: the cheap estimate and its correction
-    f_small_ev, tm["small_eval"] = shells(evalf, usmall)
+    net = MLP(device=device, seed=seed)
+    x_tr = features(qlen, f_small["B"], d_a, scale, nb, tmax, qmax)
+    y_tr = (d_b.real / scale[:, None]).ravel().astype(np.float32)
     t0 = time.perf_counter()
-    g_ev = nb * net.predict(features(qlen, f_small_ev, nb, tmax, qmax)).reshape(f_small_ev.shape).astype(np.float64)
-    f_cv = f_small_ev + coef[:, None] * (g_ev - f_small_ev.real)
+    loss = net.train(x_tr, y_tr, steps=steps, batch=batch, lr=lr, seed=seed)
+    tm["train"] = time.perf_counter() - t0
+    note("network training", tm["train"])
+    # out-of-sample validation in the reverse direction, per shell
+    g_val = net.predict(features(qlen, f_small["A"], d_b, scale, nb, tmax, qmax)).reshape(d_a.shape)
+    f_val = f_small["A"] + scale[:, None] * g_val
+    _, e_val = shell_errors(f_val, f_many["A"])
+    use_cv = e_val <= tol
+
+    # evaluation frames: the cheap estimate, its correction, the fallback shells
+    f_small_ev, tm["small_eval"] = shells(evalf, usmall)
+    note("16-orientation estimate, evaluation frames", tm["small_eval"])
+    t0 = time.perf_counter()
+    g_ev = net.predict(features(qlen, f_small_ev, d_b, scale, nb, tmax, qmax)).reshape(f_small_ev.shape)
+    f_net = f_small_ev + scale[:, None] * g_ev.astype(np.float64)
     tm["predict"] = time.perf_counter() - t0
-    # the judge: the plain many-orientation average on the same frames
+    f_cv = f_net.copy()
+    tm["fallback"] = 0.0
+    if not np.all(use_cv):
+        fb, tm["fallback"] = shells(evalf, u, np.flatnonzero(~use_cv))
+        f_cv[~use_cv] = fb
+        note(f"fallback on {int(np.count_nonzero(~use_cv))} shells", tm["fallback"])
+    # the judge: the plain many-orientation average on every shell
     f_many_ev, tm["many_eval"] = shells(evalf, u)
+    note("many-orientation average (judge), evaluation frames", tm["many_eval"])
     net.close()
 
-    e_small_max, e_small_mean = shell_errors(f_small_ev, f_many_ev)
-    e_cv_max, e_cv_mean = shell_errors(f_cv, f_many_ev)
-    e_g_max, e_g_mean = shell_errors(g_ev + 1j * f_small_ev.imag, f_many_ev)
-    bias = float(np.mean((f_cv.real - f_many_ev.real) / np.abs(f_many_ev[:, :1].real)))
+    def summary(f):
+        emax, emean = shell_errors(f, f_many_ev)
+        return {"shell_max": float(emax.mean()), "shell_mean": float(emean.mean())}
+
+    bias = float(np.mean((f_net.real - f_many_ev.real) / np.abs(f_many_ev[:, :1].real)))
+    cv_time = tm["small_eval"] + tm["predict"] + tm["fallback"]
     return {
-        "f_cv": f_cv, "f_small": f_small_ev, "f_many": f_many_ev, "g": g_ev, "coef": coef, "train_loss": loss,
-        "timing_s": tm,
-        "error_small": {"shell_max": float(e_small_max.mean()), "shell_mean": float(e_small_mean.mean())},
-        "error_cv": {"shell_max": float(e_cv_max.mean()), "shell_mean": float(e_cv_mean.mean())},
-        "error_network": {"shell_max": float(e_g_max.mean()), "shell_mean": float(e_g_mean.mean())},
-        "bias_cv": bias,
+        "f_cv": f_cv, "f_net": f_net, "f_small": f_small_ev, "f_many": f_many_ev, "use_cv": use_cv,
+        "validation_error": e_val, "train_loss": loss, "timing_s": tm,
+        "error_small": summary(f_small_ev), "error_net": summary(f_net), "error_cv": summary(f_cv),
+        "bias_net": bias, "cv_shells": int(np.count_nonzero(use_cv)),
+        "speedup_eval": tm["many_eval"] / cv_time if cv_time > 0 else None,
     }

@@ -120,6 +120,24 @@ def ou_walk(rng, NA, NF, box, sigma, tau):
     return xyz
 
 
+def lattice_vibrations(rng, NA, NF, spacing, sigma, tau):
+    """Atoms on a simple cubic lattice (the first NA sites of the smallest
+    cube holding them), each doing an Ornstein-Uhlenbeck vibration (stationary
+    std sigma per component, time constant tau frames) about its site: a
+    crystal, whose orientational anisotropy persists for the whole run."""
+    n = int(math.ceil(NA ** (1.0 / 3.0)))
+    g = np.stack(np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij"), axis=0).reshape(3, -1)
+    sites = spacing * g[:, :NA].astype(np.float64)
+    xyz = np.empty((3, NA, NF), np.float32)
+    decay = math.exp(-1.0 / tau)
+    kick = sigma * math.sqrt(1.0 - math.exp(-2.0 / tau))
+    u = sigma * rng.normal(size=(3, NA))
+    for t in range(NF):
+        xyz[:, :, t] = sites + u
+        u = u * decay + kick * rng.normal(size=(3, NA))
+    return xyz
+
+
 def case(name, seed=None, scale=1.0):
     """Build one of c1..c5 (or a scaled-down copy with scale<1 for tests)."""
     seeds = {"c1": 101, "c2": 202, "c3": 303, "c5": 505}
