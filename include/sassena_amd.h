@@ -171,7 +171,7 @@ typedef struct sx_trajectory {
 
 typedef struct sx_batch_timing {
     double total_ms;      /* graph launch to completion (HIP events) */
-    double amplitude_ms;  /* persistent amplitude kernel */
+    double amplitude_ms;  /* persistent amplitude kernel (last sx_batch_time_amplitude) */
     size_t terms;         /* sum over trajectories of shells*NM*NA*NF */
     size_t tiles;         /* amplitude work items in the queue */
     int tile_shape;       /* oriented tile shape used (0..3) */
@@ -190,6 +190,10 @@ int sx_batch_run(sx_batch *batch);
    *ms receives its device latency. */
 int sx_batch_run_one(sx_batch *batch, size_t k, double *ms);
 int sx_batch_get_timing(const sx_batch *batch, sx_batch_timing *out);
+/* The persistent amplitude kernel of the whole batch alone, outside the
+   graph (its amplitudes are recomputed, nothing else runs); *ms receives its
+   device time.  For per-kernel rooflines. */
+int sx_batch_time_amplitude(sx_batch *batch, double *ms);
 
 /* ------------------------------------------------------------------------
    The network of the learned control variate.  No counterpart in the

@@ -26,7 +26,8 @@ EXPORTS = ("sx_version", "sx_create", "sx_destroy", "sx_last_error", "sx_stage",
            "sx_compute_shell", "sx_compute_shells", "sx_compute_shells_oriented", "sx_compute_multipole",
            "sx_get_timing",
            "sx_pad_length", "sx_batch_create", "sx_batch_destroy", "sx_batch_last_error", "sx_batch_stage",
-           "sx_batch_run", "sx_batch_run_one", "sx_batch_get_timing", "sx_mlp_param_count", "sx_mlp_create",
+           "sx_batch_run", "sx_batch_run_one", "sx_batch_get_timing", "sx_batch_time_amplitude",
+           "sx_mlp_param_count", "sx_mlp_create",
            "sx_mlp_destroy", "sx_mlp_last_error", "sx_mlp_set_params", "sx_mlp_get_params", "sx_mlp_gradients",
            "sx_mlp_train", "sx_mlp_predict")
 

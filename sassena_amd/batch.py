@@ -44,6 +44,8 @@ def _bind(L):
     L.sx_batch_run_one.restype = ctypes.c_int
     L.sx_batch_get_timing.argtypes = [vp, ctypes.POINTER(sx_batch_timing)]
     L.sx_batch_get_timing.restype = ctypes.c_int
+    L.sx_batch_time_amplitude.argtypes = [vp, ctypes.POINTER(ctypes.c_double)]
+    L.sx_batch_time_amplitude.restype = ctypes.c_int
     L._sx_batch_bound = True
     return L
 
@@ -106,6 +108,12 @@ class Batch:
         ms = ctypes.c_double()
         self._check(self._L.sx_batch_run_one(self._h, k, ctypes.byref(ms)), "sx_batch_run_one")
         return self._results(k), ms.value
+
+    def time_amplitude(self):
+        """Device time (ms) of the batch's amplitude kernel alone."""
+        ms = ctypes.c_double()
+        self._check(self._L.sx_batch_time_amplitude(self._h, ctypes.byref(ms)), "sx_batch_time_amplitude")
+        return ms.value
 
     def timing(self):
         t = sx_batch_timing()

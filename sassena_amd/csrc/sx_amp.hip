@@ -15,6 +15,9 @@
 // (orientation types file and sphere, abstract_vectors_scatter_device.cpp:115-129),
 // amp_oriented_kernel forms d = u_m . r once per atom and orientation and the
 // phase of each shell as the single FP64 product (|q_s| / 2 pi) d.
+#include <algorithm>
+#include <cmath>
+
 #include "sx_amp_device.h"
 
 namespace sx {
@@ -231,7 +234,21 @@ OrientKernelArgs make_orient_args(const OrientArgs &a) {
     k.atoms_per_split = (int)aps;
     k.per_shell_b = a.per_shell_b;
     k.amp = a.amp;
+    k.ap = a.ap;
+    k.qstep = a.qstep;
     return k;
+}
+
+bool shells_in_progression(const double *qlen, int ns, double *step) {
+    *step = 0.0;
+    if (ns < 2) return false;
+    const double d = (qlen[ns - 1] - qlen[0]) / (double)(ns - 1);
+    double qmax = 0.0;
+    for (int s = 0; s < ns; ++s) qmax = std::max(qmax, std::fabs(qlen[s]));
+    for (int s = 0; s < ns; ++s)
+        if (std::fabs(qlen[s] - (qlen[0] + (double)s * d)) > 1e-12 * qmax) return false;
+    *step = d;
+    return true;
 }
 
 hipError_t launch_amp_oriented(const OrientArgs &a, hipStream_t stream) {

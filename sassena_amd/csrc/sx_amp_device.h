@@ -15,6 +15,8 @@ struct OrientKernelArgs {
     size_t NAp;
     int NM, ns, V, ftiles, ogroups, sgroups, split, atoms_per_split, per_shell_b;
     float2 *amp;           // [split][V][NFp], v = s * NM + m
+    int ap;                // shells in arithmetic progression with step qstep (|q| / 2 pi)
+    double qstep;
 };
 
 OrientKernelArgs make_orient_args(const OrientArgs &a);
@@ -79,11 +81,31 @@ __device__ __forceinline__ void oriented_tile(const OrientKernelArgs &a, int ft,
                 double d = ux[i] * xd;
                 d = fma(uy[i], yd, d);
                 d = fma(uz[i], zd, d);
+                if (a.ap) {
+                    // exp(2 pi i q_k d) = exp(2 pi i q_0 d) exp(2 pi i qstep d)^k: two
+                    // hardware sin/cos pairs per (orientation, atom, frame), then one
+                    // complex product per further shell
+                    const float f0 = (float)__builtin_amdgcn_fract(ql[0] * d);
+                    const float fs = (float)__builtin_amdgcn_fract(a.qstep * d);
+                    float zr = __builtin_amdgcn_cosf(f0), zi = __builtin_amdgcn_sinf(f0);
+                    const float wr = __builtin_amdgcn_cosf(fs), wi = __builtin_amdgcn_sinf(fs);
 #pragma unroll
-                for (int k = 0; k < VS; ++k) {
-                    const float f = (float)__builtin_amdgcn_fract(ql[k] * d);
-                    cr[i][k] = fmaf(bk[k], __builtin_amdgcn_cosf(f), cr[i][k]);
-                    ci[i][k] = fmaf(bk[k], __builtin_amdgcn_sinf(f), ci[i][k]);
+                    for (int k = 0; k < VS; ++k) {
+                        cr[i][k] = fmaf(bk[k], zr, cr[i][k]);
+                        ci[i][k] = fmaf(bk[k], zi, ci[i][k]);
+                        if (k + 1 < VS) {
+                            const float nr = fmaf(zr, wr, -zi * wi);
+                            zi = fmaf(zr, wi, zi * wr);
+                            zr = nr;
+                        }
+                    }
+                } else {
+#pragma unroll
+                    for (int k = 0; k < VS; ++k) {
+                        const float f = (float)__builtin_amdgcn_fract(ql[k] * d);
+                        cr[i][k] = fmaf(bk[k], __builtin_amdgcn_cosf(f), cr[i][k]);
+                        ci[i][k] = fmaf(bk[k], __builtin_amdgcn_sinf(f), ci[i][k]);
+                    }
                 }
             }
         }

@@ -97,6 +97,8 @@ constexpr int kXcds = 8;
 struct TrajWork {
     sx_ctx *ctx = nullptr;  // staged coordinates, factors and FFT tables
     DevBuf qlen, u, amp, ppart, apart, fqt, fq, fq2;
+    bool ap = false;       // shells in arithmetic progression
+    double qstep = 0.0;    // their step in |q| / 2 pi
     int chunks = 1;
 };
 
@@ -168,6 +170,8 @@ sx::OrientKernelArgs orient_args(const TrajWork &w, const sx_trajectory &t, int 
     oa.per_shell_b = 0;
     oa.shape = shape;
     oa.amp = w.amp.as<float2>();
+    oa.ap = w.ap ? 1 : 0;
+    oa.qstep = w.qstep;
     return sx::make_orient_args(oa);
 }
 
@@ -428,6 +432,7 @@ int sx_batch_stage(sx_batch *b, const sx_trajectory *trajs, size_t ntraj) {
         if (!w.ctx->plan_ok) return bfail(b, SX_ERR_UNSUPPORTED, "trajectory %zu: NF=%zu too long", k, t.NF);
         std::vector<double> ql(t.nshells);
         for (size_t s = 0; s < t.nshells; ++s) ql[s] = t.qlen[s] / kTwoPi;
+        w.ap = sx::shells_in_progression(ql.data(), (int)t.nshells, &w.qstep);
         w.chunks = (int)std::min<size_t>(t.NM, 8);
         SXB_HIP(b, w.qlen.ensure(sizeof(double) * t.nshells));
         SXB_HIP(b, hipMemcpy(w.qlen.p, ql.data(), sizeof(double) * t.nshells, hipMemcpyHostToDevice));
@@ -497,13 +502,20 @@ int sx_batch_run(sx_batch *b) {
     int rc = copy_out(b, *b->all);
     if (rc != SX_OK) return rc;
     b->timing.total_ms = elapsed(b->e0, b->e1);
-    // amplitude stage alone, outside the graph, for the per-kernel figure
+    return SX_OK;
+}
+
+int sx_batch_time_amplitude(sx_batch *b, double *ms) {
+    if (!b || !ms) return SX_ERR_ARG;
+    if (!b->all) return bfail(b, SX_ERR_STATE, "sx_batch_time_amplitude: stage the batch first");
+    SXB_HIP(b, hipSetDevice(b->cfg.device));
     SXB_HIP(b, hipEventRecord(b->e0, b->stream));
-    rc = launch_amplitude(b, *b->all, b->shape);
+    int rc = launch_amplitude(b, *b->all, b->shape);
     if (rc != SX_OK) return rc;
     SXB_HIP(b, hipEventRecord(b->e1, b->stream));
     SXB_HIP(b, hipEventSynchronize(b->e1));
-    b->timing.amplitude_ms = elapsed(b->e0, b->e1);
+    *ms = elapsed(b->e0, b->e1);
+    b->timing.amplitude_ms = *ms;
     return SX_OK;
 }
 

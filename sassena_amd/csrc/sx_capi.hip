@@ -134,7 +134,8 @@ int pick_shape(sx_ctx *c, int ns, int NM, const float *bptr, bool per_shell, int
     *shape = 0;
     const double terms = (double)ns * NM * (double)c->NA * (double)c->NF;
     if (terms < kTuneTerms) return SX_OK;
-    const std::array<size_t, 5> key = {c->NFp, c->NAp, (size_t)NM, (size_t)ns, per_shell ? 1u : 0u};
+    const std::array<size_t, 6> key = {c->NFp, c->NAp, (size_t)NM, (size_t)ns, per_shell ? 1u : 0u,
+                                       c->q_ap ? 1u : 0u};
     auto it = c->tuned.find(key);
     if (it != c->tuned.end()) {
         *shape = it->second;
@@ -160,6 +161,8 @@ int pick_shape(sx_ctx *c, int ns, int NM, const float *bptr, bool per_shell, int
         oa.per_shell_b = per_shell ? 1 : 0;
         oa.shape = sh;
         oa.amp = c->amp.as<float2>();
+        oa.ap = c->q_ap ? 1 : 0;
+        oa.qstep = c->q_step;
         SX_HIP(c, hipEventRecord(e0, c->stream));
         SX_HIP(c, sx::launch_amp_oriented(oa, c->stream));
         SX_HIP(c, hipEventRecord(e1, c->stream));
@@ -209,6 +212,7 @@ int run_batch(sx_ctx *c, int ns, size_t NM, const double *qv, const double *qlen
     if (oriented) {
         std::vector<double> ql(ns);
         for (int s = 0; s < ns; ++s) ql[s] = qlen[s] / kTwoPi;
+        c->q_ap = sx::shells_in_progression(ql.data(), ns, &c->q_step);
         SX_HIP(c, c->qlen.ensure(sizeof(double) * ns));
         SX_HIP(c, hipMemcpyAsync(c->qlen.p, ql.data(), sizeof(double) * ns, hipMemcpyHostToDevice, c->stream));
         SX_HIP(c, c->uvec.ensure(sizeof(double) * 3 * NM));
@@ -303,6 +307,8 @@ int run_batch(sx_ctx *c, int ns, size_t NM, const double *qv, const double *qlen
             oa.per_shell_b = b_shell ? 1 : 0;
             oa.shape = shape;
             oa.amp = c->amp.as<float2>();
+            oa.ap = c->q_ap ? 1 : 0;
+            oa.qstep = c->q_step;
             SX_HIP(c, sx::launch_amp_oriented(oa, c->stream));
         } else {
             sx::AmpArgs aa;

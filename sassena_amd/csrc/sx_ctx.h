@@ -65,6 +65,9 @@ struct sx_ctx {
     sxh::DevBuf qrev, qlen, uvec, brow, bshell, amp, ppart, apart, fqt, fq, fq2, stage, mpmom, mpq;
     sx_timing timing{};
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
-    // autotuned oriented tile shape per (NFp, NAp, NM, shells, per-shell b)
-    std::map<std::array<size_t, 5>, int> tuned;
+    // oriented shells of the current call in arithmetic progression (step in |q| / 2 pi)
+    bool q_ap = false;
+    double q_step = 0.0;
+    // autotuned oriented tile shape per (NFp, NAp, NM, shells, per-shell b, progression)
+    std::map<std::array<size_t, 6>, int> tuned;
 };

@@ -483,27 +483,33 @@ __device__ __forceinline__ void final_block(const FinalArgs &a, int s, int tile,
         __syncthreads();
         const int tau = tile * 256 + tid;
         if (tau < a.NF) {
+            // e^{+2 pi i f tau / L} by repeated products with e^{+2 pi i tau / L},
+            // re-anchored on the FP64 table every 64 bins
+            const double2 st = a.tw64[tau];
+            const long L = a.L;
+            long anchor = 0;  // (f * tau) mod L at the anchors
+            const long jump = (64L * tau) % L;
             double rr0 = 0.0, ri0 = 0.0, rr1 = 0.0, ri1 = 0.0;
-            int idx = 0;
-            int f = 0;
-            for (; f + 1 < a.L; f += 2) {
-                const double2 t0 = a.tw64[idx];
-                idx += tau;
-                if (idx >= a.L) idx -= a.L;
-                const double2 t1 = a.tw64[idx];
-                idx += tau;
-                if (idx >= a.L) idx -= a.L;
-                rr0 = fma(Psh[f], t0.x, rr0);
-                ri0 = fma(Psh[f], t0.y, ri0);
-                rr1 = fma(Psh[f + 1], t1.x, rr1);
-                ri1 = fma(Psh[f + 1], t1.y, ri1);
-            }
-            for (; f < a.L; ++f) {
-                const double2 t0 = a.tw64[idx];
-                idx += tau;
-                if (idx >= a.L) idx -= a.L;
-                rr0 = fma(Psh[f], t0.x, rr0);
-                ri0 = fma(Psh[f], t0.y, ri0);
+            for (int f0 = 0; f0 < a.L; f0 += 64) {
+                double2 z = a.tw64[anchor];
+                anchor += jump;
+                if (anchor >= L) anchor -= L;
+                const int f1 = min(f0 + 64, a.L);
+                int f = f0;
+                for (; f + 1 < f1; f += 2) {
+                    const double p0 = Psh[f], p1 = Psh[f + 1];
+                    rr0 = fma(p0, z.x, rr0);
+                    ri0 = fma(p0, z.y, ri0);
+                    double zr = z.x * st.x - z.y * st.y, zi = z.x * st.y + z.y * st.x;
+                    rr1 = fma(p1, zr, rr1);
+                    ri1 = fma(p1, zi, ri1);
+                    z.x = zr * st.x - zi * st.y;
+                    z.y = zr * st.y + zi * st.x;
+                }
+                if (f < f1) {
+                    rr0 = fma(Psh[f], z.x, rr0);
+                    ri0 = fma(Psh[f], z.y, ri0);
+                }
             }
             const double norm = a.norm != 0.0 ? a.norm : 1.0 / (double)a.NM;
             const double scale = norm / ((double)a.L * (double)(a.NF - tau));

@@ -77,7 +77,15 @@ struct OrientArgs {
     int NM, ns, split, per_shell_b;
     int shape = 0;         // tile shape index (< kOrientShapes)
     float2 *amp;           // [split][ns*NM][NFp]
+    // Shells in arithmetic progression (qlen[s] = qlen[0] + s qstep): the
+    // tile evaluates exp(2 pi i qlen[s] d) for its first shell and steps to
+    // the next ones by multiplying with exp(2 pi i qstep d).
+    int ap = 0;
+    double qstep = 0.0;
 };
+// True if |q| (any units) is an arithmetic progression to 1e-12 relative;
+// *step receives the common difference.
+bool shells_in_progression(const double *qlen, int ns, double *step);
 constexpr int kOrientShapes = 4;
 hipError_t launch_amp_oriented(const OrientArgs &a, hipStream_t stream);
 int oriented_base_blocks(size_t NFp, int NM, int ns, int shape);

@@ -74,3 +74,18 @@ def test_batch_errors():
         with pytest.raises(sassena_amd.SxError) as e:
             bt.stage([(np.ascontiguousarray(frames.transpose(2, 1, 0)), b, [1.0], unit_vectors(2, 1))])
         assert e.value.code == -5
+
+
+def test_time_amplitude_leaves_results():
+    """The amplitude-only timing call does not disturb the next batch run."""
+    from sassena_amd import Batch, synth
+    trajs = synth.c4_batch(count=3, scale=0.05)
+    bt = Batch()
+    bt.stage([(T.xyz, T.b, T.qlen, T.orient) for T in trajs])
+    first = bt.run()
+    ms = bt.time_amplitude()
+    assert ms > 0
+    again = bt.run()
+    for a, b in zip(first, again):
+        assert np.array_equal(a[0], b[0])
+    bt.close()
