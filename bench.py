@@ -27,6 +27,10 @@ sys.path.insert(0, ROOT)
 # MI355X: 256 CUs, 2.4 GHz, 64 lanes/CU; v_sin_f32 / v_cos_f32 issue at a
 # quarter of the full VALU rate.
 TRANS_PEAK = 256 * 64 * 2.4e9 / 4.0   # transcendental ops/s
+# FP32 VALU lane-operations per second at one wave64 instruction per CU per
+# clock (the non-packed issue rate; packed FP32 is not used)
+VALU_PEAK = 256 * 64 * 2.4e9
+VALU_OPS_PER_TERM = 6.0
 # HBM bytes per amplitude launch on this workload, from rocprofv3 --pmc
 # FETCH_SIZE (x2.0) and WRITE_SIZE (x1.0), the factors a 1 GiB device copy
 # reads for the same counters on MI355X.
@@ -163,6 +167,8 @@ def bench_c4(args):
     bt.close()
     terms = tm["terms"]
     amp_s = amp_ms * 1e-3
+    stepped = all(len(T.qlen) > 1 and np.allclose(np.diff(T.qlen), (T.qlen[-1] - T.qlen[0]) / (len(T.qlen) - 1),
+                                                   rtol=0, atol=1e-12 * np.max(np.abs(T.qlen))) for T in trajs)
     out = {
         "metric": "atom*frame*q-vector terms/s (64 uneven coherent trajectories, one graph launch)",
         "value": terms * args.steps / wall,
@@ -177,9 +183,16 @@ def bench_c4(args):
         "graph_ms": tm["total_ms"],
         "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
                        "min": float(np.min(lat)), "max": float(np.max(lat))},
-        "roofline": {"bound": "valu", "achieved": 2.0 * terms / amp_s / 1e12, "peak": TRANS_PEAK / 1e12,
-                     "unit": "Tsincos-ops/s", "frac": 2.0 * terms / amp_s / TRANS_PEAK, "traffic": None,
-                     "kernel": "amp_persistent_kernel"},
+        "roofline": ({"bound": "valu", "achieved": VALU_OPS_PER_TERM * terms / amp_s / 1e12,
+                      "peak": VALU_PEAK / 1e12, "unit": "Tlane-ops/s",
+                      "frac": VALU_OPS_PER_TERM * terms / amp_s / VALU_PEAK, "traffic": None,
+                      "kernel": "amp_persistent_kernel",
+                      "note": "every trajectory's shells are linearly spaced: stepped phasors, 6 FP32 lane-ops "
+                              "per term at one wave64 instruction per CU per clock"}
+                     if stepped else
+                     {"bound": "valu", "achieved": 2.0 * terms / amp_s / 1e12, "peak": TRANS_PEAK / 1e12,
+                      "unit": "Tsincos-ops/s", "frac": 2.0 * terms / amp_s / TRANS_PEAK, "traffic": None,
+                      "kernel": "amp_persistent_kernel"}),
     }
     print(json.dumps(out), flush=True)
 
@@ -301,7 +314,21 @@ def main():
     value = terms / wall
     amp_avg_s = amp_ms / amp_launches * 1e-3
     terms_per_launch = T.terms() * args.steps / amp_launches
-    achieved = 2.0 * terms_per_launch / amp_avg_s          # sin + cos per term
+    stepped = bool(tm.get("shell_progression", 0))
+    if stepped:
+        # linearly spaced shells: per term one complex product (4 VALU ops) and
+        # two FMAs of the sums; sin/cos only per (orientation, atom, frame)
+        achieved = VALU_OPS_PER_TERM * terms_per_launch / amp_avg_s
+        roof = {"bound": "valu", "achieved": achieved / 1e12, "peak": VALU_PEAK / 1e12,
+                "unit": "Tlane-ops/s", "frac": achieved / VALU_PEAK,
+                "note": "VALU-issue bound: 6 FP32 lane-ops per term (one complex product to step the phasor to "
+                        "the next shell, two FMAs into the sums) at one wave64 instruction per CU per clock; "
+                        f"tile shape {tm.get('tile_shape')}"}
+    else:
+        achieved = 2.0 * terms_per_launch / amp_avg_s      # sin + cos per term
+        roof = {"bound": "valu", "achieved": achieved / 1e12, "peak": TRANS_PEAK / 1e12,
+                "unit": "Tsincos-ops/s", "frac": achieved / TRANS_PEAK,
+                "note": "transcendental-issue bound: 2 hardware sin/cos per term at quarter rate"}
     out = {
         "metric": "atom*frame*q-vector terms/s (coherent F(q,t), FFT autocorrelation)",
         "value": value,
@@ -320,13 +347,10 @@ def main():
                    "NA": T.NA, "NF": T.NF, "shells": T.nshells, "vectors": T.NM, "terms_per_step": T.terms()},
         "kernel_ms_per_step": {"amplitude": amp_ms / args.steps, "correlate": corr_ms / args.steps,
                                "finalize": fin_ms / args.steps, "device_total": dev_ms / args.steps},
-        "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": TRANS_PEAK / 1e12,
-                     "unit": "Tsincos-ops/s", "frac": achieved / TRANS_PEAK,
-                     "traffic": AMP_TRAFFIC_BYTES if not (args.generic or args.vector_block or args.atom_split)
-                     else None,
-                     "traffic_unit": "bytes/launch (rocprofv3 FETCH_SIZE+WRITE_SIZE, copy-calibrated)",
-                     "kernel": "amp_all_kernel" if args.generic else "amp_oriented_kernel",
-                     "note": "transcendental-issue bound: 2 hardware sin/cos per term at quarter rate"},
+        "roofline": dict(roof, traffic=AMP_TRAFFIC_BYTES if not (args.generic or args.vector_block or
+                                                                 args.atom_split) else None,
+                         traffic_unit="bytes/launch (rocprofv3 FETCH_SIZE+WRITE_SIZE, copy-calibrated)",
+                         kernel="amp_all_kernel" if args.generic else "amp_oriented_kernel"),
     }
     dev.close()
     if not args.no_cpu:

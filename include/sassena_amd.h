@@ -84,6 +84,9 @@ typedef struct sx_timing {
     size_t correlate_launches;
     size_t terms;          /* atom*frame*vector terms of the call */
     size_t pad_length;     /* FFT length L used for the time correlation */
+    int tile_shape;        /* oriented shells: tile shape of the last amplitude launch (1..4), else 0 */
+    int shell_progression; /* oriented shells: 1 if the last launch stepped phasors across shells in
+                              arithmetic progression, else 0 */
 } sx_timing;
 
 const char *sx_version(void);

@@ -49,7 +49,8 @@ class sx_timing(ctypes.Structure):
     _fields_ = [("amplitude_ms", ctypes.c_double), ("correlate_ms", ctypes.c_double),
                 ("finalize_ms", ctypes.c_double), ("total_ms", ctypes.c_double),
                 ("amplitude_launches", ctypes.c_size_t), ("correlate_launches", ctypes.c_size_t),
-                ("terms", ctypes.c_size_t), ("pad_length", ctypes.c_size_t)]
+                ("terms", ctypes.c_size_t), ("pad_length", ctypes.c_size_t), ("tile_shape", ctypes.c_int),
+                ("shell_progression", ctypes.c_int)]
 
 
 _lib = None

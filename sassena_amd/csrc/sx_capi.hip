@@ -310,6 +310,8 @@ int run_batch(sx_ctx *c, int ns, size_t NM, const double *qv, const double *qlen
             oa.ap = c->q_ap ? 1 : 0;
             oa.qstep = c->q_step;
             SX_HIP(c, sx::launch_amp_oriented(oa, c->stream));
+            c->timing.tile_shape = shape + 1;
+            c->timing.shell_progression = oa.ap;
         } else {
             sx::AmpArgs aa;
             aa.coords = c->coords.as<float>();
