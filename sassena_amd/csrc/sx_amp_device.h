@@ -82,21 +82,29 @@ __device__ __forceinline__ void oriented_tile(const OrientKernelArgs &a, int ft,
                 d = fma(uy[i], yd, d);
                 d = fma(uz[i], zd, d);
                 if (a.ap) {
-                    // exp(2 pi i q_k d) = exp(2 pi i q_0 d) exp(2 pi i qstep d)^k: two
-                    // hardware sin/cos pairs per (orientation, atom, frame), then one
-                    // complex product per further shell
+                    // phases phi_k = phi_0 + k delta: cos and sin of phi_0 and delta from
+                    // the hardware (two pairs per orientation, atom and frame), phi_1 by one
+                    // complex product, then the three-term recurrence
+                    // v_{k+1} = 2 cos(delta) v_k - v_{k-1} on the cosine and the sine
+                    // (one FMA each per shell)
                     const float f0 = (float)__builtin_amdgcn_fract(ql[0] * d);
                     const float fs = (float)__builtin_amdgcn_fract(a.qstep * d);
-                    float zr = __builtin_amdgcn_cosf(f0), zi = __builtin_amdgcn_sinf(f0);
-                    const float wr = __builtin_amdgcn_cosf(fs), wi = __builtin_amdgcn_sinf(fs);
+                    const float cw = __builtin_amdgcn_cosf(fs), sw = __builtin_amdgcn_sinf(fs);
+                    const float c2 = cw + cw;
+                    float xp = __builtin_amdgcn_cosf(f0), yp = __builtin_amdgcn_sinf(f0);
+                    float xk = fmaf(xp, cw, -yp * sw), yk = fmaf(yp, cw, xp * sw);
+                    cr[i][0] = fmaf(bk[0], xp, cr[i][0]);
+                    ci[i][0] = fmaf(bk[0], yp, ci[i][0]);
 #pragma unroll
-                    for (int k = 0; k < VS; ++k) {
-                        cr[i][k] = fmaf(bk[k], zr, cr[i][k]);
-                        ci[i][k] = fmaf(bk[k], zi, ci[i][k]);
+                    for (int k = 1; k < VS; ++k) {
+                        cr[i][k] = fmaf(bk[k], xk, cr[i][k]);
+                        ci[i][k] = fmaf(bk[k], yk, ci[i][k]);
                         if (k + 1 < VS) {
-                            const float nr = fmaf(zr, wr, -zi * wi);
-                            zi = fmaf(zr, wi, zi * wr);
-                            zr = nr;
+                            const float xn = fmaf(c2, xk, -xp), yn = fmaf(c2, yk, -yp);
+                            xp = xk;
+                            yp = yk;
+                            xk = xn;
+                            yk = yn;
                         }
                     }
                 } else {
