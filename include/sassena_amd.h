@@ -69,8 +69,10 @@ typedef struct sx_config {
     int atom_split;      /* 0: automatic; else atom ranges per amplitude tile */
     int tile_shape;      /* oriented shells: 0 autotuned, 1..4 fixed (2x8, 4x4, 1x8, 1x16
                             orientations x shells per wavefront) */
-    int fft_halves;      /* "self" autocorrelation: 0 or 2 transforms both halves of a padded
-                            series in one sweep, 1 one after the other (half the LDS) */
+    int fft_halves;      /* "self" autocorrelation, the two halves of a padded series: 3 in
+                            different blocks, 2 both in one block's sweep, 1 one after the
+                            other in one block; 0 picks 3 where the FFT plan and memory
+                            allow, else 2 */
     int reserved[7];
 } sx_config;
 

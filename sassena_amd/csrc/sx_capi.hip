@@ -74,6 +74,9 @@ int pick_split(const sx_ctx *c, int V, int vb) {
     return (int)std::max<long>(split, 1);
 }
 
+// Largest buffer of per-pair parts of a the split "self" path may use.
+constexpr size_t kSplitBudgetBytes = (size_t)8 << 30;
+
 int prepare_tables(sx_ctx *c) {
     c->plan_ok = sx::make_fft_plan(c->NF, &c->plan);
     if (!c->plan_ok) return SX_OK;  // only the autocorrelation needs it
@@ -410,7 +413,8 @@ int run_batch(sx_ctx *c, int ns, size_t NM, const double *qv, const double *qlen
         const long pairs = (long)c->aidx_h.size() * (long)NM;
         int chunks = (int)std::max<long>(1, std::min<long>(pairs, std::max<long>(1, (2048 + ns - 1) / ns)));
         const sx::FftPlan &P = c->plan;
-        SX_HIP(c, c->apart.ensure(sizeof(double) * 4 * (size_t)ns * chunks));
+        const int rblocks = (int)std::max<long>(1, std::min<long>(256, pairs / 256));
+        SX_HIP(c, c->apart.ensure(sizeof(double) * 4 * (size_t)ns * std::max(chunks, rblocks)));
         SX_HIP(c, c->ppart.ensure(sizeof(double) * (size_t)ns * chunks * P.L));
         sx::CorrArgs ca{};
         ca.plan = P;
@@ -431,7 +435,30 @@ int run_batch(sx_ctx *c, int ns, size_t NM, const double *qv, const double *qlen
         ca.qrev = c->qrev.as<double>();
         ca.b = bptr;
         ca.brow_per_shell = b_shell ? 1 : 0;
-        SX_HIP(c, sx::launch_corr_self(ca, c->cfg.fft_halves == 1 ? 1 : 2, c->stream));
+        // halves of a padded series: in different blocks (3, or 0 when the plan
+        // allows and the per-pair parts of a fit kSplitBudgetBytes), both in one
+        // sweep (2, or 0 otherwise), one after the other in one block (1)
+        const size_t apair_bytes = sizeof(double2) * 2 * (size_t)ns * (size_t)pairs;
+        int halves = c->cfg.fft_halves;
+        if (halves == 0) halves = (P.T <= 256 && apair_bytes <= kSplitBudgetBytes) ? 3 : 2;
+        if (halves == 3 && P.T > 256)
+            return fail(c, SX_ERR_UNSUPPORTED, "fft_halves=3 needs an FFT plan of at most 256 threads (NF=%zu)", NF);
+        if (halves == 3) {
+            SX_HIP(c, c->apair.ensure(std::max<size_t>(apair_bytes, sizeof(double2))));
+            SX_HIP(c, sx::launch_corr_self_split(ca, c->apair.as<double2>(), c->stream));
+            SX_HIP(c, sx::launch_self_a_reduce(c->apair.as<double2>(), pairs, ns, rblocks, c->apart.as<double>(),
+                                               c->stream));
+            fa.achunks = rblocks;
+        } else {
+            SX_HIP(c, sx::launch_corr_self(ca, halves, c->stream));
+        }
+        // the last lags by direct FP64 sums
+        const int tblocks = (int)std::max<long>(1, std::min<long>(1024, (pairs + 63) / 64));
+        SX_HIP(c, c->tail.ensure(sizeof(double2) * (size_t)ns * tblocks * sx::kTailLags));
+        SX_HIP(c, sx::launch_self_tail(ca, c->tail.as<double2>(), tblocks, c->stream));
+        fa.tail = c->tail.as<double2>();
+        fa.tail_k = (int)std::min<size_t>(sx::kTailLags, NF);
+        fa.tail_blocks = tblocks;
         c->timing.correlate_launches++;
         fa.chunks = chunks;
         fa.apart = c->apart.as<double>();
@@ -481,7 +508,7 @@ int sx_create(const sx_config *cfg, sx_ctx **out) {
     if (cfg->scattering_type != SX_SCATTERING_ALL && cfg->scattering_type != SX_SCATTERING_SELF) return SX_ERR_ARG;
     if (cfg->dsp_type < SX_DSP_AUTOCORRELATE || cfg->dsp_type > SX_DSP_PLAIN) return SX_ERR_ARG;
     if (cfg->dsp_method != SX_METHOD_FFT && cfg->dsp_method != SX_METHOD_DIRECT) return SX_ERR_ARG;
-    if (cfg->fft_halves < 0 || cfg->fft_halves > 2) return SX_ERR_ARG;
+    if (cfg->fft_halves < 0 || cfg->fft_halves > 3) return SX_ERR_ARG;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || cfg->device < 0 || cfg->device >= ndev) return SX_ERR_HIP;
     if (hipSetDevice(cfg->device) != hipSuccess) return SX_ERR_HIP;
@@ -505,7 +532,7 @@ void sx_destroy(sx_ctx *c) {
     if (!c) return;
     (void)hipSetDevice(c->cfg.device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
-    for (DevBuf *d : {&c->coords, &c->b, &c->twL, &c->twL64, &c->twp, &c->twp64, &c->tw64, &c->w, &c->aidx, &c->qrev, &c->qlen, &c->uvec, &c->brow, &c->bshell, &c->mpmom, &c->mpq, &c->amp,
+    for (DevBuf *d : {&c->coords, &c->b, &c->twL, &c->twL64, &c->twp, &c->twp64, &c->tw64, &c->w, &c->aidx, &c->apair, &c->tail, &c->qrev, &c->qlen, &c->uvec, &c->brow, &c->bshell, &c->mpmom, &c->mpq, &c->amp,
                       &c->ppart, &c->apart, &c->fqt, &c->fq, &c->fq2, &c->stage})
         d->release();
     for (auto &e : c->ev)

@@ -139,6 +139,161 @@ __global__ __launch_bounds__(512) void corr_self_kernel(CorrArgs a) {
     }
 }
 
+// One half (H) of every pair of a chunk: the pair loop of corr_self_kernel
+// with a single sweep per pair.
+template <typename C, int E, int H>
+__device__ __forceinline__ void self_half_body(const CorrArgs &a, double2 *__restrict__ apair, int s, int c,
+                                               C *buf) {
+    using Acc = typename Cx<C>::R;
+    const FftPlan &P = a.plan;
+    double2 *red = reinterpret_cast<double2 *>(buf);
+    const int tid = threadIdx.x;
+    const long npairs = (long)a.NA * a.NM;
+    const long p0 = npairs * c / a.chunks, p1 = npairs * (c + 1) / a.chunks;
+    C v[1][E];
+    Acc acc[2][E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) acc[0][e] = acc[1][e] = Acc(0);
+    double *row = a.ppart + ((size_t)s * a.chunks + c) * P.L;
+    const size_t plane = a.NAp * a.NFp;
+    bool first = true;
+    int since = 0;
+    for (long pr = p0; pr < p1; ++pr) {
+        const int ja = (int)(pr / a.NM), m = (int)(pr % a.NM);
+        const int j = a.aidx ? a.aidx[ja] : ja;
+        const double *q = a.qrev + ((size_t)s * a.NM + m) * 3;
+        SelfInput in;
+        in.X = a.coords + (size_t)j * a.NFp;
+        in.Y = in.X + plane;
+        in.Z = in.X + 2 * plane;
+        in.qx = q[0];
+        in.qy = q[1];
+        in.qz = q[2];
+        in.b = a.b[(size_t)(a.brow_per_shell ? s : 0) * a.NAp + j];
+        double ar = 0.0, ai = 0.0;
+        int t = tid;
+        asm volatile("" : "+v"(t));
+        fft_sweep<C, 1, E>(P, H, v, acc, buf, a.twp, a.twL, a.w, a.NF, in, ar, ai, t);
+        const double2 r = block_sum2(ar, ai, red, tid, P.T);
+        if (tid == 0) apair[((size_t)s * npairs + pr) * 2 + H] = r;
+        if (++since == kFlushEvery) {
+            flush_power_half<E, H>(P, acc, row, first, tid);
+            first = false;
+            since = 0;
+        }
+    }
+    if (since > 0 || first) flush_power_half<E, H>(P, acc, row, first, tid);
+}
+
+template <typename C, int E>
+__global__ __launch_bounds__(256) void corr_self_split_kernel(CorrArgs a, double2 *apair) {
+    extern __shared__ double2 lds_d[];
+    C *buf = reinterpret_cast<C *>(lds_d);
+    const int sc = blockIdx.x >> 1;
+    const int s = sc / a.chunks, c = sc % a.chunks;
+    if (blockIdx.x & 1)
+        self_half_body<C, E, 1>(a, apair, s, c, buf);
+    else
+        self_half_body<C, E, 0>(a, apair, s, c, buf);
+}
+
+// Tail lags of the self path: one wavefront per pair; lane l holds x_l and
+// x_{NF-K+l} in LDS, lane k sums S(NF-1-k) = sum_{t<=k} conj(x_t) x_{t+K-1-k+NF-K}
+// in FP64 (float products are exact in FP64).  Blocks stride over the pairs
+// of a shell; per-lane sums are combined over the block's wavefronts.
+__global__ __launch_bounds__(256) void self_tail_kernel(CorrArgs a, double2 *__restrict__ tail, int tblocks) {
+    __shared__ double2 xs[4][2][kTailLags];
+    __shared__ double2 part[4][kTailLags];
+    const int s = blockIdx.y, b = blockIdx.x;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int K = min(kTailLags, a.NF);
+    const long npairs = (long)a.NA * a.NM;
+    const size_t plane = a.NAp * a.NFp;
+    double sr = 0.0, si = 0.0;
+    for (long pr = (long)b * 4 + wave; pr < npairs; pr += (long)tblocks * 4) {
+        const int ja = (int)(pr / a.NM), m = (int)(pr % a.NM);
+        const int j = a.aidx ? a.aidx[ja] : ja;
+        const double *q = a.qrev + ((size_t)s * a.NM + m) * 3;
+        SelfInput in;
+        in.X = a.coords + (size_t)j * a.NFp;
+        in.Y = in.X + plane;
+        in.Z = in.X + 2 * plane;
+        in.qx = q[0];
+        in.qy = q[1];
+        in.qz = q[2];
+        in.b = a.b[(size_t)(a.brow_per_shell ? s : 0) * a.NAp + j];
+        if (lane < K) {
+            const float2 u0 = in((unsigned)lane), u1 = in((unsigned)(a.NF - K + lane));
+            xs[wave][0][lane] = make_double2(u0.x, u0.y);
+            xs[wave][1][lane] = make_double2(u1.x, u1.y);
+        }
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        if (lane < K) {
+            for (int t = 0; t <= lane; ++t) {
+                const double2 x0 = xs[wave][0][t], x1 = xs[wave][1][t + K - 1 - lane];
+                sr = fma(x0.x, x1.x, sr);
+                sr = fma(x0.y, x1.y, sr);
+                si = fma(x0.x, x1.y, si);
+                si = fma(-x0.y, x1.x, si);
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    }
+    part[wave][lane] = make_double2(sr, si);
+    __syncthreads();
+    if (wave == 0) {
+        double2 t = part[0][lane];
+        for (int w = 1; w < 4; ++w) {
+            t.x += part[w][lane].x;
+            t.y += part[w][lane].y;
+        }
+        tail[((size_t)s * tblocks + b) * kTailLags + lane] = t;
+    }
+}
+
+// Per shell: sum over pairs of a = a_0 + a_1 and |a|^2, in fixed order.
+__global__ __launch_bounds__(256) void self_a_reduce_kernel(const double2 *__restrict__ apair, long npairs,
+                                                            int rblocks, double *__restrict__ apart) {
+    __shared__ double2 red[4];
+    __shared__ double red2[4];
+    const int s = blockIdx.y, b = blockIdx.x, tid = threadIdx.x;
+    double sr = 0.0, si = 0.0, s2 = 0.0;
+    const double2 *base = apair + (size_t)s * npairs * 2;
+    for (long pr = (long)b * 256 + tid; pr < npairs; pr += (long)rblocks * 256) {
+        const double2 a0 = base[2 * pr], a1 = base[2 * pr + 1];
+        const double xr = a0.x + a1.x, xi = a0.y + a1.y;
+        sr += xr;
+        si += xi;
+        s2 += xr * xr + xi * xi;
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        sr += __shfl_xor(sr, off, 64);
+        si += __shfl_xor(si, off, 64);
+        s2 += __shfl_xor(s2, off, 64);
+    }
+    if ((tid & 63) == 0) {
+        red[tid >> 6] = make_double2(sr, si);
+        red2[tid >> 6] = s2;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        double r0 = 0.0, r1 = 0.0, r2 = 0.0;
+        for (int k = 0; k < 4; ++k) {
+            r0 += red[k].x;
+            r1 += red[k].y;
+            r2 += red2[k];
+        }
+        double *ap = apart + ((size_t)s * rblocks + b) * 4;
+        ap[0] = r0;
+        ap[1] = r1;
+        ap[2] = r2;
+        ap[3] = 0.0;
+    }
+}
+
 // The block reduction reuses the FFT buffer (see corr_all_block), so the
 // dynamic LDS is exactly the transform data: 160 KB for 10240 FP64 points.
 template <typename C, int NH>
@@ -168,6 +323,38 @@ static hipError_t launch_corr(CorrKernel k4, CorrKernel k8, CorrKernel k16, Corr
 }
 
 size_t corr_all_lds_bytes(const FftPlan &P) { return corr_lds_bytes<double2, 1>(P); }
+
+hipError_t launch_corr_self_split(const CorrArgs &a, double2 *apair, hipStream_t stream) {
+    using K = void (*)(CorrArgs, double2 *);
+    K k = nullptr;
+    switch (a.plan.E) {
+        case 4: k = corr_self_split_kernel<float2, 4>; break;
+        case 8: k = corr_self_split_kernel<float2, 8>; break;
+        case 16: k = corr_self_split_kernel<float2, 16>; break;
+        case 20: k = corr_self_split_kernel<float2, 20>; break;
+        case 24: k = corr_self_split_kernel<float2, 24>; break;
+        default: return hipErrorInvalidValue;
+    }
+    if (a.plan.T > 256) return hipErrorInvalidValue;  // the split kernel is built for 256-thread blocks
+    const size_t lds = corr_lds_bytes<float2, 1>(a.plan);
+    hipError_t e = hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k, dim3((unsigned)(2 * a.nshells * a.chunks)), dim3(a.plan.T), lds, stream, a, apair);
+    return hipGetLastError();
+}
+
+hipError_t launch_self_tail(const CorrArgs &a, double2 *tail, int tblocks, hipStream_t stream) {
+    hipLaunchKernelGGL(self_tail_kernel, dim3((unsigned)tblocks, (unsigned)a.nshells), dim3(256), 0, stream, a, tail,
+                       tblocks);
+    return hipGetLastError();
+}
+
+hipError_t launch_self_a_reduce(const double2 *apair, long npairs, int nshells, int rblocks, double *apart,
+                                hipStream_t stream) {
+    hipLaunchKernelGGL(self_a_reduce_kernel, dim3((unsigned)rblocks, (unsigned)nshells), dim3(256), 0, stream, apair,
+                       npairs, rblocks, apart);
+    return hipGetLastError();
+}
 
 hipError_t launch_corr_all(const CorrArgs &a, hipStream_t stream) {
     return launch_corr<double2, 1>(corr_all_kernel<double2, 1, 4>, corr_all_kernel<double2, 1, 8>,

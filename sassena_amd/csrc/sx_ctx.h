@@ -59,7 +59,7 @@ struct sx_ctx {
     bool staged = false, have_b = false;
     sx::FftPlan plan;
     bool plan_ok = false;
-    sxh::DevBuf coords, b, twL, twL64, twp, twp64, tw64, w, aidx;
+    sxh::DevBuf coords, b, twL, twL64, twp, twp64, tw64, w, aidx, apair, tail;
     std::vector<int> aidx_h;  // "self": atoms with a nonzero length in the current call
     std::vector<float> b_h;   // host copy of the float factors of sx_set_factors
     sxh::DevBuf qrev, qlen, uvec, brow, bshell, amp, ppart, apart, fqt, fq, fq2, stage, mpmom, mpq;

@@ -350,6 +350,24 @@ __device__ __forceinline__ void flush_power(const FftPlan &P, Acc (&acc)[2][E], 
     }
 }
 
+// As flush_power for the bins of half H only (the other half's row entries
+// belong to another block).
+template <int E, int H, typename Acc>
+__device__ __forceinline__ void flush_power_half(const FftPlan &P, Acc (&acc)[2][E], double *row, bool first,
+                                                 int tid) {
+    const int R = P.radix[P.npass - 1];
+    const int stride = P.Lh / R;
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        const int i = e / R, q = e % R;
+        const int f = tid + i * P.T + q * stride;
+        const double add = (double)acc[H][e];
+        double *dst = row + 2 * f + H;
+        *dst = first ? add : *dst + add;
+        acc[H][e] = Acc(0);
+    }
+}
+
 // ------------------------------------------------------------ kernels -----
 
 // FP32 power spectra are folded into the FP64 row every kFlushEvery series
@@ -512,16 +530,29 @@ __device__ __forceinline__ void final_block(const FinalArgs &a, int s, int tile,
                 }
             }
             const double norm = a.norm != 0.0 ? a.norm : 1.0 / (double)a.NM;
-            const double scale = norm / ((double)a.L * (double)(a.NF - tau));
-            const double re = (rr0 + rr1) * scale, im = (ri0 + ri1) * scale;
+            double scale = norm / ((double)a.L * (double)(a.NF - tau));
+            double re = (rr0 + rr1) * scale, im = (ri0 + ri1) * scale;
+            if (a.tail && tau >= a.NF - a.tail_k) {
+                // the last lags from the direct FP64 sums
+                const double2 *tp = a.tail + (size_t)s * a.tail_blocks * kTailLags + (a.NF - 1 - tau);
+                double tr = 0.0, ti = 0.0;
+                for (int bb = 0; bb < a.tail_blocks; ++bb) {
+                    tr += tp[(size_t)bb * kTailLags].x;
+                    ti += tp[(size_t)bb * kTailLags].y;
+                }
+                scale = norm / (double)(a.NF - tau);
+                re = tr * scale;
+                im = ti * scale;
+            }
             a.fqt[((size_t)s * a.NF + tau) * 2] = re;
             a.fqt[((size_t)s * a.NF + tau) * 2 + 1] = a.conj ? -im : im;
         }
     }
     if (tile == 0 && tid == 0) {
         double sr = 0.0, si = 0.0, s2 = 0.0;
-        const double *ap = a.apart + (size_t)s * a.chunks * 4;
-        for (int c = 0; c < a.chunks; ++c) {
+        const int na = a.achunks > 0 ? a.achunks : a.chunks;
+        const double *ap = a.apart + (size_t)s * na * 4;
+        for (int c = 0; c < na; ++c) {
             sr += ap[4 * c];
             si += ap[4 * c + 1];
             s2 += ap[4 * c + 2];

@@ -113,6 +113,20 @@ hipError_t launch_corr_all(const CorrArgs &a, hipStream_t stream);
 // halves: 1 transforms the two halves of a padded series one after the other
 // (half the LDS, twice the resident blocks), 2 both at once.
 hipError_t launch_corr_self(const CorrArgs &a, int halves, hipStream_t stream);
+// The two halves of every padded series in different blocks (half the LDS
+// of a block, more resident blocks): each block writes its half's bins of the
+// power-spectrum rows and, per pair, its half's part of a into
+// apair[shell][pair][half]; self_a_reduce then forms sum a and sum |a|^2 per
+// shell into apart[shell][rblocks][4].
+hipError_t launch_corr_self_split(const CorrArgs &a, double2 *apair, hipStream_t stream);
+// Direct FP64 sums S(tau) = sum_t conj(x_t) x_{t+tau} of the last min(64, NF)
+// lags of every (atom, vector) pair of the self path, summed over pairs into
+// tail[shell][tblocks][64] (one wavefront per pair).  The FP32 transforms
+// divide their rounding error by NF - tau, which grows to NF at the last lag.
+constexpr int kTailLags = 64;
+hipError_t launch_self_tail(const CorrArgs &a, double2 *tail, int tblocks, hipStream_t stream);
+hipError_t launch_self_a_reduce(const double2 *apair, long npairs, int nshells, int rblocks, double *apart,
+                                hipStream_t stream);
 
 // w_f = (1/NF) sum_tau e^{+2 pi i f tau / L} / (L (NF - tau)), stored at
 // ((f & 1) R + q) Lh/R + j for f/2 = j + q Lh/R, R the last radix of the plan:
@@ -121,6 +135,11 @@ hipError_t launch_weights(const FftPlan &P, int NF, double2 *w, hipStream_t stre
 
 struct FinalArgs {
     int L, NF, NM, nshells, chunks;
+    int achunks = 0;       // entries of apart per shell (0: chunks)
+    // "self": the last tail_k lags summed directly in FP64 (tail[shell][tail_blocks][tail_k],
+    // entry k = lag NF-1-k) replace the transform's values there
+    const double2 *tail = nullptr;
+    int tail_k = 0, tail_blocks = 0;
     double norm = 0.0;     // final factor (1/NM for vector averages); 0 means 1/NM
     const double *ppart;   // nullptr: fqt already written by the series kernel
     const double *apart;

@@ -251,3 +251,33 @@ def test_shell_progression_recurrence(sx, oracle_mod, shape):
             if key not in refs:
                 refs[key] = oracle_mod.compute("all", frames, T.b, qlen[s] * u, nthreads=8)
             check_close(fqt[s], fq[s], fq2[s], *refs[key])
+
+
+@pytest.mark.parametrize("halves", [3, 2, 1])
+@pytest.mark.parametrize("NF", [40, 300, 2000, 5000, 9000])
+def test_self_half_layouts(sx, oracle_mod, NF, halves):
+    """The three ways of transforming the two halves of a padded series
+    (different blocks, one sweep, one after the other) against the oracle,
+    with zero-length atoms in the list and few pairs, so that the last lags
+    (averages of a few products, taken from the direct FP64 tail sums) are
+    tested at the 1e-5 bound."""
+    frames, b = small_trajectory(40, NF, seed=NF)
+    b[::3] = 0.0
+    u = unit_vectors(6, 7)
+    qlen = np.array([0.4, 1.7, 3.1])
+    atoms = np.ascontiguousarray(frames.transpose(1, 0, 2))
+    if halves == 3 and sx.pad_length(NF) > 2 * 6144:
+        with sx.ScatterDevice("self", fft_halves=3) as dev:
+            dev.stage(frames, layout="frames")
+            dev.set_factors(b)
+            with pytest.raises(sx.SxError):
+                dev.compute_oriented(qlen, u)
+        return
+    with sx.ScatterDevice("self", fft_halves=halves) as dev:
+        dev.stage(frames, layout="frames")
+        dev.set_factors(b)
+        fqt, fq, fq2 = dev.compute_oriented(qlen, u)
+    for s in range(3):
+        ref = oracle_mod.compute("self", atoms, b, qlen[s] * u, nthreads=8)
+        print(f"NF={NF} halves={halves} shell {s}: error {shell_error(fqt[s], ref[0])[0]:.3e}")
+        check_close(fqt[s], fq[s], fq2[s], *ref)
