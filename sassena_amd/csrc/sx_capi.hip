@@ -75,7 +75,7 @@ int pick_split(const sx_ctx *c, int V, int vb) {
 }
 
 // Largest buffer of per-pair parts of a the split "self" path may use.
-constexpr size_t kSplitBudgetBytes = (size_t)8 << 30;
+constexpr size_t kSplitBudgetBytes = (size_t)16 << 30;
 
 int prepare_tables(sx_ctx *c) {
     c->plan_ok = sx::make_fft_plan(c->NF, &c->plan);
@@ -438,7 +438,8 @@ int run_batch(sx_ctx *c, int ns, size_t NM, const double *qv, const double *qlen
         // halves of a padded series: in different blocks (3, or 0 when the plan
         // allows and the per-pair parts of a fit kSplitBudgetBytes), both in one
         // sweep (2, or 0 otherwise), one after the other in one block (1)
-        const size_t apair_bytes = sizeof(double2) * 2 * (size_t)ns * (size_t)pairs;
+        const int wpp = 2 * (P.T / 64);  // parts of a per pair: halves x wavefronts
+        const size_t apair_bytes = sizeof(double2) * wpp * (size_t)ns * (size_t)pairs;
         int halves = c->cfg.fft_halves;
         if (halves == 0) halves = (P.T <= 256 && apair_bytes <= kSplitBudgetBytes) ? 3 : 2;
         if (halves == 3 && P.T > 256)
@@ -446,8 +447,8 @@ int run_batch(sx_ctx *c, int ns, size_t NM, const double *qv, const double *qlen
         if (halves == 3) {
             SX_HIP(c, c->apair.ensure(std::max<size_t>(apair_bytes, sizeof(double2))));
             SX_HIP(c, sx::launch_corr_self_split(ca, c->apair.as<double2>(), c->stream));
-            SX_HIP(c, sx::launch_self_a_reduce(c->apair.as<double2>(), pairs, ns, rblocks, c->apart.as<double>(),
-                                               c->stream));
+            SX_HIP(c, sx::launch_self_a_reduce(c->apair.as<double2>(), pairs, wpp, ns, rblocks,
+                                               c->apart.as<double>(), c->stream));
             fa.achunks = rblocks;
         } else {
             SX_HIP(c, sx::launch_corr_self(ca, halves, c->stream));

@@ -20,6 +20,8 @@
 // products, would otherwise reach 1e-5 of C(0)); the incoherent path, whose
 // cost is dominated by NA x NM FFTs, transforms in FP32, where the rounding
 // averages out over the pairs.
+#include <type_traits>
+
 #include "sx_fft_device.h"
 
 namespace sx {
@@ -146,8 +148,8 @@ __device__ __forceinline__ void self_half_body(const CorrArgs &a, double2 *__res
                                                C *buf) {
     using Acc = typename Cx<C>::R;
     const FftPlan &P = a.plan;
-    double2 *red = reinterpret_cast<double2 *>(buf);
     const int tid = threadIdx.x;
+    const int wave = tid >> 6, nw = P.T >> 6;
     const long npairs = (long)a.NA * a.NM;
     const long p0 = npairs * c / a.chunks, p1 = npairs * (c + 1) / a.chunks;
     C v[1][E];
@@ -158,24 +160,33 @@ __device__ __forceinline__ void self_half_body(const CorrArgs &a, double2 *__res
     const size_t plane = a.NAp * a.NFp;
     bool first = true;
     int since = 0;
+    using In = typename std::conditional<H == 0, SelfInput, SelfInputShifted>::type;
     for (long pr = p0; pr < p1; ++pr) {
         const int ja = (int)(pr / a.NM), m = (int)(pr % a.NM);
         const int j = a.aidx ? a.aidx[ja] : ja;
         const double *q = a.qrev + ((size_t)s * a.NM + m) * 3;
-        SelfInput in;
+        In in;
         in.X = a.coords + (size_t)j * a.NFp;
         in.Y = in.X + plane;
         in.Z = in.X + 2 * plane;
         in.qx = q[0];
         in.qy = q[1];
         in.qz = q[2];
+        if constexpr (H == 1) in.invL = 1.0 / (double)P.L;
         in.b = a.b[(size_t)(a.brow_per_shell ? s : 0) * a.NAp + j];
         double ar = 0.0, ai = 0.0;
         int t = tid;
         asm volatile("" : "+v"(t));
         fft_sweep<C, 1, E>(P, H, v, acc, buf, a.twp, a.twL, a.w, a.NF, in, ar, ai, t);
-        const double2 r = block_sum2(ar, ai, red, tid, P.T);
-        if (tid == 0) apair[((size_t)s * npairs + pr) * 2 + H] = r;
+        // per-wavefront part of a; the parts are summed per pair in fixed order
+        // by self_a_reduce (no block barrier here)
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            ar += __shfl_xor(ar, off, 64);
+            ai += __shfl_xor(ai, off, 64);
+        }
+        if ((tid & 63) == 0)
+            apair[(((size_t)s * npairs + pr) * 2 + H) * nw + wave] = make_double2(ar, ai);
         if (++since == kFlushEvery) {
             flush_power_half<E, H>(P, acc, row, first, tid);
             first = false;
@@ -253,17 +264,22 @@ __global__ __launch_bounds__(256) void self_tail_kernel(CorrArgs a, double2 *__r
     }
 }
 
-// Per shell: sum over pairs of a = a_0 + a_1 and |a|^2, in fixed order.
+// Per shell: sum over pairs of a (the sum of the pair's wpp parts: two halves
+// x wavefronts) and |a|^2, in fixed order.
 __global__ __launch_bounds__(256) void self_a_reduce_kernel(const double2 *__restrict__ apair, long npairs,
-                                                            int rblocks, double *__restrict__ apart) {
+                                                            int wpp, int rblocks, double *__restrict__ apart) {
     __shared__ double2 red[4];
     __shared__ double red2[4];
     const int s = blockIdx.y, b = blockIdx.x, tid = threadIdx.x;
     double sr = 0.0, si = 0.0, s2 = 0.0;
-    const double2 *base = apair + (size_t)s * npairs * 2;
+    const double2 *base = apair + (size_t)s * npairs * wpp;
     for (long pr = (long)b * 256 + tid; pr < npairs; pr += (long)rblocks * 256) {
-        const double2 a0 = base[2 * pr], a1 = base[2 * pr + 1];
-        const double xr = a0.x + a1.x, xi = a0.y + a1.y;
+        double xr = 0.0, xi = 0.0;
+        for (int k = 0; k < wpp; ++k) {
+            const double2 ak = base[(size_t)pr * wpp + k];
+            xr += ak.x;
+            xi += ak.y;
+        }
         sr += xr;
         si += xi;
         s2 += xr * xr + xi * xi;
@@ -349,10 +365,10 @@ hipError_t launch_self_tail(const CorrArgs &a, double2 *tail, int tblocks, hipSt
     return hipGetLastError();
 }
 
-hipError_t launch_self_a_reduce(const double2 *apair, long npairs, int nshells, int rblocks, double *apart,
-                                hipStream_t stream) {
+hipError_t launch_self_a_reduce(const double2 *apair, long npairs, int wpp, int nshells, int rblocks,
+                                double *apart, hipStream_t stream) {
     hipLaunchKernelGGL(self_a_reduce_kernel, dim3((unsigned)rblocks, (unsigned)nshells), dim3(256), 0, stream, apair,
-                       npairs, rblocks, apart);
+                       npairs, wpp, rblocks, apart);
     return hipGetLastError();
 }
 

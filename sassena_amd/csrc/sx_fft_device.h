@@ -254,7 +254,8 @@ __device__ __forceinline__ void pass_first(const FftPlan &P, int h0, C (&v)[NH][
                 C x = Cx<C>::make(0, 0);
                 if (n < unf) x = in(n);
 #pragma unroll
-                for (int k = 0; k < NH; ++k) v[k][i * R + q] = (h0 + k == 0 || n >= unf) ? x : cmul(x, pre[n]);
+                for (int k = 0; k < NH; ++k)
+                    v[k][i * R + q] = (Input::kPretwiddled || h0 + k == 0 || n >= unf) ? x : cmul(x, pre[n]);
             }
 #pragma unroll
             for (int k = 0; k < NH; ++k) {
@@ -376,6 +377,7 @@ constexpr int kFlushEvery = 16;
 
 template <typename C>
 struct AmpInput {
+    static constexpr bool kPretwiddled = false;
     const float2 *p;
     size_t split_stride;
     int split;
@@ -473,11 +475,28 @@ __device__ __forceinline__ void corr_all_block(const CorrArgs &a, int s, int c, 
 // evaluated on the fly in the first FFT pass from the SoA coordinates
 // (self_vectors_scatter_device.cpp:306-319).
 struct SelfInput {
+    static constexpr bool kPretwiddled = false;
     const float *X, *Y, *Z;
     double qx, qy, qz;
     float b;
     __device__ __forceinline__ float2 operator()(unsigned n) const {
         double p = qx * (double)X[n];
+        p = fma(qy, (double)Y[n], p);
+        p = fma(qz, (double)Z[n], p);
+        const float f = (float)__builtin_amdgcn_fract(p);
+        return make_float2(b * __builtin_amdgcn_cosf(f), b * __builtin_amdgcn_sinf(f));
+    }
+};
+
+// The odd-bin input x_n W_L^n of a self series with the pre-twiddle folded
+// into the phase: b exp(2 pi i (q.r_n - n/L)), reduced in FP64.
+struct SelfInputShifted {
+    static constexpr bool kPretwiddled = true;
+    const float *X, *Y, *Z;
+    double qx, qy, qz, invL;
+    float b;
+    __device__ __forceinline__ float2 operator()(unsigned n) const {
+        double p = fma(-(double)n, invL, qx * (double)X[n]);
         p = fma(qy, (double)Y[n], p);
         p = fma(qz, (double)Z[n], p);
         const float f = (float)__builtin_amdgcn_fract(p);

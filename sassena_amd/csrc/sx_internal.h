@@ -115,9 +115,9 @@ hipError_t launch_corr_all(const CorrArgs &a, hipStream_t stream);
 hipError_t launch_corr_self(const CorrArgs &a, int halves, hipStream_t stream);
 // The two halves of every padded series in different blocks (half the LDS
 // of a block, more resident blocks): each block writes its half's bins of the
-// power-spectrum rows and, per pair, its half's part of a into
-// apair[shell][pair][half]; self_a_reduce then forms sum a and sum |a|^2 per
-// shell into apart[shell][rblocks][4].
+// power-spectrum rows and, per pair, each wavefront's part of a into
+// apair[shell][pair][half][wavefront]; self_a_reduce then forms sum a and
+// sum |a|^2 per shell into apart[shell][rblocks][4] (wpp = 2 x wavefronts).
 hipError_t launch_corr_self_split(const CorrArgs &a, double2 *apair, hipStream_t stream);
 // Direct FP64 sums S(tau) = sum_t conj(x_t) x_{t+tau} of the last min(64, NF)
 // lags of every (atom, vector) pair of the self path, summed over pairs into
@@ -125,8 +125,8 @@ hipError_t launch_corr_self_split(const CorrArgs &a, double2 *apair, hipStream_t
 // divide their rounding error by NF - tau, which grows to NF at the last lag.
 constexpr int kTailLags = 64;
 hipError_t launch_self_tail(const CorrArgs &a, double2 *tail, int tblocks, hipStream_t stream);
-hipError_t launch_self_a_reduce(const double2 *apair, long npairs, int nshells, int rblocks, double *apart,
-                                hipStream_t stream);
+hipError_t launch_self_a_reduce(const double2 *apair, long npairs, int wpp, int nshells, int rblocks,
+                                double *apart, hipStream_t stream);
 
 // w_f = (1/NF) sum_tau e^{+2 pi i f tau / L} / (L (NF - tau)), stored at
 // ((f & 1) R + q) Lh/R + j for f/2 = j + q Lh/R, R the last radix of the plan:
