@@ -100,6 +100,22 @@ def test_frame_counts_against_oracle(sx, oracle_mod, NF):
     check_close(fqt, fq, fq2, *r)
 
 
+@pytest.mark.parametrize("NF", [1, 2, 63, 64, 65, 257, 1500, 3000, 6000])
+def test_self_frame_counts_against_oracle(sx, oracle_mod, NF):
+    """The self path over the FFT length classes, including series shorter
+    than the 64 directly summed tail lags."""
+    NA = 29
+    frames, b = small_trajectory(NA, NF, seed=NF + 1)
+    qv = 1.3 * unit_vectors(4, NF + 1)
+    atoms = np.ascontiguousarray(frames.transpose(1, 0, 2))
+    with sx.ScatterDevice("self") as dev:
+        dev.stage(frames, layout="frames")
+        dev.set_factors(b)
+        fqt, fq, fq2 = dev.compute_shell(qv)
+    r = oracle_mod.compute("self", atoms, b, qv, nthreads=8)
+    check_close(fqt, fq, fq2, *r)
+
+
 @pytest.mark.parametrize("vb,split", [(4, 1), (8, 3), (16, 2), (8, 16)])
 def test_tile_shapes_agree(sx, oracle_mod, vb, split):
     frames, b = small_trajectory(300, 130, seed=3)
