@@ -292,7 +292,9 @@ def main():
     qv = np.array([T.qvectors(s) for s in range(T.nshells)])
     dev = ScatterDevice("all", "autocorrelate", "fftw", profile=True, vector_block=args.vector_block,
                         atom_split=args.atom_split)
-    dev.stage(T.xyz, layout="soa")
+    t_stage = time.perf_counter()
+    dev.stage(T.xyz, layout="soa")   # host (pageable) -> HBM, SoA; not part of value
+    stage_s = time.perf_counter() - t_stage
     dev.set_factors(T.b)
     if args.generic:
         step = lambda: dev.compute_shells(qv)
@@ -350,6 +352,8 @@ def main():
                    "NA": T.NA, "NF": T.NF, "shells": T.nshells, "vectors": T.NM, "terms_per_step": T.terms()},
         "kernel_ms_per_step": {"amplitude": amp_ms / args.steps, "correlate": corr_ms / args.steps,
                                "finalize": fin_ms / args.steps, "device_total": dev_ms / args.steps},
+        "staging_s": stage_s,
+        "pcie_inclusive_value": T.terms() / (wall / args.steps + stage_s),
         "roofline": dict(roof, traffic=AMP_TRAFFIC_BYTES if not (args.generic or args.vector_block or
                                                                  args.atom_split) else None,
                          traffic_unit="bytes/launch (rocprofv3 FETCH_SIZE+WRITE_SIZE, copy-calibrated)",
