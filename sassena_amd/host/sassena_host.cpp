@@ -419,9 +419,8 @@ void AbstractScatterDevice::compute() {
 }
 
 // abstract_vectors_scatter_device.cpp:112-175
-std::vector<CartesianCoor3D> AbstractVectorsScatterDevice::init_subvectors(const CartesianCoor3D &q) const {
+std::vector<CartesianCoor3D> subvectors(const ScatteringParameters &sp, const CartesianCoor3D &q) {
     std::vector<CartesianCoor3D> out;
-    const ScatteringParameters &sp = params_.scattering;
     const auto &vecs = sp.vectors.vectors;
     if (sp.orientation_type == "vectors" && !vecs.empty()) {
         if (sp.vectors.type == "file" || sp.vectors.type == "sphere") {
@@ -450,6 +449,10 @@ std::vector<CartesianCoor3D> AbstractVectorsScatterDevice::init_subvectors(const
         out.push_back(q);
     }
     return out;
+}
+
+std::vector<CartesianCoor3D> AbstractVectorsScatterDevice::init_subvectors(const CartesianCoor3D &q) const {
+    return subvectors(params_.scattering, q);
 }
 
 bool AbstractVectorsScatterDevice::oriented() const {
@@ -519,6 +522,28 @@ void AbstractVectorsScatterDevice::compute_block(size_t first, size_t ns, const 
             qv[(s * NM + m) * 3 + 1] = sub[m].y;
             qv[(s * NM + m) * 3 + 2] = sub[m].z;
         }
+    }
+    // Shells whose subvectors are |q_s| times one shared set (a scan along a
+    // fixed direction: orientation none, or cylinder averaging about a fixed
+    // axis) go through the oriented entry point as well.
+    std::vector<double> ql(ns), uu(3 * NM);
+    bool shared = true;
+    for (size_t s = 0; s < ns && shared; ++s) {
+        ql[s] = vectors_[first + s].length();
+        shared = ql[s] > 0.0;
+    }
+    for (size_t m = 0; m < NM && shared; ++m)
+        for (int k = 0; k < 3; ++k) uu[3 * m + k] = qv[3 * m + k] / ql[0];
+    for (size_t s = 1; s < ns && shared; ++s)
+        for (size_t m = 0; m < NM && shared; ++m)
+            for (int k = 0; k < 3 && shared; ++k) {
+                const double want = ql[s] * uu[3 * m + k], got = qv[(s * NM + m) * 3 + k];
+                shared = std::fabs(got - want) <= 1e-12 * ql[s];
+            }
+    if (shared) {
+        check(sx_compute_shells_oriented(ctx_, ns, ql.data(), uu.data(), NM, bshell, fqt, fq, fq2),
+              "sx_compute_shells_oriented");
+        return;
     }
     check(sx_compute_shells(ctx_, ns, qv.data(), NM, bshell, fqt, fq, fq2), "sx_compute_shells");
 }

@@ -295,6 +295,11 @@ class AbstractScatterDevice : public IScatterDevice {
     std::vector<std::complex<double>> block_fq_, block_fq2_;
 };
 
+// The subvectors of shell q (abstract_vectors_scatter_device.cpp:112-175):
+// |q| u_m for vectors of type file/sphere, the cylinder construction about
+// the axis, or q itself without orientational averaging.
+std::vector<CartesianCoor3D> subvectors(const ScatteringParameters &sp, const CartesianCoor3D &q);
+
 // AbstractVectorsScatterDevice (abstract_vectors_scatter_device.cpp:31-175)
 class AbstractVectorsScatterDevice : public AbstractScatterDevice {
    public:

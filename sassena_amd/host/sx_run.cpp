@@ -35,6 +35,13 @@ static void print_vectors(const Params &p) {
     for (const auto &q : p.scattering.qvectors) std::printf("%.17g %.17g %.17g\n", q.x, q.y, q.z);
     std::printf("orientations %zu\n", p.scattering.vectors.vectors.size());
     for (const auto &u : p.scattering.vectors.vectors) std::printf("%.17g %.17g %.17g\n", u.x, u.y, u.z);
+    std::printf("subvectors %zu\n", p.scattering.qvectors.size());
+    for (const auto &q : p.scattering.qvectors) {
+        const std::vector<CartesianCoor3D> sub = subvectors(p.scattering, q);
+        std::printf("%zu", sub.size());
+        for (const auto &v : sub) std::printf(" %.17g %.17g %.17g", v.x, v.y, v.z);
+        std::printf("\n");
+    }
     std::printf("moments %zu\n", p.scattering.moments.moments.size());
     for (const auto &m : p.scattering.moments.moments) std::printf("%ld %ld\n", m.first, m.second);
 }

@@ -120,3 +120,30 @@ def test_sx_run_multipole(tmp_path, oracle_mod, mp):
         assert shell_error(fqt[i], r[0])[0] < 1e-5
         assert abs(fq[i] - r[1]) < 1e-5 * abs(r[0][0])
         assert abs(fq2[i] - r[2]) < 1e-5 * (abs(r[2]) + 1e-3 * abs(r[0][0]) ** 2)
+
+
+def test_sx_run_cylinder_vectors_along_a_scan(tmp_path, oracle_mod):
+    """Cylinder averaging about an axis over a scan of fixed direction: the
+    host hands the shells to the oriented entry point (|q| times one shared
+    subvector set); checked against the oracle on the host's subvectors."""
+    frames, b, _ = setup_case(tmp_path, NA=70, NF=80)
+    scan = ("<vectors><type>scans</type><scans><scan><from>0.3</from><to>2.1</to><points>4</points>"
+            "<base><x>0.6</x><y>0.8</y><z>0.3</z></base></scan></scans></vectors>")
+    orient = ("<type>vectors</type><axis><x>0.2</x><y>0.1</y><z>1</z></axis><vectors><type>cylinder</type>"
+              "<algorithm>raster_linear</algorithm><resolution>30</resolution></vectors>")
+    xml = config("all", "unused").replace("<vectors><type>file</type><file>unused</file></vectors>", scan)
+    xml = xml.replace("<type>vectors</type><vectors><type>file</type><file>orient.txt</file></vectors>", orient)
+    (tmp_path / "c.xml").write_text(xml)
+    out = sx_run("--config", str(tmp_path / "c.xml"), "--vectors", cwd=tmp_path).stdout.split("\n")
+    k = next(i for i, l in enumerate(out) if l.startswith("subvectors"))
+    subs = []
+    for l in out[k + 1:k + 1 + int(out[k].split()[1])]:
+        v = [float(x) for x in l.split()]
+        subs.append(np.array(v[1:]).reshape(int(v[0]), 3))
+    sx_run("--config", str(tmp_path / "c.xml"), cwd=tmp_path)
+    q, fqt, fq, fq2 = read_signal(tmp_path)
+    assert q.shape == (4, 3)
+    for i in range(4):
+        r = oracle_mod.compute("all", frames, b, subs[i], nthreads=8)
+        assert shell_error(fqt[i], r[0])[0] < 1e-5
+        assert abs(fq[i] - r[1]) < 1e-5 * abs(r[0][0])

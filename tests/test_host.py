@@ -221,3 +221,37 @@ def test_database_xml(tmp_path):
     (tmp_path / "bad.xml").write_text("<root><sizes/></root>")
     res = sx_run("--database", str(tmp_path / "bad.xml"), "0", "0", "H", check=False)
     assert res.returncode == 1 and "Need scattering factors" in res.stderr
+
+
+def subvector_sets(tmp_path, xml):
+    cfg = tmp_path / "scatter.xml"
+    cfg.write_text(xml)
+    lines = sx_run("--config", str(cfg), "--vectors").stdout.split("\n")
+    k = next(i for i, l in enumerate(lines) if l.startswith("subvectors"))
+    out = []
+    for l in lines[k + 1:k + 1 + int(lines[k].split()[1])]:
+        v = [float(x) for x in l.split()]
+        out.append(np.array(v[1:]).reshape(int(v[0]), 3))
+    return out
+
+
+def test_cylinder_subvectors(tmp_path):
+    """abstract_vectors_scatter_device.cpp:130-168: every subvector keeps |q|
+    and the component along the axis; along a scan of fixed direction the
+    subvectors are |q| times one shared set (what lets the host use the
+    oriented entry point)."""
+    xml = """<root><scattering><vectors><type>scans</type><scans><scan><from>0.3</from><to>2.1</to>
+  <points>4</points><base><x>0.6</x><y>0.8</y><z>0.3</z></base></scan></scans></vectors>
+  <average><orientation><type>vectors</type><axis><x>0.2</x><y>0.1</y><z>1</z></axis>
+  <vectors><type>cylinder</type><algorithm>raster_linear</algorithm><resolution>30</resolution></vectors>
+  </orientation></average></scattering></root>"""
+    q, _ = vectors(tmp_path, xml)
+    subs = subvector_sets(tmp_path, xml)
+    ez = np.array([0.2, 0.1, 1.0]) / np.linalg.norm([0.2, 0.1, 1.0])
+    for qq, sub in zip(q, subs):
+        assert sub.shape[0] >= 12
+        assert np.allclose(np.linalg.norm(sub, axis=1), np.linalg.norm(qq), rtol=1e-12)
+        assert np.allclose(sub @ ez, qq @ ez, rtol=0, atol=1e-12)
+    u0 = subs[0] / np.linalg.norm(q[0])
+    for qq, sub in zip(q[1:], subs[1:]):
+        assert np.allclose(sub / np.linalg.norm(qq), u0, rtol=0, atol=1e-12)
