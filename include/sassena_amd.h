@@ -67,8 +67,8 @@ typedef struct sx_config {
     int profile;         /* nonzero: time every kernel with HIP events */
     int vector_block;    /* 0: automatic; else q vectors per wavefront (4, 8 or 16) */
     int atom_split;      /* 0: automatic; else atom ranges per amplitude tile */
-    int tile_shape;      /* oriented shells: 0 autotuned, 1..4 fixed (2x8, 4x4, 1x8, 1x16
-                            orientations x shells per wavefront) */
+    int tile_shape;      /* oriented shells: 0 autotuned, 1..5 fixed (2x8, 4x4, 1x8, 1x16,
+                            1x20 orientations x shells per wavefront) */
     int fft_halves;      /* "self" autocorrelation, the two halves of a padded series: 3 in
                             different blocks, 2 both in one block's sweep, 1 one after the
                             other in one block; 0 picks 3 where the FFT plan and memory
@@ -86,7 +86,7 @@ typedef struct sx_timing {
     size_t correlate_launches;
     size_t terms;          /* atom*frame*vector terms of the call */
     size_t pad_length;     /* FFT length L used for the time correlation */
-    int tile_shape;        /* oriented shells: tile shape of the last amplitude launch (1..4), else 0 */
+    int tile_shape;        /* oriented shells: tile shape of the last amplitude launch (1..5), else 0 */
     int shell_progression; /* oriented shells: 1 if the last launch stepped phasors across shells in
                               arithmetic progression, else 0 */
 } sx_timing;
@@ -179,7 +179,7 @@ typedef struct sx_batch_timing {
     double amplitude_ms;  /* persistent amplitude kernel (last sx_batch_time_amplitude) */
     size_t terms;         /* sum over trajectories of shells*NM*NA*NF */
     size_t tiles;         /* amplitude work items in the queue */
-    int tile_shape;       /* oriented tile shape used (0..3) */
+    int tile_shape;       /* oriented tile shape used (0..4) */
 } sx_batch_timing;
 
 int sx_batch_create(const sx_config *cfg, sx_batch **out);

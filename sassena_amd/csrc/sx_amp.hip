@@ -179,7 +179,7 @@ __global__ __launch_bounds__(256) void amp_all_kernel(AmpKernelArgs a) {
 
 
 template <int VM, int VS, bool PER_ROW_B>
-__global__ __launch_bounds__(256) void amp_oriented_kernel(OrientKernelArgs a) {
+__global__ __launch_bounds__(256, 2) void amp_oriented_kernel(OrientKernelArgs a) {
     const int nitems = a.ftiles * a.split * a.ogroups * a.sgroups;
     const int per = (nitems + 7) >> 3;
     const int item = (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
@@ -195,9 +195,9 @@ __global__ __launch_bounds__(256) void amp_oriented_kernel(OrientKernelArgs a) {
 }
 
 // Oriented tile shapes (orientations x shells per wavefront), picked by the
-// host autotuner: 0: 2x8, 1: 4x4, 2: 1x8, 3: 1x16.
-static const int kShapeVM[kOrientShapes] = {2, 4, 1, 1};
-static const int kShapeVS[kOrientShapes] = {8, 4, 8, 16};
+// host autotuner: 0: 2x8, 1: 4x4, 2: 1x8, 3: 1x16, 4: 1x20.
+static const int kShapeVM[kOrientShapes] = {2, 4, 1, 1, 1};
+static const int kShapeVS[kOrientShapes] = {8, 4, 8, 16, 20};
 
 int orient_shape_vm(int shape) { return kShapeVM[shape]; }
 int orient_shape_vs(int shape) { return kShapeVS[shape]; }
@@ -261,7 +261,8 @@ hipError_t launch_amp_oriented(const OrientArgs &a, hipStream_t stream) {
         case 0: launch_oriented_shape<2, 8>(k, per_row, blocks, stream); break;
         case 1: launch_oriented_shape<4, 4>(k, per_row, blocks, stream); break;
         case 2: launch_oriented_shape<1, 8>(k, per_row, blocks, stream); break;
-        default: launch_oriented_shape<1, 16>(k, per_row, blocks, stream); break;
+        case 3: launch_oriented_shape<1, 16>(k, per_row, blocks, stream); break;
+        default: launch_oriented_shape<1, 20>(k, per_row, blocks, stream); break;
     }
     return hipGetLastError();
 }

@@ -25,7 +25,7 @@
 namespace sx {
 
 template <int VM, int VS>
-__global__ __launch_bounds__(256) void amp_persistent_kernel(const OrientKernelArgs *__restrict__ targs,
+__global__ __launch_bounds__(256, 2) void amp_persistent_kernel(const OrientKernelArgs *__restrict__ targs,
                                                              const int4 *__restrict__ items,
                                                              const int *__restrict__ qstart,
                                                              unsigned *__restrict__ counters) {
@@ -71,7 +71,8 @@ static PersistentFn persistent_fn(int shape) {
         case 0: return amp_persistent_kernel<2, 8>;
         case 1: return amp_persistent_kernel<4, 4>;
         case 2: return amp_persistent_kernel<1, 8>;
-        default: return amp_persistent_kernel<1, 16>;
+        case 3: return amp_persistent_kernel<1, 16>;
+        default: return amp_persistent_kernel<1, 20>;
     }
 }
 

@@ -86,7 +86,7 @@ struct OrientArgs {
 // True if |q| (any units) is an arithmetic progression to 1e-12 relative;
 // *step receives the common difference.
 bool shells_in_progression(const double *qlen, int ns, double *step);
-constexpr int kOrientShapes = 4;
+constexpr int kOrientShapes = 5;
 hipError_t launch_amp_oriented(const OrientArgs &a, hipStream_t stream);
 int oriented_base_blocks(size_t NFp, int NM, int ns, int shape);
 int orient_shape_vm(int shape);

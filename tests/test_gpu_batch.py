@@ -26,7 +26,7 @@ def uneven(n, seed):
     return out
 
 
-@pytest.mark.parametrize("shape", [0, 1, 2, 3, 4])
+@pytest.mark.parametrize("shape", [0, 1, 2, 3, 4, 5])
 def test_batch_against_oracle(oracle_mod, shape):
     import sassena_amd
     trajs = uneven(6, 17)

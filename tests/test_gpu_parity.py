@@ -243,17 +243,18 @@ def test_c2_full_size_oriented_subset(sx, oracle_mod):
         check_close(fqt[s], fq[s], fq2[s], *oracle_mod.compute("all", frames, T.b, qlen[s] * u, nthreads=8))
 
 
-@pytest.mark.parametrize("shape", [1, 2, 3, 4])
+@pytest.mark.parametrize("shape", [1, 2, 3, 4, 5])
 def test_shell_progression_recurrence(sx, oracle_mod, shape):
     """Linearly spaced shells take the stepped-phasor path (two sin/cos pairs
-    per orientation, atom and frame, complex products for the other shells of
-    a tile); a perturbed list takes the direct path.  Full-size headline
-    trajectory, 16 shells up to 4 A^-1, every tile shape."""
+    per orientation, atom and frame, the three-term recurrence for the other
+    shells of a tile); a perturbed list takes the direct path.  Full-size
+    headline trajectory, 40 shells up to 4 A^-1 (up to 19 recurrence steps in
+    the widest tile), every tile shape."""
     from sassena_amd import synth
     T = synth.case("c2")
     frames = synth.to_frames_aos(T.xyz)
     u = T.orient[:2]
-    ap = np.linspace(0.1, 4.0, 16)
+    ap = np.linspace(0.1, 4.0, 40)
     off = ap.copy()
     off[5] += 1e-3
     refs = {}
@@ -262,7 +263,7 @@ def test_shell_progression_recurrence(sx, oracle_mod, shape):
             dev.stage(T.xyz, layout="soa")
             dev.set_factors(T.b)
             fqt, fq, fq2 = dev.compute_oriented(qlen, u)
-        for s in (0, 5, 9, 15):
+        for s in (0, 5, 9, 15, 19, 31, 39):
             key = float(qlen[s])
             if key not in refs:
                 refs[key] = oracle_mod.compute("all", frames, T.b, qlen[s] * u, nthreads=8)
