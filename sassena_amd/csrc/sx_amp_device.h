@@ -29,6 +29,10 @@ OrientKernelArgs make_orient_args(const OrientArgs &a);
 //
 // Atom sums run in FP32 over sub-chunks of CH atoms (coordinates loaded
 // together) and are folded into FP64 totals every kFold atoms.
+// Word 3 of a raw (unswizzled, stride 0) buffer resource on CDNA: 32-bit
+// float data format, no index stride.
+constexpr int kBufferDword3 = 0x00020000;
+
 template <int VM, int VS, bool PER_ROW_B, bool AP>
 __device__ __forceinline__ void oriented_tile_body(const OrientKernelArgs &a, int ft, int sp, int og, int sg) {
     constexpr int CH = 16;     // atoms per coordinate batch
@@ -63,9 +67,15 @@ __device__ __forceinline__ void oriented_tile_body(const OrientKernelArgs &a, in
 
     const size_t j0 = (size_t)sp * a.atoms_per_split;
     const size_t j1 = min(a.NAp, j0 + (size_t)a.atoms_per_split);
-    const float *__restrict__ X = a.X + t;
-    const float *__restrict__ Y = a.Y + t;
-    const float *__restrict__ Z = a.Z + t;
+    // coordinates through buffer loads: a wave-uniform resource per coordinate
+    // batch (base = first atom of the batch, this frame tile), the atom's row
+    // as the scalar offset and the lane's frame as the vector offset, so the
+    // loads need no per-lane address arithmetic
+    const float *X = a.X + (size_t)ft * kWave;
+    const float *Y = a.Y + (size_t)ft * kWave;
+    const float *Z = a.Z + (size_t)ft * kWave;
+    const unsigned voff = (unsigned)lane * 4u;
+    const int rows = (int)(CH * a.NFp * sizeof(float));
 
     for (size_t jf = j0; jf < j1; jf += kFold) {
         const size_t jend = min(j1, jf + (size_t)kFold);
@@ -77,11 +87,15 @@ __device__ __forceinline__ void oriented_tile_body(const OrientKernelArgs &a, in
 #pragma unroll 1
         for (size_t j = jf; j < jend; j += CH) {
             float xs[CH], ys[CH], zs[CH];
+            const __amdgpu_buffer_rsrc_t rx = __builtin_amdgcn_make_buffer_rsrc((void *)(X + j * a.NFp), (short)0, rows, kBufferDword3);
+            const __amdgpu_buffer_rsrc_t ry = __builtin_amdgcn_make_buffer_rsrc((void *)(Y + j * a.NFp), (short)0, rows, kBufferDword3);
+            const __amdgpu_buffer_rsrc_t rz = __builtin_amdgcn_make_buffer_rsrc((void *)(Z + j * a.NFp), (short)0, rows, kBufferDword3);
 #pragma unroll
             for (int c = 0; c < CH; ++c) {
-                xs[c] = X[(j + c) * a.NFp];
-                ys[c] = Y[(j + c) * a.NFp];
-                zs[c] = Z[(j + c) * a.NFp];
+                const int so = (int)(c * a.NFp * sizeof(float));
+                xs[c] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rx, voff, so, 0));
+                ys[c] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(ry, voff, so, 0));
+                zs[c] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rz, voff, so, 0));
             }
 #pragma unroll
             for (int c = 0; c < CH; ++c) {
