@@ -33,7 +33,7 @@ OrientKernelArgs make_orient_args(const OrientArgs &a);
 // float data format, no index stride.
 constexpr int kBufferDword3 = 0x00020000;
 
-template <int VM, int VS, bool PER_ROW_B, bool AP>
+template <int VM, int VS, bool PER_ROW_B, bool AP, bool BUF>
 __device__ __forceinline__ void oriented_tile_body(const OrientKernelArgs &a, int ft, int sp, int og, int sg) {
     constexpr int CH = 16;     // atoms per coordinate batch
     constexpr int kFold = 64;  // atoms per FP32 partial sum
@@ -67,10 +67,11 @@ __device__ __forceinline__ void oriented_tile_body(const OrientKernelArgs &a, in
 
     const size_t j0 = (size_t)sp * a.atoms_per_split;
     const size_t j1 = min(a.NAp, j0 + (size_t)a.atoms_per_split);
-    // coordinates through buffer loads: a wave-uniform resource per coordinate
-    // batch (base = first atom of the batch, this frame tile), the atom's row
-    // as the scalar offset and the lane's frame as the vector offset, so the
-    // loads need no per-lane address arithmetic
+    // BUF: coordinates through buffer loads, a wave-uniform resource per
+    // coordinate batch (base = first atom of the batch, this frame tile), the
+    // atom's row as the scalar offset and the lane's frame as the vector
+    // offset, so the loads need no per-lane address arithmetic.  Else plain
+    // global loads (the persistent batched kernel: measured faster there).
     const float *X = a.X + (size_t)ft * kWave;
     const float *Y = a.Y + (size_t)ft * kWave;
     const float *Z = a.Z + (size_t)ft * kWave;
@@ -87,15 +88,27 @@ __device__ __forceinline__ void oriented_tile_body(const OrientKernelArgs &a, in
 #pragma unroll 1
         for (size_t j = jf; j < jend; j += CH) {
             float xs[CH], ys[CH], zs[CH];
-            const __amdgpu_buffer_rsrc_t rx = __builtin_amdgcn_make_buffer_rsrc((void *)(X + j * a.NFp), (short)0, rows, kBufferDword3);
-            const __amdgpu_buffer_rsrc_t ry = __builtin_amdgcn_make_buffer_rsrc((void *)(Y + j * a.NFp), (short)0, rows, kBufferDword3);
-            const __amdgpu_buffer_rsrc_t rz = __builtin_amdgcn_make_buffer_rsrc((void *)(Z + j * a.NFp), (short)0, rows, kBufferDword3);
+            if constexpr (BUF) {
+                const __amdgpu_buffer_rsrc_t rx =
+                    __builtin_amdgcn_make_buffer_rsrc((void *)(X + j * a.NFp), (short)0, rows, kBufferDword3);
+                const __amdgpu_buffer_rsrc_t ry =
+                    __builtin_amdgcn_make_buffer_rsrc((void *)(Y + j * a.NFp), (short)0, rows, kBufferDword3);
+                const __amdgpu_buffer_rsrc_t rz =
+                    __builtin_amdgcn_make_buffer_rsrc((void *)(Z + j * a.NFp), (short)0, rows, kBufferDword3);
 #pragma unroll
-            for (int c = 0; c < CH; ++c) {
-                const int so = (int)(c * a.NFp * sizeof(float));
-                xs[c] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rx, voff, so, 0));
-                ys[c] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(ry, voff, so, 0));
-                zs[c] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rz, voff, so, 0));
+                for (int c = 0; c < CH; ++c) {
+                    const int so = (int)(c * a.NFp * sizeof(float));
+                    xs[c] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rx, voff, so, 0));
+                    ys[c] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(ry, voff, so, 0));
+                    zs[c] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rz, voff, so, 0));
+                }
+            } else {
+#pragma unroll
+                for (int c = 0; c < CH; ++c) {
+                    xs[c] = X[(j + c) * a.NFp + lane];
+                    ys[c] = Y[(j + c) * a.NFp + lane];
+                    zs[c] = Z[(j + c) * a.NFp + lane];
+                }
             }
 #pragma unroll
             for (int c = 0; c < CH; ++c) {
@@ -204,12 +217,12 @@ __device__ __forceinline__ void oriented_tile_body(const OrientKernelArgs &a, in
         }
 }
 
-template <int VM, int VS, bool PER_ROW_B>
+template <int VM, int VS, bool PER_ROW_B, bool BUF = true>
 __device__ __forceinline__ void oriented_tile(const OrientKernelArgs &a, int ft, int sp, int og, int sg) {
     if (a.ap)
-        oriented_tile_body<VM, VS, PER_ROW_B, true>(a, ft, sp, og, sg);
+        oriented_tile_body<VM, VS, PER_ROW_B, true, BUF>(a, ft, sp, og, sg);
     else
-        oriented_tile_body<VM, VS, PER_ROW_B, false>(a, ft, sp, og, sg);
+        oriented_tile_body<VM, VS, PER_ROW_B, false, BUF>(a, ft, sp, og, sg);
 }
 
 }  // namespace sx

@@ -39,7 +39,7 @@ __global__ __launch_bounds__(256, 2) void amp_persistent_kernel(const OrientKern
         __syncthreads();  // every thread has read cur before it is overwritten
         if (it >= n) break;
         const int4 w = items[begin + it];
-        oriented_tile<VM, VS, false>(targs[w.x], w.y, w.w, w.z & 0xffff, w.z >> 16);
+        oriented_tile<VM, VS, false, false>(targs[w.x], w.y, w.w, w.z & 0xffff, w.z >> 16);
     }
 }
 
