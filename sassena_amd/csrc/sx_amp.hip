@@ -9,12 +9,13 @@
 // by 2 pi), reduced to [0, 1) turns in FP64 (v_fract_f64), and evaluated with
 // the hardware v_sin_f32 / v_cos_f32, which take turns (max abs error 1.3e-7
 // over the reduced range, measured against FP64 on gfx950).  Sums run in FP32
-// over chunks of atoms and are folded into a Kahan-compensated FP32 total.
+// over 64 atoms and are folded into FP64 totals.
 //
 // When the vectors of all shells are one orientation set scaled by |q_s|
 // (orientation types file and sphere, abstract_vectors_scatter_device.cpp:115-129),
 // amp_oriented_kernel forms d = u_m . r once per atom and orientation and the
-// phase of each shell as the single FP64 product (|q_s| / 2 pi) d.
+// phase of each shell as the single FP64 product (|q_s| / 2 pi) d, or, with
+// linearly spaced shells, steps the phasor from shell to shell (sx_amp_device.h).
 #include <algorithm>
 #include <cmath>
 
@@ -121,9 +122,11 @@ __global__ __launch_bounds__(256) void amp_all_kernel(AmpKernelArgs a) {
         brow[i] = PER_ROW_B ? a.brow[v] : 0;
     }
 
-    float sr[VB], si[VB], kr[VB], ki[VB];
+    // FP32 sums over kFold atoms, folded into FP64 totals
+    constexpr int kFold = 64;
+    double tr[VB], ti[VB];
 #pragma unroll
-    for (int i = 0; i < VB; ++i) sr[i] = si[i] = kr[i] = ki[i] = 0.f;
+    for (int i = 0; i < VB; ++i) tr[i] = ti[i] = 0.0;
 
     const size_t j0 = (size_t)sp * a.atoms_per_split;
     const size_t j1 = min(a.NAp, j0 + (size_t)a.atoms_per_split);
@@ -131,49 +134,47 @@ __global__ __launch_bounds__(256) void amp_all_kernel(AmpKernelArgs a) {
     const float *__restrict__ Y = a.Y + t;
     const float *__restrict__ Z = a.Z + t;
 
-    for (size_t j = j0; j < j1; j += kAtomChunk) {
-        float xs[kAtomChunk], ys[kAtomChunk], zs[kAtomChunk];
-#pragma unroll
-        for (int c = 0; c < kAtomChunk; ++c) {
-            xs[c] = X[(j + c) * a.NFp];
-            ys[c] = Y[(j + c) * a.NFp];
-            zs[c] = Z[(j + c) * a.NFp];
-        }
+    for (size_t jf = j0; jf < j1; jf += kFold) {
+        const size_t jend = min(j1, jf + (size_t)kFold);
         float cr[VB], ci[VB];
 #pragma unroll
         for (int i = 0; i < VB; ++i) cr[i] = ci[i] = 0.f;
+#pragma unroll 1
+        for (size_t j = jf; j < jend; j += kAtomChunk) {
+            float xs[kAtomChunk], ys[kAtomChunk], zs[kAtomChunk];
 #pragma unroll
-        for (int c = 0; c < kAtomChunk; ++c) {
-            const double xd = xs[c], yd = ys[c], zd = zs[c];
-            const float b0 = a.b[j + c];
+            for (int c = 0; c < kAtomChunk; ++c) {
+                xs[c] = X[(j + c) * a.NFp];
+                ys[c] = Y[(j + c) * a.NFp];
+                zs[c] = Z[(j + c) * a.NFp];
+            }
 #pragma unroll
-            for (int i = 0; i < VB; ++i) {
-                const float bj = PER_ROW_B ? a.b[(size_t)brow[i] * a.NAp + j + c] : b0;
-                double p = qx[i] * xd;
-                p = fma(qy[i], yd, p);
-                p = fma(qz[i], zd, p);
-                const float f = (float)__builtin_amdgcn_fract(p);
-                cr[i] = fmaf(bj, __builtin_amdgcn_cosf(f), cr[i]);
-                ci[i] = fmaf(bj, __builtin_amdgcn_sinf(f), ci[i]);
+            for (int c = 0; c < kAtomChunk; ++c) {
+                const double xd = xs[c], yd = ys[c], zd = zs[c];
+                const float b0 = a.b[j + c];
+#pragma unroll
+                for (int i = 0; i < VB; ++i) {
+                    const float bj = PER_ROW_B ? a.b[(size_t)brow[i] * a.NAp + j + c] : b0;
+                    double p = qx[i] * xd;
+                    p = fma(qy[i], yd, p);
+                    p = fma(qz[i], zd, p);
+                    const float f = (float)__builtin_amdgcn_fract(p);
+                    cr[i] = fmaf(bj, __builtin_amdgcn_cosf(f), cr[i]);
+                    ci[i] = fmaf(bj, __builtin_amdgcn_sinf(f), ci[i]);
+                }
             }
         }
 #pragma unroll
         for (int i = 0; i < VB; ++i) {
-            float y = cr[i] - kr[i];
-            float s = sr[i] + y;
-            kr[i] = (s - sr[i]) - y;
-            sr[i] = s;
-            y = ci[i] - ki[i];
-            s = si[i] + y;
-            ki[i] = (s - si[i]) - y;
-            si[i] = s;
+            tr[i] += (double)cr[i];
+            ti[i] += (double)ci[i];
         }
     }
 
 #pragma unroll
     for (int i = 0; i < VB; ++i) {
         const int v = vbase + i;
-        if (v < a.V) a.amp[((size_t)sp * a.V + v) * a.NFp + t] = make_float2(sr[i] - kr[i], si[i] - ki[i]);
+        if (v < a.V) a.amp[((size_t)sp * a.V + v) * a.NFp + t] = make_float2((float)tr[i], (float)ti[i]);
     }
 }
 

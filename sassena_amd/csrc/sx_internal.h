@@ -20,7 +20,7 @@
 namespace sx {
 
 constexpr int kWave = 64;         // CDNA wavefront width
-constexpr int kAtomChunk = 8;     // atoms per FP32 chunk before the Kahan update
+constexpr int kAtomChunk = 8;     // atoms per coordinate batch of amp_all_kernel
 constexpr int kMaxPasses = 16;
 
 // Stockham plan of one half-length FFT (Lh points); the full correlation
