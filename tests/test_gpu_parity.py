@@ -227,6 +227,30 @@ def test_oriented_ragged_groups(sx, oracle_mod, NM, ns, split):
             check_close(fqt[s], fq[s], fq2[s], *oracle_mod.compute("all", frames, bb, qlen[s] * u))
 
 
+@pytest.mark.parametrize("shape", [1, 2, 3, 4, 5])
+def test_shell_progression_edge_lists(sx, oracle_mod, shape):
+    """Linearly spaced lists the stepped-phasor path must handle on every tile
+    shape: descending, starting at |q| = 0, two shells, and 21 shells (a
+    partial last tile whose middle shell lies past the list), with shared and
+    per-shell scattering factors."""
+    frames, b = small_trajectory(150, 97, seed=3)
+    u = unit_vectors(9, 4)
+    lists = [np.linspace(3.0, 0.5, 21), np.linspace(0.0, 2.0, 11), np.array([0.7, 1.9]),
+             np.linspace(0.05, 4.0, 21)]
+    for qlen in lists:
+        ns = len(qlen)
+        bs = np.stack([b * (1 + 0.05 * s) for s in range(ns)])
+        for per_shell in (False, True):
+            with sx.ScatterDevice("all", tile_shape=shape) as dev:
+                dev.stage(frames, layout="frames")
+                dev.set_factors(b)
+                fqt, fq, fq2 = dev.compute_oriented(qlen, u, b_shell=bs if per_shell else None)
+                assert dev.timing()["shell_progression"] == 1
+            for s in range(ns):
+                bb = bs[s] if per_shell else b
+                check_close(fqt[s], fq[s], fq2[s], *oracle_mod.compute("all", frames, bb, qlen[s] * u))
+
+
 def test_c2_full_size_oriented_subset(sx, oracle_mod):
     """Full-size headline trajectory, oriented entry point, three shells x two
     orientations, against the oracle."""
