@@ -124,8 +124,17 @@ int prepare_tables(sx_ctx *c) {
 // Tile shape of the oriented amplitude kernel.  A fixed cfg.tile_shape wins;
 // otherwise, for calls of at least kTuneTerms terms, every shape is timed once
 // on the real data (HIP events on the context stream) and the fastest is kept
-// for this problem size.  Smaller calls use shape 0 (2 x 8).
+// for this problem size.  Smaller calls take a default: shells stepped by the
+// recurrence amortise the per-atom phase work over the tile's shells, so more
+// than 16 such shells take 1 x 20 and 9..16 take 1 x 16 (with one factor per
+// atom: per-shell factors need more registers than those shapes have without
+// spilling); everything else 2 x 8.
 constexpr double kTuneTerms = 2e10;
+
+int default_shape(const sx_ctx *c, int ns, bool per_shell) {
+    if (!c->q_ap || per_shell) return 0;
+    return ns > 16 ? 4 : ns > 8 ? 3 : 0;
+}
 
 int pick_shape(sx_ctx *c, int ns, int NM, const float *bptr, bool per_shell, int *shape) {
     if (c->cfg.tile_shape > 0) {
@@ -134,7 +143,7 @@ int pick_shape(sx_ctx *c, int ns, int NM, const float *bptr, bool per_shell, int
         *shape = c->cfg.tile_shape - 1;
         return SX_OK;
     }
-    *shape = 0;
+    *shape = default_shape(c, ns, per_shell);
     const double terms = (double)ns * NM * (double)c->NA * (double)c->NF;
     if (terms < kTuneTerms) return SX_OK;
     const std::array<size_t, 6> key = {c->NFp, c->NAp, (size_t)NM, (size_t)ns, per_shell ? 1u : 0u,
