@@ -223,6 +223,7 @@ OrientKernelArgs make_orient_args(const OrientArgs &a) {
     k.u = a.u;
     k.b = a.b;
     k.NAp = a.NAp;
+    k.natoms = a.natoms ? std::min(a.natoms, a.NAp) : a.NAp;
     k.NM = a.NM;
     k.ns = a.ns;
     k.V = a.NM * a.ns;
@@ -230,7 +231,7 @@ OrientKernelArgs make_orient_args(const OrientArgs &a) {
     k.ogroups = (a.NM + 4 * VM - 1) / (4 * VM);
     k.sgroups = (a.ns + VS - 1) / VS;
     k.split = a.split;
-    size_t aps = (a.NAp + a.split - 1) / a.split;
+    size_t aps = (k.natoms + a.split - 1) / a.split;
     aps = (aps + CH - 1) / CH * CH;
     k.atoms_per_split = (int)aps;
     k.per_shell_b = a.per_shell_b;

@@ -12,7 +12,8 @@ struct OrientKernelArgs {
     const double *qlen;    // [ns] |q_s| / 2 pi
     const double *u;       // [NM][3] unit orientations
     const float *b;        // [nb][NAp]
-    size_t NAp;
+    size_t NAp;            // row stride of the coordinate planes and of b
+    size_t natoms;         // atoms summed (<= NAp)
     int NM, ns, V, ftiles, ogroups, sgroups, split, atoms_per_split, per_shell_b;
     float2 *amp;           // [split][V][NFp], v = s * NM + m
     int ap;                // shells in arithmetic progression with step qstep (|q| / 2 pi)
@@ -66,7 +67,7 @@ __device__ __forceinline__ void oriented_tile_body(const OrientKernelArgs &a, in
         for (int k = 0; k < VS; ++k) tr[i][k] = ti[i][k] = 0.0;
 
     const size_t j0 = (size_t)sp * a.atoms_per_split;
-    const size_t j1 = min(a.NAp, j0 + (size_t)a.atoms_per_split);
+    const size_t j1 = min(a.natoms, j0 + (size_t)a.atoms_per_split);
     // BUF: coordinates through buffer loads, a wave-uniform resource per
     // coordinate batch (base = first atom of the batch, this frame tile), the
     // atom's row as the scalar offset and the lane's frame as the vector

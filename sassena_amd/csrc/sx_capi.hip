@@ -124,12 +124,16 @@ int prepare_tables(sx_ctx *c) {
 // Tile shape of the oriented amplitude kernel.  A fixed cfg.tile_shape wins;
 // otherwise, for calls of at least kTuneTerms terms, every shape is timed once
 // on the real data (HIP events on the context stream) and the fastest is kept
-// for this problem size.  Smaller calls take a default: shells stepped by the
+// for this problem size.  The trials sum over a prefix of the atoms of about
+// kTuneSampleTerms terms (the launch grid is that of the full call; its time
+// is linear in the atoms summed), so tuning a large call costs a few
+// milliseconds per shape.  Smaller calls take a default: shells stepped by the
 // recurrence amortise the per-atom phase work over the tile's shells, so more
 // than 16 such shells take 1 x 20 and 9..16 take 1 x 16 (with one factor per
 // atom: per-shell factors need more registers than those shapes have without
 // spilling); everything else 2 x 8.
 constexpr double kTuneTerms = 2e10;
+constexpr double kTuneSampleTerms = 5e10;
 
 int default_shape(const sx_ctx *c, int ns, bool per_shell) {
     if (!c->q_ap || per_shell) return 0;
@@ -175,6 +179,11 @@ int pick_shape(sx_ctx *c, int ns, int NM, const float *bptr, bool per_shell, int
         oa.amp = c->amp.as<float2>();
         oa.ap = c->q_ap ? 1 : 0;
         oa.qstep = c->q_step;
+        // atom prefix: whole 64-atom rows, at least one per atom range
+        const double frac = std::min(1.0, kTuneSampleTerms / terms);
+        size_t na = (size_t)std::ceil(frac * (double)c->NAp / 64.0) * 64;
+        na = std::max(na, (size_t)64 * split);
+        oa.natoms = std::min(na, c->NAp);
         SX_HIP(c, hipEventRecord(e0, c->stream));
         SX_HIP(c, sx::launch_amp_oriented(oa, c->stream));
         SX_HIP(c, hipEventRecord(e1, c->stream));

@@ -78,10 +78,13 @@ struct OrientArgs {
     int shape = 0;         // tile shape index (< kOrientShapes)
     float2 *amp;           // [split][ns*NM][NFp]
     // Shells in arithmetic progression (qlen[s] = qlen[0] + s qstep): the
-    // tile evaluates exp(2 pi i qlen[s] d) for its first shell and steps to
-    // the next ones by multiplying with exp(2 pi i qstep d).
+    // tile evaluates the phasor of one shell and steps to the others by the
+    // three-term recurrence with exp(2 pi i qstep d).
     int ap = 0;
     double qstep = 0.0;
+    // Atoms summed (the first natoms of the NAp rows); 0: all NAp.  The
+    // autotuner times shapes on a prefix of the atoms.
+    size_t natoms = 0;
 };
 // True if |q| (any units) is an arithmetic progression to 1e-12 relative;
 // *step receives the common difference.
