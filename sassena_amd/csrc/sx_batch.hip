@@ -177,7 +177,10 @@ sx::OrientKernelArgs orient_args(const TrajWork &w, const sx_trajectory &t, int 
 }
 
 // Builds the device work lists of trajectories ks for tile shape `shape`;
-// stride > 1 keeps every stride-th amplitude tile (autotuning samples).
+// stride > 1 keeps every stride-th amplitude tile (autotuning samples), but
+// never fewer than about kMinSampleTiles tiles.
+constexpr size_t kMinSampleTiles = 2048;
+
 int build_plan(sx_batch *b, const std::vector<int> &ks, int shape, int stride, Plan &P) {
     P.ks = ks;
     const int VM = sx::orient_shape_vm(shape), VS = sx::orient_shape_vs(shape);
@@ -200,6 +203,7 @@ int build_plan(sx_batch *b, const std::vector<int> &ks, int shape, int stride, P
     (void)VS;
     // largest tiles first; within a trajectory, tiles sharing a frame tile stay adjacent
     std::stable_sort(items.begin(), items.end(), [](const Item &a, const Item &c) { return a.cost > c.cost; });
+    if (stride > 1) stride = std::max(1, std::min(stride, (int)(items.size() / kMinSampleTiles)));
     if (stride > 1) {
         std::vector<Item> keep;
         for (size_t i = 0; i < items.size(); i += stride) keep.push_back(items[i]);
@@ -458,14 +462,15 @@ int sx_batch_stage(sx_batch *b, const sx_trajectory *trajs, size_t ntraj) {
 
     std::vector<int> all(ntraj);
     std::iota(all.begin(), all.end(), 0);
-    // autotune: each shape on every 8th amplitude tile of the batch
+    // autotune: each shape on every 64th amplitude tile of the batch (at least
+    // kMinSampleTiles tiles)
     if (b->cfg.tile_shape > 0) {
         b->shape = b->cfg.tile_shape - 1;
     } else {
         float best = 1e30f;
         for (int sh = 0; sh < sx::kOrientShapes; ++sh) {
             Plan P;
-            int rc = build_plan(b, all, sh, 8, P);
+            int rc = build_plan(b, all, sh, 64, P);
             if (rc != SX_OK) return rc;
             SXB_HIP(b, hipEventRecord(b->e0, b->stream));
             rc = launch_amplitude(b, P, sh);
