@@ -1,6 +1,6 @@
 """Parity of the gfx950 path (through the C ABI) with the FP64 oracle and the
 golden fixtures.  Tolerance: the per-shell error max_tau |F - F_ref| /
-|F_ref(0)| must stay below 1e-5 (FP32 sincos, Kahan sums and FP32 FFT against
+|F_ref(0)| must stay below 1e-5 (FP32 sincos, FP32 partial sums folded into FP64 and FP32 FFT against
 FP64 references); fq and fq2 are held to the same relative bound."""
 import glob
 import os
