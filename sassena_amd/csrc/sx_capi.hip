@@ -10,6 +10,7 @@
 #include <array>
 #include <map>
 #include <cmath>
+#include <numeric>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -463,6 +464,26 @@ int run_batch(sx_ctx *c, int ns, size_t NM, const double *qv, const double *qlen
         if (halves == 3 && P.T > 256)
             return fail(c, SX_ERR_UNSUPPORTED, "fft_halves=3 needs an FFT plan of at most 256 threads (NF=%zu)", NF);
         if (halves == 3) {
+            // Equal blocks, 2 ns chunks of them: a count that is not a multiple of
+            // the blocks resident at once leaves a last round with the GPU mostly
+            // idle (4,120 blocks on 1,024 slots ran 5 rounds for 4.02 of work).
+            // Round the chunks up to the smallest count that fills whole rounds,
+            // when that keeps at least one pair per chunk and at most twice the
+            // chunks.
+            int per_cu = 0, cus = 0;
+            SX_HIP(c, sx::self_split_blocks_per_cu(P, &per_cu));
+            SX_HIP(c, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->cfg.device));
+            const long resident = (long)std::max(per_cu, 1) * std::max(cus, 1);
+            const long unit = resident / std::gcd(2L * ns, resident);  // chunks per whole round set
+            const long filled = (chunks + unit - 1) / unit * unit;
+            if (filled <= pairs && filled <= 2L * chunks) {
+                chunks = (int)filled;
+                ca.chunks = chunks;
+                SX_HIP(c, c->apart.ensure(sizeof(double) * 4 * (size_t)ns * std::max(chunks, rblocks)));
+                SX_HIP(c, c->ppart.ensure(sizeof(double) * (size_t)ns * chunks * P.L));
+                ca.ppart = c->ppart.as<double>();
+                ca.apart = c->apart.as<double>();
+            }
             SX_HIP(c, c->apair.ensure(std::max<size_t>(apair_bytes, sizeof(double2))));
             SX_HIP(c, sx::launch_corr_self_split(ca, c->apair.as<double2>(), c->stream));
             SX_HIP(c, sx::launch_self_a_reduce(c->apair.as<double2>(), pairs, wpp, ns, rblocks,

@@ -340,17 +340,32 @@ static hipError_t launch_corr(CorrKernel k4, CorrKernel k8, CorrKernel k16, Corr
 
 size_t corr_all_lds_bytes(const FftPlan &P) { return corr_lds_bytes<double2, 1>(P); }
 
-hipError_t launch_corr_self_split(const CorrArgs &a, double2 *apair, hipStream_t stream) {
-    using K = void (*)(CorrArgs, double2 *);
-    K k = nullptr;
-    switch (a.plan.E) {
-        case 4: k = corr_self_split_kernel<float2, 4>; break;
-        case 8: k = corr_self_split_kernel<float2, 8>; break;
-        case 16: k = corr_self_split_kernel<float2, 16>; break;
-        case 20: k = corr_self_split_kernel<float2, 20>; break;
-        case 24: k = corr_self_split_kernel<float2, 24>; break;
-        default: return hipErrorInvalidValue;
+using SelfSplitFn = void (*)(CorrArgs, double2 *);
+
+static SelfSplitFn self_split_fn(int E) {
+    switch (E) {
+        case 4: return corr_self_split_kernel<float2, 4>;
+        case 8: return corr_self_split_kernel<float2, 8>;
+        case 16: return corr_self_split_kernel<float2, 16>;
+        case 20: return corr_self_split_kernel<float2, 20>;
+        case 24: return corr_self_split_kernel<float2, 24>;
+        default: return nullptr;
     }
+}
+
+hipError_t self_split_blocks_per_cu(const FftPlan &P, int *blocks) {
+    *blocks = 0;
+    const SelfSplitFn k = self_split_fn(P.E);
+    if (!k || P.T > 256) return hipErrorInvalidValue;
+    const size_t lds = corr_lds_bytes<float2, 1>(P);
+    hipError_t e = hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, (const void *)k, P.T, lds);
+}
+
+hipError_t launch_corr_self_split(const CorrArgs &a, double2 *apair, hipStream_t stream) {
+    const SelfSplitFn k = self_split_fn(a.plan.E);
+    if (!k) return hipErrorInvalidValue;
     if (a.plan.T > 256) return hipErrorInvalidValue;  // the split kernel is built for 256-thread blocks
     const size_t lds = corr_lds_bytes<float2, 1>(a.plan);
     hipError_t e = hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
