@@ -269,6 +269,25 @@ hipError_t launch_amp_oriented(const OrientArgs &a, hipStream_t stream) {
     return hipGetLastError();
 }
 
+template <int VM, int VS>
+static hipError_t oriented_occupancy(bool per_row, int *blocks) {
+    const void *k = per_row ? (const void *)amp_oriented_kernel<VM, VS, true>
+                            : (const void *)amp_oriented_kernel<VM, VS, false>;
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, k, 256, 0);
+}
+
+hipError_t oriented_blocks_per_cu(int shape, bool per_row, int *blocks) {
+    *blocks = 0;
+    switch (shape) {
+        case 0: return oriented_occupancy<2, 8>(per_row, blocks);
+        case 1: return oriented_occupancy<4, 4>(per_row, blocks);
+        case 2: return oriented_occupancy<1, 8>(per_row, blocks);
+        case 3: return oriented_occupancy<1, 16>(per_row, blocks);
+        case 4: return oriented_occupancy<1, 20>(per_row, blocks);
+        default: return hipErrorInvalidValue;
+    }
+}
+
 int oriented_base_blocks(size_t NFp, int NM, int ns, int shape) {
     const int VM = kShapeVM[shape], VS = kShapeVS[shape];
     return (int)(NFp / kWave) * ((NM + 4 * VM - 1) / (4 * VM)) * ((ns + VS - 1) / VS);

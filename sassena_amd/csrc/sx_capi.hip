@@ -64,6 +64,33 @@ int pick_split_blocks(const sx_ctx *c, long base) {
     return (int)std::max<long>(split, 1);
 }
 
+// Atom ranges of the oriented kernel.  Its base*split workgroups do equal
+// work, so a count just above a multiple of the workgroups resident at once
+// (occupancy x CUs) leaves a last round with most of the GPU idle (3,200 tiles
+// on 512 slots: 7 rounds for 6.25 of work).  Among the splits from the rule
+// above to 16 (at least 64 atoms a range), take the one with the fewest
+// rounds per unit of work; ties go to the smaller split.
+int pick_oriented_split(sx_ctx *c, long base, int shape, bool per_row, int *split) {
+    base = std::max<long>(base, 1);
+    *split = pick_split_blocks(c, base);
+    if (c->cfg.atom_split > 0) return SX_OK;
+    int per_cu = 0, cus = 0;
+    SX_HIP(c, sx::oriented_blocks_per_cu(shape, per_row, &per_cu));
+    SX_HIP(c, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->cfg.device));
+    const long resident = (long)std::max(per_cu, 1) * std::max(cus, 1);
+    const long smax = std::max<long>(*split, std::min<long>(16, (long)(c->NAp / 64)));
+    double best = 0.0;
+    for (long s = *split; s <= smax; ++s) {
+        const long blocks = (base * s + 7) / 8 * 8;
+        const double rounds = (double)((blocks + resident - 1) / resident) / (double)s;
+        if (s == *split || rounds < best * 0.99) {
+            best = rounds;
+            *split = (int)s;
+        }
+    }
+    return SX_OK;
+}
+
 int pick_split(const sx_ctx *c, int V, int vb) {
     if (c->cfg.atom_split > 0) return std::max(1, std::min(c->cfg.atom_split, (int)(c->NAp / sx::kAtomChunk)));
     const long ftiles = (long)(c->NFp / sx::kWave);
@@ -163,7 +190,9 @@ int pick_shape(sx_ctx *c, int ns, int NM, const float *bptr, bool per_shell, int
     SX_HIP(c, hipEventCreate(&e0));
     SX_HIP(c, hipEventCreate(&e1));
     for (int sh = 0; sh < sx::kOrientShapes; ++sh) {
-        const int split = pick_split_blocks(c, sx::oriented_base_blocks(c->NFp, NM, ns, sh));
+        int split = 1;
+        const int src = pick_oriented_split(c, sx::oriented_base_blocks(c->NFp, NM, ns, sh), sh, per_shell, &split);
+        if (src != SX_OK) return src;
         SX_HIP(c, c->amp.ensure(sizeof(float2) * (size_t)ns * NM * c->NFp * split));
         sx::OrientArgs oa;
         oa.coords = c->coords.as<float>();
@@ -287,9 +316,12 @@ int run_batch(sx_ctx *c, int ns, size_t NM, const double *qv, const double *qlen
             int rc = pick_shape(c, ns, (int)NM, bptr, b_shell != nullptr, &shape);
             if (rc != SX_OK) return rc;
         }
-        const int split = mp ? 1
-                             : oriented ? pick_split_blocks(c, sx::oriented_base_blocks(c->NFp, (int)NM, ns, shape))
-                                        : pick_split(c, V, vb);
+        int split = mp ? 1 : oriented ? 1 : pick_split(c, V, vb);
+        if (oriented && !mp) {
+            int rc = pick_oriented_split(c, sx::oriented_base_blocks(c->NFp, (int)NM, ns, shape), shape,
+                                         b_shell != nullptr, &split);
+            if (rc != SX_OK) return rc;
+        }
         const size_t split_stride = (size_t)V * c->NFp;
         SX_HIP(c, c->amp.ensure(sizeof(float2) * split_stride * split));
         if (mp) {

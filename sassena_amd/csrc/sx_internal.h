@@ -92,6 +92,8 @@ bool shells_in_progression(const double *qlen, int ns, double *step);
 constexpr int kOrientShapes = 5;
 hipError_t launch_amp_oriented(const OrientArgs &a, hipStream_t stream);
 int oriented_base_blocks(size_t NFp, int NM, int ns, int shape);
+// Workgroups of the oriented kernel of this shape that fit one CU at once.
+hipError_t oriented_blocks_per_cu(int shape, bool per_row, int *blocks);
 int orient_shape_vm(int shape);
 int orient_shape_vs(int shape);
 
