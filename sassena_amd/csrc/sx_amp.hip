@@ -332,4 +332,16 @@ hipError_t launch_amp_all(const AmpArgs &a, hipStream_t stream) {
     return hipGetLastError();
 }
 
+hipError_t all_blocks_per_cu(int vb, bool per_row, int *blocks) {
+    *blocks = 0;
+    const void *k = nullptr;
+    switch (vb) {
+        case 4: k = per_row ? (const void *)amp_all_kernel<4, true> : (const void *)amp_all_kernel<4, false>; break;
+        case 8: k = per_row ? (const void *)amp_all_kernel<8, true> : (const void *)amp_all_kernel<8, false>; break;
+        case 16: k = per_row ? (const void *)amp_all_kernel<16, true> : (const void *)amp_all_kernel<16, false>; break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, k, 256, 0);
+}
+
 }  // namespace sx

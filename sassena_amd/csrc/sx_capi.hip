@@ -70,6 +70,8 @@ int pick_split_blocks(const sx_ctx *c, long base) {
 // on 512 slots: 7 rounds for 6.25 of work).  Among the splits from the rule
 // above to 16 (at least 64 atoms a range), take the one with the fewest
 // rounds per unit of work; ties go to the smaller split.
+int round_filling_split(long base, int first, long resident, size_t NAp);
+
 int pick_oriented_split(sx_ctx *c, long base, int shape, bool per_row, int *split) {
     base = std::max<long>(base, 1);
     *split = pick_split_blocks(c, base);
@@ -77,17 +79,7 @@ int pick_oriented_split(sx_ctx *c, long base, int shape, bool per_row, int *spli
     int per_cu = 0, cus = 0;
     SX_HIP(c, sx::oriented_blocks_per_cu(shape, per_row, &per_cu));
     SX_HIP(c, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->cfg.device));
-    const long resident = (long)std::max(per_cu, 1) * std::max(cus, 1);
-    const long smax = std::max<long>(*split, std::min<long>(16, (long)(c->NAp / 64)));
-    double best = 0.0;
-    for (long s = *split; s <= smax; ++s) {
-        const long blocks = (base * s + 7) / 8 * 8;
-        const double rounds = (double)((blocks + resident - 1) / resident) / (double)s;
-        if (s == *split || rounds < best * 0.99) {
-            best = rounds;
-            *split = (int)s;
-        }
-    }
+    *split = round_filling_split(base, *split, (long)std::max(per_cu, 1) * std::max(cus, 1), c->NAp);
     return SX_OK;
 }
 
@@ -100,6 +92,37 @@ int pick_split(const sx_ctx *c, int V, int vb) {
     split = std::min<long>(split, 16);
     split = std::min<long>(split, (long)(c->NAp / 64));
     return (int)std::max<long>(split, 1);
+}
+
+// Splits from `first` to 16 (at least 64 atoms a range) for `base` equal
+// workgroups per split: the one with the fewest rounds of `resident`
+// workgroups per unit of work, ties to the smaller split.
+int round_filling_split(long base, int first, long resident, size_t NAp) {
+    const long smax = std::max<long>(first, std::min<long>(16, (long)(NAp / 64)));
+    int split = first;
+    double best = 0.0;
+    for (long s = first; s <= smax; ++s) {
+        const long blocks = (base * s + 7) / 8 * 8;
+        const double rounds = (double)((blocks + resident - 1) / resident) / (double)s;
+        if (s == first || rounds < best * 0.99) {
+            best = rounds;
+            split = (int)s;
+        }
+    }
+    return split;
+}
+
+// Atom ranges of the general-vector kernel: the rule above, then the split
+// that fills whole rounds of resident workgroups (see pick_oriented_split).
+int pick_all_split(sx_ctx *c, int V, int vb, bool per_row, int *split) {
+    *split = pick_split(c, V, vb);
+    if (c->cfg.atom_split > 0) return SX_OK;
+    int per_cu = 0, cus = 0;
+    SX_HIP(c, sx::all_blocks_per_cu(vb, per_row, &per_cu));
+    SX_HIP(c, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->cfg.device));
+    const long base = (long)(c->NFp / sx::kWave) * ((V + 4L * vb - 1) / (4L * vb));
+    *split = round_filling_split(base, *split, (long)std::max(per_cu, 1) * std::max(cus, 1), c->NAp);
+    return SX_OK;
 }
 
 // Largest buffer of per-pair parts of a the split "self" path may use.
@@ -316,7 +339,11 @@ int run_batch(sx_ctx *c, int ns, size_t NM, const double *qv, const double *qlen
             int rc = pick_shape(c, ns, (int)NM, bptr, b_shell != nullptr, &shape);
             if (rc != SX_OK) return rc;
         }
-        int split = mp ? 1 : oriented ? 1 : pick_split(c, V, vb);
+        int split = 1;
+        if (!oriented && !mp) {
+            int rc = pick_all_split(c, V, vb, browp != nullptr, &split);
+            if (rc != SX_OK) return rc;
+        }
         if (oriented && !mp) {
             int rc = pick_oriented_split(c, sx::oriented_base_blocks(c->NFp, (int)NM, ns, shape), shape,
                                          b_shell != nullptr, &split);

@@ -65,6 +65,8 @@ struct AmpArgs {
     float2 *amp;           // [split][V][NFp]
 };
 hipError_t launch_amp_all(const AmpArgs &a, hipStream_t stream);
+// Workgroups of amp_all_kernel with vb vectors per wavefront that fit one CU.
+hipError_t all_blocks_per_cu(int vb, bool per_row, int *blocks);
 
 // Oriented shells: v = s * NM + m has q_v = |q_s| u_m.  NAp must be a multiple
 // of 16 (it is a multiple of 64).
