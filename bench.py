@@ -33,10 +33,10 @@ FP32_PEAK = 157.3e12
 # Linearly spaced shells: per term two FMAs step the cosine and sine of the
 # phase to the next shell and two FMAs add b cos, b sin to the sums.
 FLOPS_PER_TERM = 8.0
-# HBM bytes per amplitude launch on this workload (tile 1 x 20), from
-# rocprofv3 --pmc FETCH_SIZE (x2.0) and WRITE_SIZE (x1.0), the factors a
-# 1 GiB device copy reads for the same counters on MI355X.
-AMP_TRAFFIC_BYTES = 1.97e9 + 0.131e9
+# HBM bytes per amplitude launch on this workload (tile 1 x 20, 4 atom
+# ranges), from rocprofv3 --pmc FETCH_SIZE (x2.0) and WRITE_SIZE (x1.0), the
+# factors a 1 GiB device copy reads for the same counters on MI355X.
+AMP_TRAFFIC_BYTES = 1.97e9 + 0.524e9
 
 
 def log(msg):
