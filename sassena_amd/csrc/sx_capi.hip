@@ -526,17 +526,26 @@ int run_batch(sx_ctx *c, int ns, size_t NM, const double *qv, const double *qlen
             // Equal blocks, 2 ns chunks of them: a count that is not a multiple of
             // the blocks resident at once leaves a last round with the GPU mostly
             // idle (4,120 blocks on 1,024 slots ran 5 rounds for 4.02 of work).
-            // Round the chunks up to the smallest count that fills whole rounds,
-            // when that keeps at least one pair per chunk and at most twice the
-            // chunks.
+            // Among half to twice the chunk count (at least one pair a chunk),
+            // take the one with the fewest rounds per unit of work, ties (within
+            // 0.5 %) to the count nearest the original.
             int per_cu = 0, cus = 0;
             SX_HIP(c, sx::self_split_blocks_per_cu(P, &per_cu));
             SX_HIP(c, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->cfg.device));
             const long resident = (long)std::max(per_cu, 1) * std::max(cus, 1);
-            const long unit = resident / std::gcd(2L * ns, resident);  // chunks per whole round set
-            const long filled = (chunks + unit - 1) / unit * unit;
-            if (filled <= pairs && filled <= 2L * chunks) {
-                chunks = (int)filled;
+            const long c0 = chunks;
+            long pick = c0;
+            double best = 0.0;
+            for (long ch = std::max(1L, c0 / 2); ch <= std::min(pairs, 2 * c0); ++ch) {
+                const double per = (double)((2L * ns * ch + resident - 1) / resident) / (double)ch;
+                const bool closer = std::labs(ch - c0) < std::labs(pick - c0);
+                if (best == 0.0 || per < best * 0.995 || (per <= best * 1.005 && closer)) {
+                    if (per < best || best == 0.0) best = per;
+                    pick = ch;
+                }
+            }
+            if (pick != c0) {
+                chunks = (int)pick;
                 ca.chunks = chunks;
                 SX_HIP(c, c->apart.ensure(sizeof(double) * 4 * (size_t)ns * std::max(chunks, rblocks)));
                 SX_HIP(c, c->ppart.ensure(sizeof(double) * (size_t)ns * chunks * P.L));
