@@ -70,16 +70,17 @@ int pick_split_blocks(const sx_ctx *c, long base) {
 // on 512 slots: 7 rounds for 6.25 of work).  Among the splits from the rule
 // above to 16 (at least 64 atoms a range), take the one with the fewest
 // rounds per unit of work; ties go to the smaller split.
-int round_filling_split(long base, int first, long resident, size_t NAp);
+int round_filling_split(long base, int first, long resident, size_t NAp, size_t split_bytes);
 
-int pick_oriented_split(sx_ctx *c, long base, int shape, bool per_row, int *split) {
+int pick_oriented_split(sx_ctx *c, long base, int shape, bool per_row, size_t V, int *split) {
     base = std::max<long>(base, 1);
     *split = pick_split_blocks(c, base);
     if (c->cfg.atom_split > 0) return SX_OK;
     int per_cu = 0, cus = 0;
     SX_HIP(c, sx::oriented_blocks_per_cu(shape, per_row, &per_cu));
     SX_HIP(c, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->cfg.device));
-    *split = round_filling_split(base, *split, (long)std::max(per_cu, 1) * std::max(cus, 1), c->NAp);
+    const size_t split_bytes = sizeof(float2) * c->NFp * std::max<size_t>(V, 1);
+    *split = round_filling_split(base, *split, (long)std::max(per_cu, 1) * std::max(cus, 1), c->NAp, split_bytes);
     return SX_OK;
 }
 
@@ -94,11 +95,16 @@ int pick_split(const sx_ctx *c, int V, int vb) {
     return (int)std::max<long>(split, 1);
 }
 
-// Splits from `first` to 16 (at least 64 atoms a range) for `base` equal
+// Amplitude bytes the extra atom ranges of round_filling_split may add.
+constexpr size_t kSplitAmpBudget = (size_t)4 << 30;
+
+// Splits from `first` to 16 (at least 64 atoms a range, partial amplitude
+// sets of split_bytes each within kSplitAmpBudget) for `base` equal
 // workgroups per split: the one with the fewest rounds of `resident`
 // workgroups per unit of work, ties to the smaller split.
-int round_filling_split(long base, int first, long resident, size_t NAp) {
-    const long smax = std::max<long>(first, std::min<long>(16, (long)(NAp / 64)));
+int round_filling_split(long base, int first, long resident, size_t NAp, size_t split_bytes) {
+    const long smem = (long)(kSplitAmpBudget / std::max<size_t>(split_bytes, 1));
+    const long smax = std::max<long>(first, std::min<long>({16L, (long)(NAp / 64), smem}));
     int split = first;
     double best = 0.0;
     for (long s = first; s <= smax; ++s) {
@@ -121,7 +127,8 @@ int pick_all_split(sx_ctx *c, int V, int vb, bool per_row, int *split) {
     SX_HIP(c, sx::all_blocks_per_cu(vb, per_row, &per_cu));
     SX_HIP(c, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->cfg.device));
     const long base = (long)(c->NFp / sx::kWave) * ((V + 4L * vb - 1) / (4L * vb));
-    *split = round_filling_split(base, *split, (long)std::max(per_cu, 1) * std::max(cus, 1), c->NAp);
+    *split = round_filling_split(base, *split, (long)std::max(per_cu, 1) * std::max(cus, 1), c->NAp,
+                                 sizeof(float2) * c->NFp * (size_t)V);
     return SX_OK;
 }
 
@@ -214,7 +221,8 @@ int pick_shape(sx_ctx *c, int ns, int NM, const float *bptr, bool per_shell, int
     SX_HIP(c, hipEventCreate(&e1));
     for (int sh = 0; sh < sx::kOrientShapes; ++sh) {
         int split = 1;
-        const int src = pick_oriented_split(c, sx::oriented_base_blocks(c->NFp, NM, ns, sh), sh, per_shell, &split);
+        const int src = pick_oriented_split(c, sx::oriented_base_blocks(c->NFp, NM, ns, sh), sh, per_shell,
+                                            (size_t)ns * NM, &split);
         if (src != SX_OK) return src;
         SX_HIP(c, c->amp.ensure(sizeof(float2) * (size_t)ns * NM * c->NFp * split));
         sx::OrientArgs oa;
@@ -346,7 +354,7 @@ int run_batch(sx_ctx *c, int ns, size_t NM, const double *qv, const double *qlen
         }
         if (oriented && !mp) {
             int rc = pick_oriented_split(c, sx::oriented_base_blocks(c->NFp, (int)NM, ns, shape), shape,
-                                         b_shell != nullptr, &split);
+                                         b_shell != nullptr, (size_t)ns * NM, &split);
             if (rc != SX_OK) return rc;
         }
         const size_t split_stride = (size_t)V * c->NFp;
