@@ -140,6 +140,47 @@ def bench_c3(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_mp(args):
+    """Multipole sphere averaging (MPSphereScatterDevice) on the headline
+    trajectory: 20,000 atoms x 2,000 frames, moments (0,0) and l = 1..10,
+    m = -l..l (121, the reference's list for resolution 10), 10 shells over
+    0.2-2.0 A^-1, FFT autocorrelation.  One term = one (atom, frame, moment,
+    shell)."""
+    from sassena_amd import ScatterDevice, synth
+    T = synth.case("c2")
+    lmax = 10
+    moments = [(0, 0)] + [(l, m) for l in range(1, lmax + 1) for m in range(-l, l + 1)]
+    qv = np.array([[0.0, 0.0, q] for q in np.linspace(0.2, 2.0, 10)])
+    dev = ScatterDevice("all", "autocorrelate", "fftw", profile=True)
+    dev.stage(T.xyz, layout="soa")
+    dev.set_factors(T.b)
+    step = lambda: dev.compute_multipole("sphere", qv, moments)
+    for _ in range(args.warmup):
+        step()
+    amp_ms = dev_ms = 0.0
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+        tm = dev.timing()
+        amp_ms += tm["amplitude_ms"]
+        dev_ms += tm["total_ms"]
+    wall = time.perf_counter() - t0
+    dev.close()
+    terms = T.NA * T.NF * len(moments) * len(qv)
+    out = {
+        "metric": "atom*frame*moment*shell terms/s (multipole sphere average, FFT autocorrelation)",
+        "value": terms * args.steps / wall, "unit": "terms/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": wall / args.steps * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64 special functions and sums / f32 amplitudes",
+        "data": "synthetic",
+        "config": {"workload": f"multipole sphere, {T.NA} atoms x {T.NF} frames, l <= {lmax} "
+                               f"({len(moments)} moments) x {len(qv)} shells",
+                   "NA": T.NA, "NF": T.NF, "moments": len(moments), "shells": len(qv), "terms_per_step": terms},
+        "kernel_ms_per_step": {"multipole": amp_ms / args.steps, "device_total": dev_ms / args.steps},
+    }
+    print(json.dumps(out))
+
+
 def bench_c4(args):
     """64 trajectories (atoms log-uniform 500-50,000, frames 256-4,096, box
     (NA/0.1)^(1/3) A, sigma 0.1 walk, 10-60 shells over 0.1-4.0 A^-1 with
@@ -248,10 +289,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--vector-block", type=int, default=0)
     ap.add_argument("--atom-split", type=int, default=0)
-    ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c4", "c5"],
+    ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c4", "c5", "mp"],
                     help="c2: the headline coherent case; c3: incoherent self with per-pair FFTs; "
                          "c4: 64 uneven trajectories in one batched graph; "
-                         "c5: learned control variate against the 1,000-orientation average")
+                         "c5: learned control variate against the 1,000-orientation average; "
+                         "mp: multipole sphere average (121 moments x 10 shells) on the c2 trajectory")
     ap.add_argument("--c5-scale", type=float, default=1.0, help="shrink the c5 atoms and frames (tests)")
     ap.add_argument("--c4-scale", type=float, default=1.0, help="shrink the c4 trajectories (tests)")
     ap.add_argument("--c5-tol", type=float, default=1e-3,
@@ -287,6 +329,8 @@ def main():
         return bench_c4(args)
     if args.workload == "c5":
         return bench_c5(args)
+    if args.workload == "mp":
+        return bench_mp(args)
 
     T = synth.case("c2")
     qv = np.array([T.qvectors(s) for s in range(T.nshells)])

@@ -35,19 +35,18 @@ struct MultipoleKernelArgs {
     float2 *amp;            // [ns * nmo][NFp]
 };
 
-// j_l(x) for one l >= 0, x >= 0: upward recurrence where stable (x > l),
-// else Miller's downward recurrence normalised to j_0 (or j_1 near a zero of j_0).
-__device__ double sph_bessel(int l, double x) {
+// j_l(x) for one l >= 0, x >= 0, from j_0, j_1 and 1/x computed once per
+// argument (shared by the moments of a wavefront): upward recurrence where
+// stable (x > l), else Miller's downward recurrence normalised to j_0 (or j_1
+// near a zero of j_0).
+__device__ double sph_bessel_from(int l, double x, double ix, double j0, double j1) {
     if (x == 0.0) return l == 0 ? 1.0 : 0.0;
-    double s, c;
-    sincos(x, &s, &c);
-    const double j0 = s / x, j1 = s / (x * x) - c / x;
     if (l == 0) return j0;
     if (l == 1) return j1;
     if (x > (double)l) {
         double a = j0, bb = j1;
         for (int k = 1; k < l; ++k) {
-            const double nx = (2.0 * k + 1.0) / x * bb - a;
+            const double nx = (2.0 * k + 1.0) * ix * bb - a;
             a = bb;
             bb = nx;
         }
@@ -59,7 +58,7 @@ __device__ double sph_bessel(int l, double x) {
         if (k == l) fl = f;
         if (k == 1) f1 = f;
         if (k == 0) f0 = f;
-        const double fm1 = (2.0 * k + 1.0) / x * f - fp1;
+        const double fm1 = (2.0 * k + 1.0) * ix * f - fp1;
         fp1 = f;
         f = fm1;
         if (fabs(f) > 1e250) {
@@ -106,28 +105,31 @@ __device__ double cyl_bessel(int n, double x) {
     return fn / norm;
 }
 
-// sqrt((2l+1)/(4 pi) (l-m)!/(l+m)!) P_l^m(x), m >= 0, Condon-Shortley phase.
-__device__ double ylm_theta(int l, int m, double x) {
-    const double somx2 = sqrt((1.0 - x) * (1.0 + x));
+// P_l^m(x) with the Condon-Shortley phase, m >= 0, somx2 = sqrt(1 - x^2) given
+// (the caller applies the normalisation ylm_norm).
+__device__ double plm_raw(int l, int m, double x, double somx2) {
     double pmm = 1.0, fact = 1.0;
     for (int i = 1; i <= m; ++i) {
         pmm *= -fact * somx2;
         fact += 2.0;
     }
-    double plm = pmm;
-    if (l > m) {
-        double pmmp1 = x * (2.0 * m + 1.0) * pmm;
-        plm = pmmp1;
-        for (int ll = m + 2; ll <= l; ++ll) {
-            const double pll = ((2.0 * ll - 1.0) * x * pmmp1 - (ll + m - 1.0) * pmm) / (ll - m);
-            pmm = pmmp1;
-            pmmp1 = pll;
-            plm = pll;
-        }
+    if (l == m) return pmm;
+    double pmmp1 = x * (2.0 * m + 1.0) * pmm;
+    double plm = pmmp1;
+    for (int ll = m + 2; ll <= l; ++ll) {
+        const double pll = ((2.0 * ll - 1.0) * x * pmmp1 - (ll + m - 1.0) * pmm) / (ll - m);
+        pmm = pmmp1;
+        pmmp1 = pll;
+        plm = pll;
     }
+    return plm;
+}
+
+// sqrt((2l+1)/(4 pi) (l-m)!/(l+m)!), m >= 0
+__device__ double ylm_norm(int l, int m) {
     double ratio = 1.0;
     for (int k = l - m + 1; k <= l + m; ++k) ratio /= (double)k;
-    return sqrt((2.0 * l + 1.0) / (4.0 * kPi) * ratio) * plm;
+    return sqrt((2.0 * l + 1.0) / (4.0 * kPi) * ratio);
 }
 
 // coor3d.cpp:168-202
@@ -186,9 +188,12 @@ __global__ __launch_bounds__(256) void multipole_kernel(MultipoleKernelArgs a) {
         ls[k] = a.moments[2 * kk];
         ms[k] = a.moments[2 * kk + 1];
     }
-    double ar[kMoments], ai[kMoments];
+    double ar[kMoments], ai[kMoments], ynorm[kMoments];
 #pragma unroll
-    for (int k = 0; k < kMoments; ++k) ar[k] = ai[k] = 0.0;
+    for (int k = 0; k < kMoments; ++k) {
+        ar[k] = ai[k] = 0.0;
+        ynorm[k] = ylm_norm(ls[k], ms[k] < 0 ? -ms[k] : ms[k]);
+    }
     const float *brow = a.b + (a.per_shell_b ? (size_t)s * a.NAp : 0);
     double b9[9];
     if (a.kind == 1)
@@ -203,14 +208,24 @@ __global__ __launch_bounds__(256) void multipole_kernel(MultipoleKernelArgs a) {
             to_spherical(x, y, z, r, phi, theta);
             const double rf = (float)r, phif = (float)phi, thetaf = (float)theta;
             const double ct = cos(thetaf);
+            const double somx2 = sqrt((1.0 - ct) * (1.0 + ct));
             double sp, cp;
             sincos(phif, &sp, &cp);
+            // the argument's sin/cos, j_0, j_1 and 1/x once for all moments
+            const double xq = ql * rf;
+            double sx = 0.0, cx = 1.0, ix = 0.0, j0 = 1.0, j1 = 0.0;
+            if (xq != 0.0) {
+                sincos(xq, &sx, &cx);
+                ix = 1.0 / xq;
+                j0 = sx * ix;
+                j1 = (j0 - cx) * ix;
+            }
 #pragma unroll
             for (int k = 0; k < kMoments; ++k) {
                 if (k >= nk) break;
                 const int l = ls[k], m = ms[k], am = m < 0 ? -m : m;
-                const double jl = sph_bessel(l, ql * rf);
-                const double th = ylm_theta(l, am, ct);
+                const double jl = sph_bessel_from(l, xq, ix, j0, j1);
+                const double th = ynorm[k] * plm_raw(l, am, ct, somx2);
                 // e^{i am phi} by repeated products of e^{i phi}
                 double er = 1.0, ei = 0.0;
                 for (int q = 0; q < am; ++q) {
