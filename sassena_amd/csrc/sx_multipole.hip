@@ -35,39 +35,59 @@ struct MultipoleKernelArgs {
     float2 *amp;            // [ns * nmo][NFp]
 };
 
-// j_l(x) for one l >= 0, x >= 0, from j_0, j_1 and 1/x computed once per
-// argument (shared by the moments of a wavefront): upward recurrence where
-// stable (x > l), else Miller's downward recurrence normalised to j_0 (or j_1
-// near a zero of j_0).
-__device__ double sph_bessel_from(int l, double x, double ix, double j0, double j1) {
-    if (x == 0.0) return l == 0 ? 1.0 : 0.0;
-    if (l == 0) return j0;
-    if (l == 1) return j1;
-    if (x > (double)l) {
-        double a = j0, bb = j1;
-        for (int k = 1; k < l; ++k) {
-            const double nx = (2.0 * k + 1.0) * ix * bb - a;
-            a = bb;
-            bb = nx;
-        }
-        return bb;
+// j_l(x) for the moment degrees ls[0..kMoments) of a wavefront in one
+// recurrence: upward from j_0, j_1 to lhi where that is stable (x > lhi),
+// else Miller's downward recurrence from above lhi normalised to j_0 (or j_1
+// near a zero of j_0); each degree is captured as the recurrence passes it.
+__device__ __forceinline__ void sph_bessel_moments(const int (&ls)[kMoments], int lhi, double x, double ix,
+                                                   double j0, double j1, double (&jv)[kMoments]) {
+    if (x == 0.0) {
+#pragma unroll
+        for (int k = 0; k < kMoments; ++k) jv[k] = ls[k] == 0 ? 1.0 : 0.0;
+        return;
     }
-    const int N = l + 32 + (int)(2.0 * sqrt((double)l + x));
-    double fp1 = 0.0, f = 1e-280, fl = 0.0, f0 = 0.0, f1 = 0.0;
-    for (int k = N; k >= 0; --k) {
-        if (k == l) fl = f;
-        if (k == 1) f1 = f;
-        if (k == 0) f0 = f;
-        const double fm1 = (2.0 * k + 1.0) * ix * f - fp1;
+#pragma unroll
+    for (int k = 0; k < kMoments; ++k) jv[k] = ls[k] == 0 ? j0 : j1;  // degrees >= 2 are set below
+    if (lhi < 2) return;
+    if (x > (double)lhi) {
+        double a = j0, bb = j1;
+        for (int n = 1; n < lhi; ++n) {
+            const double nx = (2.0 * n + 1.0) * ix * bb - a;
+            a = bb;
+            bb = nx;  // j_{n+1}
+#pragma unroll
+            for (int k = 0; k < kMoments; ++k)
+                if (ls[k] == n + 1) jv[k] = bb;
+        }
+        return;
+    }
+    const int N = lhi + 32 + (int)(2.0 * sqrt((double)lhi + x));
+    double fp1 = 0.0, f = 1e-280, f0 = 0.0, f1 = 0.0;
+    double cap[kMoments];
+#pragma unroll
+    for (int k = 0; k < kMoments; ++k) cap[k] = 0.0;
+    for (int n = N; n >= 0; --n) {
+#pragma unroll
+        for (int k = 0; k < kMoments; ++k)
+            if (ls[k] == n) cap[k] = f;
+        if (n == 1) f1 = f;
+        if (n == 0) f0 = f;
+        const double fm1 = (2.0 * n + 1.0) * ix * f - fp1;
         fp1 = f;
         f = fm1;
         if (fabs(f) > 1e250) {
             f *= 1e-250;
             fp1 *= 1e-250;
-            fl *= 1e-250;
+            f0 *= 1e-250;  // captured values share the running scale
+            f1 *= 1e-250;
+#pragma unroll
+            for (int k = 0; k < kMoments; ++k) cap[k] *= 1e-250;
         }
     }
-    return fabs(j0) >= fabs(j1) ? fl * (j0 / f0) : fl * (j1 / f1);
+    const double sc = fabs(j0) >= fabs(j1) ? j0 / f0 : j1 / f1;
+#pragma unroll
+    for (int k = 0; k < kMoments; ++k)
+        if (ls[k] >= 2) jv[k] = cap[k] * sc;
 }
 
 // J_n(x) for n >= 0, x >= 0: upward recurrence from J_0, J_1 where x > n,
@@ -117,7 +137,11 @@ __device__ double plm_raw(int l, int m, double x, double somx2) {
     double pmmp1 = x * (2.0 * m + 1.0) * pmm;
     double plm = pmmp1;
     for (int ll = m + 2; ll <= l; ++ll) {
-        const double pll = ((2.0 * ll - 1.0) * x * pmmp1 - (ll + m - 1.0) * pmm) / (ll - m);
+        // 1 / (ll - m): hardware reciprocal and one Newton step (no division)
+        const double dd = (double)(ll - m);
+        double rr = __builtin_amdgcn_rcp(dd);
+        rr = fma(rr, fma(-dd, rr, 1.0), rr);
+        const double pll = ((2.0 * ll - 1.0) * x * pmmp1 - (ll + m - 1.0) * pmm) * rr;
         pmm = pmmp1;
         pmmp1 = pll;
         plm = pll;
@@ -189,10 +213,12 @@ __global__ __launch_bounds__(256) void multipole_kernel(MultipoleKernelArgs a) {
         ms[k] = a.moments[2 * kk + 1];
     }
     double ar[kMoments], ai[kMoments], ynorm[kMoments];
+    int lhi = 0;
 #pragma unroll
     for (int k = 0; k < kMoments; ++k) {
         ar[k] = ai[k] = 0.0;
         ynorm[k] = ylm_norm(ls[k], ms[k] < 0 ? -ms[k] : ms[k]);
+        lhi = max(lhi, ls[k]);
     }
     const float *brow = a.b + (a.per_shell_b ? (size_t)s * a.NAp : 0);
     double b9[9];
@@ -220,11 +246,13 @@ __global__ __launch_bounds__(256) void multipole_kernel(MultipoleKernelArgs a) {
                 j0 = sx * ix;
                 j1 = (j0 - cx) * ix;
             }
+            double jv[kMoments];
+            sph_bessel_moments(ls, lhi, xq, ix, j0, j1, jv);
 #pragma unroll
             for (int k = 0; k < kMoments; ++k) {
                 if (k >= nk) break;
                 const int l = ls[k], m = ms[k], am = m < 0 ? -m : m;
-                const double jl = sph_bessel_from(l, xq, ix, j0, j1);
+                const double jl = jv[k];
                 const double th = ynorm[k] * plm_raw(l, am, ct, somx2);
                 // e^{i am phi} by repeated products of e^{i phi}
                 double er = 1.0, ei = 0.0;
