@@ -39,6 +39,10 @@ struct MultipoleKernelArgs {
 // recurrence: upward from j_0, j_1 to lhi where that is stable (x > lhi),
 // else Miller's downward recurrence from above lhi normalised to j_0 (or j_1
 // near a zero of j_0); each degree is captured as the recurrence passes it.
+// SORTED: ls is non-decreasing (the reference's moment lists), so the
+// recurrence runs in segments between consecutive degrees with no per-step
+// comparisons; otherwise every step compares with every degree.
+template <bool SORTED>
 __device__ __forceinline__ void sph_bessel_moments(const int (&ls)[kMoments], int lhi, double x, double ix,
                                                    double j0, double j1, double (&jv)[kMoments]) {
     if (x == 0.0) {
@@ -50,41 +54,66 @@ __device__ __forceinline__ void sph_bessel_moments(const int (&ls)[kMoments], in
     for (int k = 0; k < kMoments; ++k) jv[k] = ls[k] == 0 ? j0 : j1;  // degrees >= 2 are set below
     if (lhi < 2) return;
     if (x > (double)lhi) {
-        double a = j0, bb = j1;
-        for (int n = 1; n < lhi; ++n) {
-            const double nx = (2.0 * n + 1.0) * ix * bb - a;
-            a = bb;
-            bb = nx;  // j_{n+1}
+        double a = j0, bb = j1;  // j_{n-1}, j_n
+        int n = 1;
+        if constexpr (SORTED) {
 #pragma unroll
-            for (int k = 0; k < kMoments; ++k)
-                if (ls[k] == n + 1) jv[k] = bb;
+            for (int k = 0; k < kMoments; ++k) {
+                for (; n < ls[k]; ++n) {
+                    const double nx = (2.0 * n + 1.0) * ix * bb - a;
+                    a = bb;
+                    bb = nx;
+                }
+                if (ls[k] >= 2) jv[k] = bb;
+            }
+        } else {
+            for (; n < lhi; ++n) {
+                const double nx = (2.0 * n + 1.0) * ix * bb - a;
+                a = bb;
+                bb = nx;  // j_{n+1}
+#pragma unroll
+                for (int k = 0; k < kMoments; ++k)
+                    if (ls[k] == n + 1) jv[k] = bb;
+            }
         }
         return;
     }
     const int N = lhi + 32 + (int)(2.0 * sqrt((double)lhi + x));
-    double fp1 = 0.0, f = 1e-280, f0 = 0.0, f1 = 0.0;
+    double fp1 = 0.0, f = 1e-280;  // f_{n+1}, f_n
     double cap[kMoments];
 #pragma unroll
     for (int k = 0; k < kMoments; ++k) cap[k] = 0.0;
-    for (int n = N; n >= 0; --n) {
-#pragma unroll
-        for (int k = 0; k < kMoments; ++k)
-            if (ls[k] == n) cap[k] = f;
-        if (n == 1) f1 = f;
-        if (n == 0) f0 = f;
+    int n = N;
+    // one downward step: f_n -> f_{n-1}, rescaling everything captured so far
+    auto step = [&]() {
         const double fm1 = (2.0 * n + 1.0) * ix * f - fp1;
         fp1 = f;
         f = fm1;
+        --n;
         if (fabs(f) > 1e250) {
             f *= 1e-250;
             fp1 *= 1e-250;
-            f0 *= 1e-250;  // captured values share the running scale
-            f1 *= 1e-250;
 #pragma unroll
             for (int k = 0; k < kMoments; ++k) cap[k] *= 1e-250;
         }
+    };
+    if constexpr (SORTED) {
+#pragma unroll
+        for (int k = kMoments - 1; k >= 0; --k) {
+            while (n > ls[k]) step();
+            cap[k] = f;  // f_{ls[k]}
+        }
+    } else {
+        while (n >= 2) {
+#pragma unroll
+            for (int k = 0; k < kMoments; ++k)
+                if (ls[k] == n) cap[k] = f;
+            step();
+        }
     }
-    const double sc = fabs(j0) >= fabs(j1) ? j0 / f0 : j1 / f1;
+    while (n > 0) step();
+    // now f = f_0 and fp1 = f_1, at the same scale as every capture
+    const double sc = fabs(j0) >= fabs(j1) ? j0 / f : j1 / fp1;
 #pragma unroll
     for (int k = 0; k < kMoments; ++k)
         if (ls[k] >= 2) jv[k] = cap[k] * sc;
@@ -214,11 +243,13 @@ __global__ __launch_bounds__(256) void multipole_kernel(MultipoleKernelArgs a) {
     }
     double ar[kMoments], ai[kMoments], ynorm[kMoments];
     int lhi = 0;
+    bool sorted = true;  // degrees non-decreasing over the wavefront's moments
 #pragma unroll
     for (int k = 0; k < kMoments; ++k) {
         ar[k] = ai[k] = 0.0;
         ynorm[k] = ylm_norm(ls[k], ms[k] < 0 ? -ms[k] : ms[k]);
         lhi = max(lhi, ls[k]);
+        if (k > 0 && ls[k] < ls[k - 1]) sorted = false;
     }
     const float *brow = a.b + (a.per_shell_b ? (size_t)s * a.NAp : 0);
     double b9[9];
@@ -247,7 +278,10 @@ __global__ __launch_bounds__(256) void multipole_kernel(MultipoleKernelArgs a) {
                 j1 = (j0 - cx) * ix;
             }
             double jv[kMoments];
-            sph_bessel_moments(ls, lhi, xq, ix, j0, j1, jv);
+            if (sorted)
+                sph_bessel_moments<true>(ls, lhi, xq, ix, j0, j1, jv);
+            else
+                sph_bessel_moments<false>(ls, lhi, xq, ix, j0, j1, jv);
 #pragma unroll
             for (int k = 0; k < kMoments; ++k) {
                 if (k >= nk) break;

@@ -77,6 +77,20 @@ def test_large_moment_lists_and_per_shell_factors(oracle_mod):
         check(fqt[s], fq[s], fq2[s], oracle_mod.compute_multipole("sphere", frames, bs[s], qv[s], mom))
 
 
+def test_unsorted_moment_list(oracle_mod):
+    """Moments in no particular order (the Bessel recurrence then compares
+    every step with every degree instead of running in segments), small and
+    large |q| r (Miller's and the upward recurrence)."""
+    frames, b = centred_walk(90, 140, 14)
+    qv = np.array([[0.0, 0.05, 0.0], [0.7, -0.4, 0.3], [2.5, 0.0, 1.0]])
+    mom = oracle_mod.sphere_moments(9)
+    order = np.random.default_rng(3).permutation(len(mom))
+    mom = [mom[i] for i in order]
+    fqt, fq, fq2 = run("sphere", frames, b, qv, mom)
+    for s in range(len(qv)):
+        check(fqt[s], fq[s], fq2[s], oracle_mod.compute_multipole("sphere", frames, b, qv[s], mom))
+
+
 def test_sphere_converges_to_vector_average(oracle_mod):
     """A static cluster: the truncated multipole sum and a dense vector
     average both tend to the Debye sum."""
