@@ -97,6 +97,8 @@ int pick_split(const sx_ctx *c, int V, int vb) {
 
 // Amplitude bytes the extra atom ranges of round_filling_split may add.
 constexpr size_t kSplitAmpBudget = (size_t)4 << 30;
+// Largest buffer of per-(atom, frame) spherical terms of the sphere multipole path.
+constexpr size_t kSphereTermBudget = (size_t)16 << 30;
 
 // Splits from `first` to 16 (at least 64 atoms a range, partial amplitude
 // sets of split_bytes each within kSplitAmpBudget) for `base` equal
@@ -381,6 +383,14 @@ int run_batch(sx_ctx *c, int ns, size_t NM, const double *qv, const double *qlen
             ma.b = bptr;
             ma.per_shell_b = b_shell ? 1 : 0;
             ma.amp = c->amp.as<float2>();
+            // Sphere: the spherical terms of every (atom, frame) once per call
+            // (36 B each) when they fit the budget; else the kernel forms them.
+            const size_t sph_bytes = sx::kSphereTermBytes * c->NAp * c->NFp;
+            if (mp->kind == 0 && sph_bytes <= kSphereTermBudget) {
+                SX_HIP(c, c->mpsph.ensure(sph_bytes));
+                SX_HIP(c, sx::launch_multipole_sphere_terms(ma, c->mpsph.p, c->stream));
+                ma.sph = c->mpsph.p;
+            }
             SX_HIP(c, sx::launch_multipole(ma, c->stream));
         } else if (oriented) {
             sx::OrientArgs oa;
@@ -648,7 +658,7 @@ void sx_destroy(sx_ctx *c) {
     if (!c) return;
     (void)hipSetDevice(c->cfg.device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
-    for (DevBuf *d : {&c->coords, &c->b, &c->twL, &c->twL64, &c->twp, &c->twp64, &c->tw64, &c->w, &c->aidx, &c->apair, &c->tail, &c->qrev, &c->qlen, &c->uvec, &c->brow, &c->bshell, &c->mpmom, &c->mpq, &c->amp,
+    for (DevBuf *d : {&c->coords, &c->b, &c->twL, &c->twL64, &c->twp, &c->twp64, &c->tw64, &c->w, &c->aidx, &c->apair, &c->tail, &c->qrev, &c->qlen, &c->uvec, &c->brow, &c->bshell, &c->mpmom, &c->mpq, &c->mpsph, &c->amp,
                       &c->ppart, &c->apart, &c->fqt, &c->fq, &c->fq2, &c->stage})
         d->release();
     for (auto &e : c->ev)

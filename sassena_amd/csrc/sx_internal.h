@@ -176,8 +176,15 @@ struct MultipoleArgs {
     const float *b;
     int per_shell_b;
     float2 *amp;           // [ns * nmo][NFp]
+    // sphere: per (atom, frame) spherical terms from launch_multipole_sphere_terms
+    // ([NAp][NFp] planes: r as float, then cos(theta), sin(theta), sin(phi),
+    // cos(phi) as double), or nullptr to form them in the kernel
+    const void *sph = nullptr;
 };
 hipError_t launch_multipole(const MultipoleArgs &m, hipStream_t stream);
+// Bytes of the sphere-term planes for NAp x NFp (atom, frame) pairs.
+constexpr size_t kSphereTermBytes = sizeof(float) + 4 * sizeof(double);
+hipError_t launch_multipole_sphere_terms(const MultipoleArgs &m, void *sph, hipStream_t stream);
 
 // "self" with dsp.type square (mode 1) / plain (mode 2); NF <= kMaxSeriesFrames
 constexpr int kMaxSeriesFrames = 8192;

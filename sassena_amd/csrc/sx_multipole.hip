@@ -33,6 +33,8 @@ struct MultipoleKernelArgs {
     const float *b;         // [nb][NAp]
     int per_shell_b;
     float2 *amp;            // [ns * nmo][NFp]
+    const float *srf;       // sphere terms per (atom, frame), or nullptr: r (float-rounded),
+    const double *sct, *sst, *ssp, *scp;  // cos theta, sin theta, sin phi, cos phi
 };
 
 // j_l(x) for the moment degrees ls[0..kMoments) of a wavefront in one
@@ -207,6 +209,36 @@ __device__ void to_spherical(double x, double y, double z, double &r, double &ph
     phi = p;
 }
 
+// The per-(atom, frame) terms of the sphere average: r, theta, phi rounded to
+// float as the reference stages them, then cos(theta), sin(theta), sin/cos(phi)
+// in FP64.
+__device__ __forceinline__ void sphere_terms(double x, double y, double z, double &rf, double &ct, double &st,
+                                             double &sp, double &cp) {
+    double r, phi, theta;
+    to_spherical(x, y, z, r, phi, theta);
+    rf = (float)r;
+    const double phif = (float)phi, thetaf = (float)theta;
+    ct = cos(thetaf);
+    st = sqrt((1.0 - ct) * (1.0 + ct));
+    sincos(phif, &sp, &cp);
+}
+
+// One thread per (atom, frame): the sphere terms, formed once per call
+// instead of once per (shell, moment block) wavefront.
+__global__ __launch_bounds__(256) void sphere_terms_kernel(const float *__restrict__ X, const float *__restrict__ Y,
+                                                           const float *__restrict__ Z, size_t n, float *srf,
+                                                           double *sct, double *sst, double *ssp, double *scp) {
+    const size_t e = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= n) return;
+    double rf, ct, st, sp, cp;
+    sphere_terms(X[e], Y[e], Z[e], rf, ct, st, sp, cp);
+    srf[e] = (float)rf;
+    sct[e] = ct;
+    sst[e] = st;
+    ssp[e] = sp;
+    scp[e] = cp;
+}
+
 // coor3d.cpp:116-140 (the pi offsets pass through a float-valued sign())
 __device__ void to_cylinder(double x, double y, double z, double &r, double &phi, double &zz) {
     r = sqrt(x * x + y * y);
@@ -258,16 +290,25 @@ __global__ __launch_bounds__(256) void multipole_kernel(MultipoleKernelArgs a) {
     for (size_t j = 0; j < a.NAp; ++j) {
         const float bj = brow[j];
         if (bj == 0.f) continue;  // padding atoms and zero lengths add nothing
-        const double x = a.X[j * a.NFp + t], y = a.Y[j * a.NFp + t], z = a.Z[j * a.NFp + t];
+        double x = 0.0, y = 0.0, z = 0.0;
+        if (a.kind != 0 || !a.srf) {
+            x = a.X[j * a.NFp + t];
+            y = a.Y[j * a.NFp + t];
+            z = a.Z[j * a.NFp + t];
+        }
         if (a.kind == 0) {
             const double ql = a.qinfo[s];
-            double r, phi, theta;
-            to_spherical(x, y, z, r, phi, theta);
-            const double rf = (float)r, phif = (float)phi, thetaf = (float)theta;
-            const double ct = cos(thetaf);
-            const double somx2 = sqrt((1.0 - ct) * (1.0 + ct));
-            double sp, cp;
-            sincos(phif, &sp, &cp);
+            double rf, ct, somx2, sp, cp;
+            if (a.srf) {
+                const size_t e = j * a.NFp + t;
+                rf = a.srf[e];
+                ct = a.sct[e];
+                somx2 = a.sst[e];
+                sp = a.ssp[e];
+                cp = a.scp[e];
+            } else {
+                sphere_terms(x, y, z, rf, ct, somx2, sp, cp);
+            }
             // the argument's sin/cos, j_0, j_1 and 1/x once for all moments
             const double xq = ql * rf;
             double sx = 0.0, cx = 1.0, ix = 0.0, j0 = 1.0, j1 = 0.0;
@@ -372,8 +413,30 @@ hipError_t launch_multipole(const MultipoleArgs &m, hipStream_t stream) {
     k.b = m.b;
     k.per_shell_b = m.per_shell_b;
     k.amp = m.amp;
+    const size_t n = m.NAp * m.NFp;
+    if (m.sph && m.kind == 0) {
+        k.srf = static_cast<const float *>(m.sph);
+        const double *d = reinterpret_cast<const double *>(static_cast<const char *>(m.sph) + sizeof(float) * n);
+        k.sct = d;
+        k.sst = d + n;
+        k.ssp = d + 2 * n;
+        k.scp = d + 3 * n;
+    } else {
+        k.srf = nullptr;
+        k.sct = k.sst = k.ssp = k.scp = nullptr;
+    }
     dim3 grid((unsigned)(m.NFp / 64), (unsigned)m.ns, (unsigned)((k.mblocks + 3) / 4));
     hipLaunchKernelGGL(multipole_kernel, grid, dim3(256), 0, stream, k);
+    return hipGetLastError();
+}
+
+hipError_t launch_multipole_sphere_terms(const MultipoleArgs &m, void *sph, hipStream_t stream) {
+    const size_t n = m.NAp * m.NFp;
+    float *srf = static_cast<float *>(sph);
+    double *d = reinterpret_cast<double *>(static_cast<char *>(sph) + sizeof(float) * n);
+    const float *X = m.coords, *Y = m.coords + n, *Z = m.coords + 2 * n;
+    hipLaunchKernelGGL(sphere_terms_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, X, Y, Z, n, srf,
+                       d, d + n, d + 2 * n, d + 3 * n);
     return hipGetLastError();
 }
 
