@@ -267,6 +267,40 @@ def test_c2_full_size_oriented_subset(sx, oracle_mod):
         check_close(fqt[s], fq[s], fq2[s], *oracle_mod.compute("all", frames, T.b, qlen[s] * u, nthreads=8))
 
 
+def test_c2_full_headline_properties(sx):
+    """The whole headline workload (20000 atoms x 2000 frames, 40 shells x 200
+    orientations) through the oriented entry point, checked by properties that
+    hold at any size: a repeat call reproduces bit for bit; tile shapes 1 and 5
+    (different tiles, atom ranges and summation orders) agree to the parity
+    tolerance; the explicit-vector entry point (a different kernel, one FP64
+    phase per vector) agrees on every fifth shell; F(q, 0) is the mean
+    |A_v(t)|^2 over vectors and frames, so it is real and positive; fq and fq2
+    are the means over vectors of a_v = <C_v(tau)>_tau and of |a_v|^2, so
+    fq2 >= |fq|^2."""
+    from sassena_amd import synth
+    T = synth.case("c2")
+    res = {}
+    for shape in (5, 1):
+        with sx.ScatterDevice("all", tile_shape=shape) as dev:
+            dev.stage(T.xyz, layout="soa")
+            dev.set_factors(T.b)
+            res[shape] = dev.compute_oriented(T.qlen, T.orient)
+            if shape == 5:
+                again = dev.compute_oriented(T.qlen, T.orient)
+                for x, y in zip(res[shape], again):
+                    assert np.array_equal(x, y)
+                vec = {s: dev.compute_shell(T.qvectors(s)) for s in range(0, T.nshells, 5)}
+    fqt, fq, fq2 = res[5]
+    assert fqt.shape[0] == T.nshells == 40 and T.orient.shape[0] == 200
+    for s in range(T.nshells):
+        check_close(fqt[s], fq[s], fq2[s], res[1][0][s], res[1][1][s], res[1][2][s])
+        f0 = fqt[s][0]
+        assert f0.real > 0 and abs(f0.imag) <= 1e-5 * f0.real
+        assert fq2[s].real >= abs(fq[s]) ** 2 * (1 - 1e-5)
+    for s, (vfqt, vfq, vfq2) in vec.items():
+        check_close(fqt[s], fq[s], fq2[s], vfqt, vfq, vfq2)
+
+
 @pytest.mark.parametrize("shape", [1, 2, 3, 4, 5])
 def test_shell_progression_recurrence(sx, oracle_mod, shape):
     """Linearly spaced shells take the stepped-phasor path (two sin/cos pairs
