@@ -328,6 +328,39 @@ def test_shell_progression_recurrence(sx, oracle_mod, shape):
             check_close(fqt[s], fq[s], fq2[s], *refs[key])
 
 
+def test_c3_full_size_self_properties(sx, oracle_mod):
+    """The incoherent workload at full size (50000 atoms x 5000 frames, two of
+    its shells x 100 orientations): F_s(q, 0) = sum_j b_j^2 at every q, since
+    each atom's series has modulus b_j; a repeat call reproduces bit for bit;
+    the one-after-the-other half layout agrees with the default to the parity
+    tolerance; fq and fq2 sum a_jv = <C_jv(tau)>_tau and |a_jv|^2 over the N
+    atoms and average over vectors, so fq2 >= |fq|^2 / N (Cauchy-Schwarz); and
+    one orientation of the outer shell against the oracle."""
+    from sassena_amd import synth
+    T = synth.case("c3")
+    qlen = T.qlen[[0, T.nshells - 1]]
+    b2 = float(np.sum(T.b.astype(np.float64) ** 2))
+    with sx.ScatterDevice("self") as dev:
+        dev.stage(T.xyz, layout="soa")
+        dev.set_factors(T.b)
+        fqt, fq, fq2 = dev.compute_oriented(qlen, T.orient)
+        again = dev.compute_oriented(qlen, T.orient)
+        sub = dev.compute_oriented(qlen[1:], T.orient[:1])
+    for x, y in zip((fqt, fq, fq2), again):
+        assert np.array_equal(x, y)
+    with sx.ScatterDevice("self", fft_halves=1) as dev:
+        dev.stage(T.xyz, layout="soa")
+        dev.set_factors(T.b)
+        h1 = dev.compute_oriented(qlen, T.orient)
+    for s in range(2):
+        assert abs(fqt[s][0] - b2) <= TOL * b2, (fqt[s][0], b2)
+        assert fq2[s].real >= abs(fq[s]) ** 2 / np.count_nonzero(T.b) * (1 - 1e-5)
+        check_close(fqt[s], fq[s], fq2[s], h1[0][s], h1[1][s], h1[2][s])
+    atoms = np.ascontiguousarray(synth.to_frames_aos(T.xyz).transpose(1, 0, 2))
+    ref = oracle_mod.compute("self", atoms, T.b, qlen[1] * T.orient[:1], nthreads=16)
+    check_close(sub[0][0], sub[1][0], sub[2][0], *ref)
+
+
 @pytest.mark.parametrize("halves", [3, 2, 1])
 @pytest.mark.parametrize("NF", [40, 300, 2000, 5000, 9000])
 def test_self_half_layouts(sx, oracle_mod, NF, halves):
