@@ -120,6 +120,21 @@ int round_filling_split(long base, int first, long resident, size_t NAp, size_t 
     return split;
 }
 
+// Atom ranges of the multipole kernel.  Its workgroups each sum every atom of
+// their range for one (frame tile, shell, group of 4 moment blocks): the rule
+// above, then the split that fills whole rounds of resident workgroups.
+int pick_multipole_split(sx_ctx *c, int ns, int nmo, int *split) {
+    const long base = sx::multipole_base_blocks(c->NFp, ns, nmo);
+    *split = pick_split_blocks(c, base);
+    if (c->cfg.atom_split > 0) return SX_OK;
+    int per_cu = 0, cus = 0;
+    SX_HIP(c, sx::multipole_blocks_per_cu(&per_cu));
+    SX_HIP(c, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->cfg.device));
+    *split = round_filling_split(base, *split, (long)std::max(per_cu, 1) * std::max(cus, 1), c->NAp,
+                                 sizeof(float2) * c->NFp * (size_t)ns * nmo);
+    return SX_OK;
+}
+
 // Atom ranges of the general-vector kernel: the rule above, then the split
 // that fills whole rounds of resident workgroups (see pick_oriented_split).
 int pick_all_split(sx_ctx *c, int V, int vb, bool per_row, int *split) {
@@ -359,6 +374,10 @@ int run_batch(sx_ctx *c, int ns, size_t NM, const double *qv, const double *qlen
                                          b_shell != nullptr, (size_t)ns * NM, &split);
             if (rc != SX_OK) return rc;
         }
+        if (mp) {
+            int rc = pick_multipole_split(c, ns, (int)NM, &split);
+            if (rc != SX_OK) return rc;
+        }
         const size_t split_stride = (size_t)V * c->NFp;
         SX_HIP(c, c->amp.ensure(sizeof(float2) * split_stride * split));
         if (mp) {
@@ -383,6 +402,7 @@ int run_batch(sx_ctx *c, int ns, size_t NM, const double *qv, const double *qlen
             ma.b = bptr;
             ma.per_shell_b = b_shell ? 1 : 0;
             ma.amp = c->amp.as<float2>();
+            ma.split = split;
             // Sphere: the spherical terms of every (atom, frame) once per call
             // (36 B each) when they fit the budget; else the kernel forms them.
             const size_t sph_bytes = sx::kSphereTermBytes * c->NAp * c->NFp;

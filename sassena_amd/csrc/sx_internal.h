@@ -175,13 +175,17 @@ struct MultipoleArgs {
     int nmo, ns;
     const float *b;
     int per_shell_b;
-    float2 *amp;           // [ns * nmo][NFp]
+    float2 *amp;           // [split][ns * nmo][NFp]: one partial set per atom range
+    int split = 1;
     // sphere: per (atom, frame) spherical terms from launch_multipole_sphere_terms
     // ([NAp][NFp] planes: r as float, then cos(theta), sin(theta), sin(phi),
     // cos(phi) as double), or nullptr to form them in the kernel
     const void *sph = nullptr;
 };
 hipError_t launch_multipole(const MultipoleArgs &m, hipStream_t stream);
+// Workgroups of the multipole kernel that fit one CU, and its workgroups per atom range.
+hipError_t multipole_blocks_per_cu(int *blocks);
+int multipole_base_blocks(size_t NFp, int ns, int nmo);
 // Bytes of the sphere-term planes for NAp x NFp (atom, frame) pairs.
 constexpr size_t kSphereTermBytes = sizeof(float) + 4 * sizeof(double);
 hipError_t launch_multipole_sphere_terms(const MultipoleArgs &m, void *sph, hipStream_t stream);

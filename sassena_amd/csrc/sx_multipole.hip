@@ -13,6 +13,7 @@
 // Lane = frame; a wavefront accumulates up to kMoments moments of one
 // (shell, moment block) in FP64; the amplitudes feed the same correlation
 // and finalisation kernels as the vector averages.
+#include <algorithm>
 #include <cmath>
 
 #include "sx_internal.h"
@@ -32,7 +33,9 @@ struct MultipoleKernelArgs {
     int nmo, ns, mblocks;
     const float *b;         // [nb][NAp]
     int per_shell_b;
-    float2 *amp;            // [ns * nmo][NFp]
+    float2 *amp;            // [split][ns * nmo][NFp]
+    int split, zgroups;     // atom ranges; groups of 4 moment blocks per range
+    size_t range;           // atoms per range (the last may be shorter)
     const float *srf;       // sphere terms per (atom, frame), or nullptr: r (float-rounded),
     const double *sct, *sst, *ssp, *scp;  // cos theta, sin theta, sin phi, cos phi
 };
@@ -258,10 +261,12 @@ __device__ void to_cylinder(double x, double y, double z, double &r, double &phi
 __global__ __launch_bounds__(256) void multipole_kernel(MultipoleKernelArgs a) {
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
-    // block = (frame tile, shell, group of 4 moment blocks)
+    // block = (frame tile, shell, group of 4 moment blocks + zgroups x atom range)
     const int ftile = blockIdx.x;
     const int s = blockIdx.y;
-    const int mb = blockIdx.z * 4 + wave;
+    const int zg = blockIdx.z % a.zgroups;
+    const int sp = blockIdx.z / a.zgroups;
+    const int mb = zg * 4 + wave;
     if (mb >= a.mblocks) return;  // whole wavefront exits; the kernel has no barriers
     const int k0 = mb * kMoments;
     const int nk = min(kMoments, a.nmo - k0);
@@ -287,7 +292,8 @@ __global__ __launch_bounds__(256) void multipole_kernel(MultipoleKernelArgs a) {
     double b9[9];
     if (a.kind == 1)
         for (int k = 0; k < 9; ++k) b9[k] = a.base[k];
-    for (size_t j = 0; j < a.NAp; ++j) {
+    const size_t jend = min(a.NAp, (size_t)(sp + 1) * a.range);
+    for (size_t j = (size_t)sp * a.range; j < jend; ++j) {
         const float bj = brow[j];
         if (bj == 0.f) continue;  // padding atoms and zero lengths add nothing
         double x = 0.0, y = 0.0, z = 0.0;
@@ -391,7 +397,7 @@ __global__ __launch_bounds__(256) void multipole_kernel(MultipoleKernelArgs a) {
 #pragma unroll
     for (int k = 0; k < kMoments; ++k) {
         if (k >= nk) break;
-        const size_t v = (size_t)s * a.nmo + k0 + k;
+        const size_t v = ((size_t)sp * a.ns + s) * a.nmo + k0 + k;
         a.amp[v * a.NFp + t] = make_float2((float)(norm * ar[k]), (float)(norm * ai[k]));
     }
 }
@@ -413,6 +419,9 @@ hipError_t launch_multipole(const MultipoleArgs &m, hipStream_t stream) {
     k.b = m.b;
     k.per_shell_b = m.per_shell_b;
     k.amp = m.amp;
+    k.split = std::max(m.split, 1);
+    k.zgroups = (k.mblocks + 3) / 4;
+    k.range = (m.NAp + k.split - 1) / k.split;
     const size_t n = m.NAp * m.NFp;
     if (m.sph && m.kind == 0) {
         k.srf = static_cast<const float *>(m.sph);
@@ -425,9 +434,19 @@ hipError_t launch_multipole(const MultipoleArgs &m, hipStream_t stream) {
         k.srf = nullptr;
         k.sct = k.sst = k.ssp = k.scp = nullptr;
     }
-    dim3 grid((unsigned)(m.NFp / 64), (unsigned)m.ns, (unsigned)((k.mblocks + 3) / 4));
+    if ((size_t)k.zgroups * k.split > 65535) return hipErrorInvalidValue;
+    dim3 grid((unsigned)(m.NFp / 64), (unsigned)m.ns, (unsigned)(k.zgroups * k.split));
     hipLaunchKernelGGL(multipole_kernel, grid, dim3(256), 0, stream, k);
     return hipGetLastError();
+}
+
+hipError_t multipole_blocks_per_cu(int *blocks) {
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, multipole_kernel, 256, 0);
+}
+
+int multipole_base_blocks(size_t NFp, int ns, int nmo) {
+    const int mblocks = (nmo + kMoments - 1) / kMoments;
+    return (int)(NFp / 64) * ns * ((mblocks + 3) / 4);
 }
 
 hipError_t launch_multipole_sphere_terms(const MultipoleArgs &m, void *sph, hipStream_t stream) {

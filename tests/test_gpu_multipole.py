@@ -34,9 +34,10 @@ def check(fqt, fq, fq2, ref):
     assert abs(fq2 - rfq2) <= TOL * (abs(rfq2) + 1e-3 * scale ** 2), (fq2, rfq2)
 
 
-def run(kind, frames, b, qv, moments, axis=(0.0, 0.0, 1.0), dsp="autocorrelate", method="fftw", b_shell=None):
+def run(kind, frames, b, qv, moments, axis=(0.0, 0.0, 1.0), dsp="autocorrelate", method="fftw", b_shell=None,
+        atom_split=0):
     import sassena_amd
-    with sassena_amd.ScatterDevice("all", dsp, method) as dev:
+    with sassena_amd.ScatterDevice("all", dsp, method, atom_split=atom_split) as dev:
         dev.stage(frames, layout="frames")
         dev.set_factors(b)
         return dev.compute_multipole(kind, qv, moments, axis=axis, b_shell=b_shell)
@@ -75,6 +76,23 @@ def test_large_moment_lists_and_per_shell_factors(oracle_mod):
     fqt, fq, fq2 = run("sphere", frames, b, qv, mom, b_shell=bs)
     for s in range(len(qv)):
         check(fqt[s], fq[s], fq2[s], oracle_mod.compute_multipole("sphere", frames, bs[s], qv[s], mom))
+
+
+@pytest.mark.parametrize("kind", ["sphere", "cylinder"])
+@pytest.mark.parametrize("split", [1, 5])
+def test_atom_ranges(oracle_mod, kind, split):
+    """The atoms summed in ranges whose partial amplitudes the correlation
+    adds: one range, and five ragged ones (256 padded atoms, 52 a range),
+    with per-shell factors and the square and autocorrelation transforms."""
+    frames, b = centred_walk(230, 150, 17)
+    qv = np.array([[0.2, 0.6, -0.1], [1.3, -0.4, 0.8]])
+    mom = oracle_mod.sphere_moments(5) if kind == "sphere" else oracle_mod.cylinder_moments(4)
+    rng = np.random.default_rng(8)
+    bs = rng.uniform(-2.0, 4.0, size=(len(qv), frames.shape[1]))
+    for dsp in ("autocorrelate", "square"):
+        fqt, fq, fq2 = run(kind, frames, b, qv, mom, dsp=dsp, b_shell=bs, atom_split=split)
+        for s in range(len(qv)):
+            check(fqt[s], fq[s], fq2[s], oracle_mod.compute_multipole(kind, frames, bs[s], qv[s], mom, dsp=dsp))
 
 
 def test_unsorted_moment_list(oracle_mod):
