@@ -123,12 +123,12 @@ int round_filling_split(long base, int first, long resident, size_t NAp, size_t 
 // Atom ranges of the multipole kernel.  Its workgroups each sum every atom of
 // their range for one (frame tile, shell, group of 4 moment blocks): the rule
 // above, then the split that fills whole rounds of resident workgroups.
-int pick_multipole_split(sx_ctx *c, int ns, int nmo, int *split) {
+int pick_multipole_split(sx_ctx *c, int kind, int ns, int nmo, int *split) {
     const long base = sx::multipole_base_blocks(c->NFp, ns, nmo);
     *split = pick_split_blocks(c, base);
     if (c->cfg.atom_split > 0) return SX_OK;
     int per_cu = 0, cus = 0;
-    SX_HIP(c, sx::multipole_blocks_per_cu(&per_cu));
+    SX_HIP(c, sx::multipole_blocks_per_cu(kind, &per_cu));
     SX_HIP(c, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->cfg.device));
     *split = round_filling_split(base, *split, (long)std::max(per_cu, 1) * std::max(cus, 1), c->NAp,
                                  sizeof(float2) * c->NFp * (size_t)ns * nmo);
@@ -375,7 +375,7 @@ int run_batch(sx_ctx *c, int ns, size_t NM, const double *qv, const double *qlen
             if (rc != SX_OK) return rc;
         }
         if (mp) {
-            int rc = pick_multipole_split(c, ns, (int)NM, &split);
+            int rc = pick_multipole_split(c, mp->kind, ns, (int)NM, &split);
             if (rc != SX_OK) return rc;
         }
         const size_t split_stride = (size_t)V * c->NFp;

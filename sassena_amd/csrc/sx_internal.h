@@ -184,7 +184,7 @@ struct MultipoleArgs {
 };
 hipError_t launch_multipole(const MultipoleArgs &m, hipStream_t stream);
 // Workgroups of the multipole kernel that fit one CU, and its workgroups per atom range.
-hipError_t multipole_blocks_per_cu(int *blocks);
+hipError_t multipole_blocks_per_cu(int kind, int *blocks);
 int multipole_base_blocks(size_t NFp, int ns, int nmo);
 // Bytes of the sphere-term planes for NAp x NFp (atom, frame) pairs.
 constexpr size_t kSphereTermBytes = sizeof(float) + 4 * sizeof(double);

@@ -258,7 +258,12 @@ __device__ void to_cylinder(double x, double y, double z, double &r, double &phi
     zz = z;
 }
 
-__global__ __launch_bounds__(256) void multipole_kernel(MultipoleKernelArgs a) {
+// KIND 0: sphere only, bounded to 3 wavefronts per SIMD (its FP64
+// recurrences are latency-bound); KIND -1: the kind read from the arguments
+// (the cylinder path, whose registers the compiler chooses).
+template <int KIND>
+__global__ __launch_bounds__(256, KIND == 0 ? 3 : 1) void multipole_kernel(MultipoleKernelArgs a) {
+    const int kind = KIND < 0 ? a.kind : KIND;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     // block = (frame tile, shell, group of 4 moment blocks + zgroups x atom range)
@@ -290,19 +295,19 @@ __global__ __launch_bounds__(256) void multipole_kernel(MultipoleKernelArgs a) {
     }
     const float *brow = a.b + (a.per_shell_b ? (size_t)s * a.NAp : 0);
     double b9[9];
-    if (a.kind == 1)
+    if (kind == 1)
         for (int k = 0; k < 9; ++k) b9[k] = a.base[k];
     const size_t jend = min(a.NAp, (size_t)(sp + 1) * a.range);
     for (size_t j = (size_t)sp * a.range; j < jend; ++j) {
         const float bj = brow[j];
         if (bj == 0.f) continue;  // padding atoms and zero lengths add nothing
         double x = 0.0, y = 0.0, z = 0.0;
-        if (a.kind != 0 || !a.srf) {
+        if (kind != 0 || !a.srf) {
             x = a.X[j * a.NFp + t];
             y = a.Y[j * a.NFp + t];
             z = a.Z[j * a.NFp + t];
         }
-        if (a.kind == 0) {
+        if (kind == 0) {
             const double ql = a.qinfo[s];
             double rf, ct, somx2, sp, cp;
             if (a.srf) {
@@ -393,7 +398,7 @@ __global__ __launch_bounds__(256) void multipole_kernel(MultipoleKernelArgs a) {
             }
         }
     }
-    const double norm = a.kind == 1 ? sqrt(2.0 * kPi) : 1.0;
+    const double norm = kind == 1 ? sqrt(2.0 * kPi) : 1.0;
 #pragma unroll
     for (int k = 0; k < kMoments; ++k) {
         if (k >= nk) break;
@@ -436,12 +441,16 @@ hipError_t launch_multipole(const MultipoleArgs &m, hipStream_t stream) {
     }
     if ((size_t)k.zgroups * k.split > 65535) return hipErrorInvalidValue;
     dim3 grid((unsigned)(m.NFp / 64), (unsigned)m.ns, (unsigned)(k.zgroups * k.split));
-    hipLaunchKernelGGL(multipole_kernel, grid, dim3(256), 0, stream, k);
+    if (m.kind == 0)
+        hipLaunchKernelGGL(multipole_kernel<0>, grid, dim3(256), 0, stream, k);
+    else
+        hipLaunchKernelGGL(multipole_kernel<-1>, grid, dim3(256), 0, stream, k);
     return hipGetLastError();
 }
 
-hipError_t multipole_blocks_per_cu(int *blocks) {
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, multipole_kernel, 256, 0);
+hipError_t multipole_blocks_per_cu(int kind, int *blocks) {
+    return kind == 0 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, multipole_kernel<0>, 256, 0)
+                     : hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, multipole_kernel<-1>, 256, 0);
 }
 
 int multipole_base_blocks(size_t NFp, int ns, int nmo) {
