@@ -258,12 +258,14 @@ __device__ void to_cylinder(double x, double y, double z, double &r, double &phi
     zz = z;
 }
 
-// KIND 0: sphere only, bounded to 3 wavefronts per SIMD (its FP64
-// recurrences are latency-bound); KIND -1: the kind read from the arguments
-// (the cylinder path, whose registers the compiler chooses).
+// KIND 0 sphere, 1 cylinder, each instantiated on its own.  Both are bound by
+// the latency of serial FP64 recurrences, so they are held to 3 (sphere) and
+// 2 (cylinder) wavefronts per SIMD, the bounds that measured fastest despite
+// some spilled registers (1 wavefront per SIMD took 1.6x (sphere) and 1.8x
+// (cylinder) as long).
 template <int KIND>
-__global__ __launch_bounds__(256, KIND == 0 ? 3 : 1) void multipole_kernel(MultipoleKernelArgs a) {
-    const int kind = KIND < 0 ? a.kind : KIND;
+__global__ __launch_bounds__(256, KIND == 0 ? 3 : 2) void multipole_kernel(MultipoleKernelArgs a) {
+    constexpr int kind = KIND;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     // block = (frame tile, shell, group of 4 moment blocks + zgroups x atom range)
@@ -444,13 +446,13 @@ hipError_t launch_multipole(const MultipoleArgs &m, hipStream_t stream) {
     if (m.kind == 0)
         hipLaunchKernelGGL(multipole_kernel<0>, grid, dim3(256), 0, stream, k);
     else
-        hipLaunchKernelGGL(multipole_kernel<-1>, grid, dim3(256), 0, stream, k);
+        hipLaunchKernelGGL(multipole_kernel<1>, grid, dim3(256), 0, stream, k);
     return hipGetLastError();
 }
 
 hipError_t multipole_blocks_per_cu(int kind, int *blocks) {
     return kind == 0 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, multipole_kernel<0>, 256, 0)
-                     : hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, multipole_kernel<-1>, 256, 0);
+                     : hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, multipole_kernel<1>, 256, 0);
 }
 
 int multipole_base_blocks(size_t NFp, int ns, int nmo) {
