@@ -141,20 +141,27 @@ def bench_c3(args):
 
 
 def bench_mp(args):
-    """Multipole sphere averaging (MPSphereScatterDevice) on the headline
-    trajectory: 20,000 atoms x 2,000 frames, moments (0,0) and l = 1..10,
-    m = -l..l (121, the reference's list for resolution 10), 10 shells over
-    0.2-2.0 A^-1, FFT autocorrelation.  One term = one (atom, frame, moment,
-    shell)."""
+    """Multipole averaging on the headline trajectory (20,000 atoms x 2,000
+    frames), 10 shells over 0.2-2.0 A^-1, FFT autocorrelation.  Sphere
+    (MPSphereScatterDevice, workload mp): moments (0,0) and l = 1..10,
+    m = -l..l (121, the reference's list for resolution 10), q along z.
+    Cylinder (MPCylinderScatterDevice, workload mpc): (0,0) and l = 1..10,
+    m = 0..3 (41), axis z, q at 30 degrees from the axis.  One term = one
+    (atom, frame, moment, shell)."""
     from sassena_amd import ScatterDevice, synth
     T = synth.case("c2")
     lmax = 10
-    moments = [(0, 0)] + [(l, m) for l in range(1, lmax + 1) for m in range(-l, l + 1)]
-    qv = np.array([[0.0, 0.0, q] for q in np.linspace(0.2, 2.0, 10)])
+    kind = "cylinder" if args.workload == "mpc" else "sphere"
+    if kind == "sphere":
+        moments = [(0, 0)] + [(l, m) for l in range(1, lmax + 1) for m in range(-l, l + 1)]
+        qv = np.array([[0.0, 0.0, q] for q in np.linspace(0.2, 2.0, 10)])
+    else:
+        moments = [(0, 0)] + [(l, m) for l in range(1, lmax + 1) for m in range(4)]
+        qv = np.array([[0.5 * q, 0.0, np.sqrt(0.75) * q] for q in np.linspace(0.2, 2.0, 10)])
     dev = ScatterDevice("all", "autocorrelate", "fftw", profile=True)
     dev.stage(T.xyz, layout="soa")
     dev.set_factors(T.b)
-    step = lambda: dev.compute_multipole("sphere", qv, moments)
+    step = lambda: dev.compute_multipole(kind, qv, moments)
     for _ in range(args.warmup):
         step()
     amp_ms = dev_ms = 0.0
@@ -168,12 +175,12 @@ def bench_mp(args):
     dev.close()
     terms = T.NA * T.NF * len(moments) * len(qv)
     out = {
-        "metric": "atom*frame*moment*shell terms/s (multipole sphere average, FFT autocorrelation)",
+        "metric": f"atom*frame*moment*shell terms/s (multipole {kind} average, FFT autocorrelation)",
         "value": terms * args.steps / wall, "unit": "terms/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": wall / args.steps * 1e3, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f64 special functions and sums / f32 amplitudes",
         "data": "synthetic",
-        "config": {"workload": f"multipole sphere, {T.NA} atoms x {T.NF} frames, l <= {lmax} "
+        "config": {"workload": f"multipole {kind}, {T.NA} atoms x {T.NF} frames, l <= {lmax} "
                                f"({len(moments)} moments) x {len(qv)} shells",
                    "NA": T.NA, "NF": T.NF, "moments": len(moments), "shells": len(qv), "terms_per_step": terms},
         "kernel_ms_per_step": {"multipole": amp_ms / args.steps, "device_total": dev_ms / args.steps},
@@ -289,11 +296,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--vector-block", type=int, default=0)
     ap.add_argument("--atom-split", type=int, default=0)
-    ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c4", "c5", "mp"],
+    ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c4", "c5", "mp", "mpc"],
                     help="c2: the headline coherent case; c3: incoherent self with per-pair FFTs; "
                          "c4: 64 uneven trajectories in one batched graph; "
                          "c5: learned control variate against the 1,000-orientation average; "
-                         "mp: multipole sphere average (121 moments x 10 shells) on the c2 trajectory")
+                         "mp: multipole sphere average (121 moments x 10 shells) on the c2 trajectory; "
+                         "mpc: multipole cylinder average (41 moments x 10 shells) on the same trajectory")
     ap.add_argument("--c5-scale", type=float, default=1.0, help="shrink the c5 atoms and frames (tests)")
     ap.add_argument("--c4-scale", type=float, default=1.0, help="shrink the c4 trajectories (tests)")
     ap.add_argument("--c5-tol", type=float, default=1e-3,
@@ -329,7 +337,7 @@ def main():
         return bench_c4(args)
     if args.workload == "c5":
         return bench_c5(args)
-    if args.workload == "mp":
+    if args.workload in ("mp", "mpc"):
         return bench_mp(args)
 
     T = synth.case("c2")
