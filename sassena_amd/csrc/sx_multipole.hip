@@ -124,39 +124,55 @@ __device__ __forceinline__ void sph_bessel_moments(const int (&ls)[kMoments], in
         if (ls[k] >= 2) jv[k] = cap[k] * sc;
 }
 
-// J_n(x) for n >= 0, x >= 0: upward recurrence from J_0, J_1 where x > n,
-// else Miller's downward recurrence normalised by J_0 + 2 sum_k J_2k = 1.
-__device__ double cyl_bessel(int n, double x) {
-    if (n == 0) return j0(x);
-    if (n == 1) return j1(x);
-    if (x == 0.0) return 0.0;
-    if (x > (double)n) {
-        double a = j0(x), bb = j1(x);
-        for (int k = 1; k < n; ++k) {
-            const double nx = (2.0 * k) / x * bb - a;
+// J_n(x), x >= 0, for the orders ns[0..kMoments) of a wavefront in one
+// recurrence: orders 0 and 1 from J0, J1; the others upward from J0, J1 where
+// x exceeds the largest order nhi, else by Miller's downward recurrence from
+// above nhi, normalised by J_0 + 2 sum_k J_2k = 1 and captured on the way.
+__device__ __forceinline__ void cyl_bessel_orders(const int (&ns)[kMoments], int nhi, double x, double J0,
+                                                  double J1, double (&jv)[kMoments]) {
+#pragma unroll
+    for (int q = 0; q < kMoments; ++q) jv[q] = ns[q] == 0 ? J0 : (ns[q] == 1 ? J1 : 0.0);
+    if (nhi < 2 || x == 0.0) return;
+    const double ix = 1.0 / x;
+    if (x > (double)nhi) {
+        double a = J0, bb = J1;  // J_{k-1}, J_k
+        for (int k = 1; k < nhi; ++k) {
+            const double nx = (2.0 * k) * ix * bb - a;
             a = bb;
-            bb = nx;
+            bb = nx;  // J_{k+1}
+#pragma unroll
+            for (int q = 0; q < kMoments; ++q)
+                if (ns[q] == k + 1) jv[q] = bb;
         }
-        return bb;
+        return;
     }
-    int N = n + 32 + (int)(2.0 * sqrt((double)n + x));
+    int N = nhi + 32 + (int)(2.0 * sqrt((double)nhi + x));
     N += N & 1;  // even start keeps the normalisation sum on even orders
-    double fp1 = 0.0, f = 1e-280, fn = 0.0, norm = 0.0;
+    double fp1 = 0.0, f = 1e-280, norm = 0.0;
+    double cap[kMoments];
+#pragma unroll
+    for (int q = 0; q < kMoments; ++q) cap[q] = 0.0;
     for (int k = N; k > 0; --k) {
-        const double fm1 = (2.0 * k) / x * f - fp1;
+        const double fm1 = (2.0 * k) * ix * f - fp1;
         fp1 = f;
-        f = fm1;  // f = J_{k-1} (unnormalised)
-        if (k - 1 == n) fn = f;
+        f = fm1;  // J_{k-1} (unnormalised)
+#pragma unroll
+        for (int q = 0; q < kMoments; ++q)
+            if (ns[q] == k - 1) cap[q] = f;
         if (((k - 1) & 1) == 0 && k - 1 > 0) norm += 2.0 * f;
         if (fabs(f) > 1e250) {
             f *= 1e-250;
             fp1 *= 1e-250;
-            fn *= 1e-250;
             norm *= 1e-250;
+#pragma unroll
+            for (int q = 0; q < kMoments; ++q) cap[q] *= 1e-250;
         }
     }
     norm += f;  // J_0
-    return fn / norm;
+    const double inorm = 1.0 / norm;
+#pragma unroll
+    for (int q = 0; q < kMoments; ++q)
+        if (ns[q] >= 2) jv[q] = cap[q] * inorm;
 }
 
 // P_l^m(x) with the Condon-Shortley phase, m >= 0, somx2 = sqrt(1 - x^2) given
@@ -297,8 +313,19 @@ __global__ __launch_bounds__(256, KIND == 0 ? 3 : 2) void multipole_kernel(Multi
     }
     const float *brow = a.b + (a.per_shell_b ? (size_t)s * a.NAp : 0);
     double b9[9];
-    if (kind == 1)
+    // cylinder: Bessel order per moment (J_0 for (0,0), J_2l for m = 0, 1,
+    // J_2l-1 for m = 2, 3) and the largest
+    int cn[kMoments];
+    int nhi = 0;
+    if (kind == 1) {
         for (int k = 0; k < 9; ++k) b9[k] = a.base[k];
+#pragma unroll
+        for (int k = 0; k < kMoments; ++k) {
+            const int l = ls[k], m = ms[k];
+            cn[k] = (l == 0 || m > 3 || m < 0) ? 0 : (m <= 1 ? 2 * l : 2 * l - 1);
+            nhi = max(nhi, cn[k]);
+        }
+    }
     const size_t jend = min(a.NAp, (size_t)(sp + 1) * a.range);
     for (size_t j = (size_t)sp * a.range; j < jend; ++j) {
         const float bj = brow[j];
@@ -378,22 +405,27 @@ __global__ __launch_bounds__(256, KIND == 0 ? 3 : 2) void multipole_kernel(Multi
             double exs, exc;
             sincos(sgn * zf * qz, &exs, &exc);
             const double p = rf * qr, psi = phif - qphi;
+            double jn[kMoments];
+            cyl_bessel_orders(cn, nhi, p, j0(p), j1(p), jn);
+            double sn = 0.0, cs = 1.0;
 #pragma unroll
             for (int k = 0; k < kMoments; ++k) {
                 if (k >= nk) break;
                 const int l = ls[k], m = ms[k];
+                // one sincos per distinct order (m = 0, 1 and m = 2, 3 share theirs)
+                if (k == 0 || cn[k] != cn[k - 1]) sincos(cn[k] * psi, &sn, &cs);
                 double cr = 0.0, ci = 0.0;
                 const double sgl = (l & 1) ? -1.0 : 1.0;
                 if (m == 0 && l == 0) {
-                    cr = cyl_bessel(0, p);
+                    cr = jn[k];
                 } else if (m == 0) {
-                    cr = sqrt(0.5) * 2.0 * sgl * cyl_bessel(2 * l, p) * cos(2 * l * psi);
+                    cr = sqrt(0.5) * 2.0 * sgl * jn[k] * cs;
                 } else if (m == 1) {
-                    cr = sqrt(0.5) * 2.0 * sgl * cyl_bessel(2 * l, p) * sin(2 * l * psi);
+                    cr = sqrt(0.5) * 2.0 * sgl * jn[k] * sn;
                 } else if (m == 2) {
-                    ci = sqrt(0.5) * 2.0 * (-sgl) * cyl_bessel(2 * l - 1, p) * cos((2 * l - 1) * psi);
+                    ci = sqrt(0.5) * 2.0 * (-sgl) * jn[k] * cs;
                 } else if (m == 3) {
-                    ci = sqrt(0.5) * 2.0 * (-sgl) * cyl_bessel(2 * l - 1, p) * sin((2 * l - 1) * psi);
+                    ci = sqrt(0.5) * 2.0 * (-sgl) * jn[k] * sn;
                 }
                 ar[k] += (double)bj * (cr * exc - ci * exs);
                 ai[k] += (double)bj * (cr * exs + ci * exc);
