@@ -301,13 +301,12 @@ __global__ __launch_bounds__(256, KIND == 0 ? 3 : 2) void multipole_kernel(Multi
         ls[k] = a.moments[2 * kk];
         ms[k] = a.moments[2 * kk + 1];
     }
-    double ar[kMoments], ai[kMoments], ynorm[kMoments];
+    double ar[kMoments], ai[kMoments];
     int lhi = 0;
     bool sorted = true;  // degrees non-decreasing over the wavefront's moments
 #pragma unroll
     for (int k = 0; k < kMoments; ++k) {
         ar[k] = ai[k] = 0.0;
-        ynorm[k] = ylm_norm(ls[k], ms[k] < 0 ? -ms[k] : ms[k]);
         lhi = max(lhi, ls[k]);
         if (k > 0 && ls[k] < ls[k - 1]) sorted = false;
     }
@@ -368,7 +367,8 @@ __global__ __launch_bounds__(256, KIND == 0 ? 3 : 2) void multipole_kernel(Multi
                 if (k >= nk) break;
                 const int l = ls[k], m = ms[k], am = m < 0 ? -m : m;
                 const double jl = jv[k];
-                const double th = ynorm[k] * plm_raw(l, am, ct, somx2);
+                // the Y_lm normalisation and 4 pi scale the finished sums
+                const double th = plm_raw(l, am, ct, somx2);
                 // e^{i am phi} by repeated products of e^{i phi}
                 double er = 1.0, ei = 0.0;
                 for (int q = 0; q < am; ++q) {
@@ -383,7 +383,7 @@ __global__ __launch_bounds__(256, KIND == 0 ? 3 : 2) void multipole_kernel(Multi
                     yi = -sg * yi;
                 }
                 // 4 pi i^l b j_l conj(Y)
-                const double f = 4.0 * kPi * (double)bj * jl;
+                const double f = (double)bj * jl;
                 const double cr = f * yr, ci = -f * yi;
                 switch (l & 3) {
                     case 0: ar[k] += cr; ai[k] += ci; break;
@@ -432,10 +432,11 @@ __global__ __launch_bounds__(256, KIND == 0 ? 3 : 2) void multipole_kernel(Multi
             }
         }
     }
-    const double norm = kind == 1 ? sqrt(2.0 * kPi) : 1.0;
 #pragma unroll
     for (int k = 0; k < kMoments; ++k) {
         if (k >= nk) break;
+        const double norm =
+            kind == 1 ? sqrt(2.0 * kPi) : 4.0 * kPi * ylm_norm(ls[k], ms[k] < 0 ? -ms[k] : ms[k]);
         const size_t v = ((size_t)sp * a.ns + s) * a.nmo + k0 + k;
         a.amp[v * a.NFp + t] = make_float2((float)(norm * ar[k]), (float)(norm * ai[k]));
     }
