@@ -362,6 +362,11 @@ __global__ __launch_bounds__(256, KIND == 0 ? 3 : 2) void multipole_kernel(Multi
                 sph_bessel_moments<true>(ls, lhi, xq, ix, j0, j1, jv);
             else
                 sph_bessel_moments<false>(ls, lhi, xq, ix, j0, j1, jv);
+            // e^{i am phi} carried from moment to moment: one product by
+            // e^{+-i phi} when |m| steps by one (the reference's lists),
+            // repeated products from 1 otherwise
+            double er = 1.0, ei = 0.0;
+            int aprev = 0;
 #pragma unroll
             for (int k = 0; k < kMoments; ++k) {
                 if (k >= nk) break;
@@ -369,13 +374,24 @@ __global__ __launch_bounds__(256, KIND == 0 ? 3 : 2) void multipole_kernel(Multi
                 const double jl = jv[k];
                 // the Y_lm normalisation and 4 pi scale the finished sums
                 const double th = plm_raw(l, am, ct, somx2);
-                // e^{i am phi} by repeated products of e^{i phi}
-                double er = 1.0, ei = 0.0;
-                for (int q = 0; q < am; ++q) {
+                if (am == aprev + 1) {
                     const double nr = er * cp - ei * sp;
                     ei = er * sp + ei * cp;
                     er = nr;
+                } else if (am == aprev - 1) {
+                    const double nr = er * cp + ei * sp;
+                    ei = ei * cp - er * sp;
+                    er = nr;
+                } else if (am != aprev) {
+                    er = 1.0;
+                    ei = 0.0;
+                    for (int q = 0; q < am; ++q) {
+                        const double nr = er * cp - ei * sp;
+                        ei = er * sp + ei * cp;
+                        er = nr;
+                    }
                 }
+                aprev = am;
                 double yr = th * er, yi = th * ei;
                 if (m < 0) {
                     const double sg = (am & 1) ? -1.0 : 1.0;
