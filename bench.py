@@ -27,16 +27,53 @@ sys.path.insert(0, ROOT)
 # MI355X: 256 CUs, 2.4 GHz, 64 lanes/CU; v_sin_f32 / v_cos_f32 issue at a
 # quarter of the full VALU rate.
 TRANS_PEAK = 256 * 64 * 2.4e9 / 4.0   # transcendental ops/s
-# Published FP32 vector peak of MI355X (two wave64 FMA instructions per CU
-# per clock: 256 CUs x 128 lanes x 2 flops x 2.4 GHz).
+# Published vector peaks of MI355X.  The FP32 figure counts packed FP32 (two
+# FMAs per lane and instruction); this build keeps FP32 code off the packed
+# instructions, so its reachable FP32 FMA rate is half of it.
 FP32_PEAK = 157.3e12
-# Linearly spaced shells: per term two FMAs step the cosine and sine of the
-# phase to the next shell and two FMAs add b cos, b sin to the sums.
-FLOPS_PER_TERM = 8.0
+FP64_PEAK = 78.6e12
+# Algorithmic cost of one (atom, frame, q vector) term of the amplitude sum:
+# the phase q.r (3 mul + 2 add) and its reduction mod 2 pi (1 mul + 1
+# round/sub) in FP64, sin and cos, and b cos, b sin accumulated (2 FMA = 4
+# FP32 flops).  Kahan or other compensation is implementation overhead.
+ALG_FP64_FLOPS = 7.0
+ALG_FP32_FLOPS = 4.0
+ALG_TRANS = 2.0
+# Executed per term on linearly spaced shells (the stepped-phasor path): two
+# FMAs step cos and sin to the next shell and two add b cos, b sin to the
+# sums; shells of a reference q scan (residual-corrected steps) add two FMAs
+# of the correction sums.  The FP64 phase and four hardware sin/cos per
+# (orientation, atom, frame) are shared by a tile's shells.
+EXEC_FLOPS_STEPPED = 8.0
+EXEC_FLOPS_RESIDUAL = 12.0
 # HBM bytes per amplitude launch on this workload (tile 1 x 20, 4 atom
 # ranges), from rocprofv3 --pmc FETCH_SIZE (x2.0) and WRITE_SIZE (x1.0), the
 # factors a 1 GiB device copy reads for the same counters on MI355X.
 AMP_TRAFFIC_BYTES = 1.97e9 + 0.524e9
+
+
+# FP64 flops per (atom, frame, moment, shell) term of the multipole kernels
+# on the mp / mpc workloads, from rocprofv3 instruction counts (FMA counted
+# as 2 flops); None until measured.
+MP_FP64_PER_TERM = {"sphere": None, "cylinder": None}
+
+
+def roofline_legs(terms_per_launch, launch_s):
+    """The algorithmic legs of the amplitude kernel (per-term unit above)
+    against the published peaks; the binding leg is the largest fraction."""
+    rate = terms_per_launch / launch_s
+    legs = {
+        "fp64": {"achieved": ALG_FP64_FLOPS * rate / 1e12, "peak": FP64_PEAK / 1e12, "unit": "TFLOP/s",
+                 "frac": ALG_FP64_FLOPS * rate / FP64_PEAK, "per_term": ALG_FP64_FLOPS},
+        "fp32": {"achieved": ALG_FP32_FLOPS * rate / 1e12, "peak": FP32_PEAK / 1e12, "unit": "TFLOP/s",
+                 "frac": ALG_FP32_FLOPS * rate / FP32_PEAK, "per_term": ALG_FP32_FLOPS},
+        "transcendental": {"achieved": ALG_TRANS * rate / 1e12, "peak": TRANS_PEAK / 1e12, "unit": "T/s",
+                           "frac": ALG_TRANS * rate / TRANS_PEAK, "per_term": ALG_TRANS,
+                           "note": "peak assumes quarter-rate v_sin/v_cos; the stepped path issues about 0.2 "
+                                   "per term (sin/cos per orientation, atom and frame only), so this leg can "
+                                   "exceed 1"},
+    }
+    return legs
 
 
 def log(msg):
@@ -55,18 +92,43 @@ def cpu_model():
     return platform.processor()
 
 
+def host_threads():
+    """Threads of the CPU baseline: the CPUs this process may run on, capped
+    at 16, the host-CPU share of one GPU on the measurement hosts."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def host_info():
+    try:
+        aff = len(os.sched_getaffinity(0))
+    except AttributeError:
+        aff = None
+    return {"cpu_model": cpu_model(), "nproc": os.cpu_count(), "affinity_cpus": aff,
+            "threads_used": host_threads(),
+            "note": "the baseline runs on the host-CPU share of one GPU (16 threads at most)"}
+
+
 def cpu_baseline(T, threads):
+    """The FP64 oracle on the host: one shell (the last) x 8 vectors per
+    thread over all atoms and frames.  Returns the record and the oracle's
+    results (for the GPU's error on the same vectors)."""
     import oracle
     frames = np.ascontiguousarray(np.transpose(T.xyz, (2, 1, 0)))
-    k = 8 * threads
+    k = min(T.NM, 8 * threads)
     qv = T.qvectors(T.nshells - 1)[:k]
     t0 = time.perf_counter()
-    oracle.compute("all", frames, T.b, qv, "autocorrelate", "fftw", nthreads=threads)
+    ref = oracle.compute("all", frames, T.b, qv, "autocorrelate", "fftw", nthreads=threads)
     dt = time.perf_counter() - t0
     terms = k * T.NA * T.NF
-    return {"value": terms / dt, "unit": "terms/s", "cores": threads, "kind": "port",
-            "sample": f"1 shell x {k} vectors x {T.NA} atoms x {T.NF} frames ({terms:.2e} terms) in {dt:.1f} s, "
-                      f"FP64 libm sin/cos, {threads} threads on {cpu_model()}"}
+    rec = {"value": terms / dt, "unit": "terms/s", "cores": threads, "kind": "port",
+           "sample": f"1 shell x {k} vectors x {T.NA} atoms x {T.NF} frames ({terms:.2e} terms) in {dt:.1f} s, "
+                     f"FP64 libm sin/cos, {threads} threads on {cpu_model()}",
+           "host": host_info()}
+    return rec, ref, k
 
 
 def fft_flops(L):
@@ -125,7 +187,7 @@ def bench_c3(args):
     }
     if not args.no_cpu:
         import oracle
-        threads = min(16, os.cpu_count() or 1)
+        threads = host_threads()
         idx = np.flatnonzero(T.b)[:threads * 4]
         atoms = np.ascontiguousarray(np.transpose(T.xyz[:, idx, :], (1, 2, 0)))
         qv = T.qvectors(T.nshells - 1)[:8]
@@ -136,7 +198,7 @@ def bench_c3(args):
         out["cpu_baseline"] = {"value": n / dt, "unit": "terms/s", "cores": threads, "kind": "port",
                                "sample": f"1 shell x {len(qv)} vectors x {len(idx)} atoms x {T.NF} frames "
                                          f"({n:.2e} terms) in {dt:.1f} s, FP64 FFT of {L}, {threads} threads "
-                                         f"on {cpu_model()}"}
+                                         f"on {cpu_model()}", "host": host_info()}
     print(json.dumps(out), flush=True)
 
 
@@ -185,7 +247,33 @@ def bench_mp(args):
                    "NA": T.NA, "NF": T.NF, "moments": len(moments), "shells": len(qv), "terms_per_step": terms},
         "kernel_ms_per_step": {"multipole": amp_ms / args.steps, "device_total": dev_ms / args.steps},
     }
-    print(json.dumps(out))
+    per = MP_FP64_PER_TERM[kind]
+    if per is not None:
+        amp_s = amp_ms / args.steps * 1e-3
+        out["roofline"] = {"bound": "valu", "achieved": per * terms / amp_s / 1e12, "peak": FP64_PEAK / 1e12,
+                           "unit": "TFLOP/s", "frac": per * terms / amp_s / FP64_PEAK, "traffic": None,
+                           "kernel": "multipole_kernel", "fp64_flops_per_term": per,
+                           "note": "FP64 flops per (atom, frame, moment, shell) from rocprofv3 counts of "
+                                   "FP64 FMA (2 flops), MUL and ADD instructions over the multipole kernel, "
+                                   "against the published FP64 vector peak"}
+    if not args.no_cpu:
+        # the FP64 oracle on the host: one shell x one moment per thread
+        # (the highest degree of the list) over all atoms and frames
+        import oracle
+        from concurrent.futures import ThreadPoolExecutor
+        threads = host_threads()
+        frames = synth.to_frames_aos(T.xyz)
+        sample = moments[-threads:]
+        t1 = time.perf_counter()
+        with ThreadPoolExecutor(threads) as ex:
+            list(ex.map(lambda mo: oracle.compute_multipole(kind, frames, T.b, qv[-1], [mo]), sample))
+        dt = time.perf_counter() - t1
+        n = len(sample) * T.NA * T.NF
+        out["cpu_baseline"] = {"value": n / dt, "unit": "terms/s", "cores": threads, "kind": "port",
+                               "sample": f"1 shell x {len(sample)} moments (l = {lmax}) x {T.NA} atoms x {T.NF} "
+                                         f"frames ({n:.2e} terms) in {dt:.1f} s, FP64, one moment per thread, "
+                                         f"{threads} threads on {cpu_model()}", "host": host_info()}
+    print(json.dumps(out), flush=True)
 
 
 def bench_c4(args):
@@ -371,25 +459,26 @@ def main():
     amp_avg_s = amp_ms / amp_launches * 1e-3
     terms_per_launch = T.terms() * args.steps / amp_launches
     stepped = bool(tm.get("shell_progression", 0))
+    residual = bool(tm.get("shell_residuals", 0))
+    legs = roofline_legs(terms_per_launch, amp_avg_s)
+    # the binding algorithmic leg: FP64 (7 flops per term against 78.6 TFLOP/s)
+    roof = {"bound": "valu", "achieved": legs["fp64"]["achieved"], "peak": legs["fp64"]["peak"],
+            "unit": "TFLOP/s", "frac": legs["fp64"]["frac"],
+            "unit_of_work": "one (atom, frame, q vector) term: 7 FP64 flops (phase and its reduction), 2 "
+                            "transcendentals, 4 FP32 flops; the FP64 leg binds",
+            "legs": legs}
     if stepped:
-        # linearly spaced shells: per term 8 FP32 flops (the three-term
-        # recurrence on cos and sin, the two FMAs of the sums); sin/cos and the
-        # FP64 phase only per (orientation, atom, frame)
-        achieved = FLOPS_PER_TERM * terms_per_launch / amp_avg_s
-        roof = {"bound": "valu", "achieved": achieved / 1e12, "peak": FP32_PEAK / 1e12,
-                "unit": "TFLOP/s", "frac": achieved / FP32_PEAK,
-                "note": "FP32 VALU bound: 8 flops per term (2 FMAs step cos/sin to the next shell, 2 FMAs into "
-                        "the sums); the per-(orientation, atom, frame) FP64 phase and 4 hardware sin/cos are "
-                        f"amortised over the tile's shells and not counted; tile shape {tm.get('tile_shape')}. "
-                        "The 157.3 TFLOP/s peak counts packed FP32 (two FMAs per lane and instruction); this "
-                        "build keeps FP32 code off the packed instructions, so its reachable FMA rate is "
-                        "78.6 TFLOP/s",
-                "frac_of_unpacked_fp32": achieved / (FP32_PEAK / 2)}
-    else:
-        achieved = 2.0 * terms_per_launch / amp_avg_s      # sin + cos per term
-        roof = {"bound": "valu", "achieved": achieved / 1e12, "peak": TRANS_PEAK / 1e12,
-                "unit": "Tsincos-ops/s", "frac": achieved / TRANS_PEAK,
-                "note": "transcendental-issue bound: 2 hardware sin/cos per term at quarter rate"}
+        ex = EXEC_FLOPS_RESIDUAL if residual else EXEC_FLOPS_STEPPED
+        roof["executed"] = {
+            "algorithm": "stepped phasors across linearly spaced shells" +
+                         (" with first-order corrections of the scan's single-precision residuals" if residual
+                          else ""),
+            "flops_per_term": ex, "achieved": ex * terms_per_launch / amp_avg_s / 1e12, "unit": "TFLOP/s",
+            "frac_of_unpacked_fp32": ex * terms_per_launch / amp_avg_s / (FP32_PEAK / 2),
+            "tile_shape": tm.get("tile_shape"),
+            "note": "FP32 FMAs executed per term; the FP64 phase and 4 hardware sin/cos per (orientation, "
+                    "atom, frame) are shared by the tile's shells and not counted; packed FP32 is not used, so "
+                    "the reachable rate is half the published 157.3 TFLOP/s"}
     out = {
         "metric": "atom*frame*q-vector terms/s (coherent F(q,t), FFT autocorrelation)",
         "value": value,
@@ -405,20 +494,33 @@ def main():
         "data": "synthetic",
         "config": {"workload": "coherent all, 20000 atoms x 2000 frames, 40 shells x 200 vectors, pad 4096",
                    "entry": "sx_compute_shells" if args.generic else "sx_compute_shells_oriented",
-                   "NA": T.NA, "NF": T.NF, "shells": T.nshells, "vectors": T.NM, "terms_per_step": T.terms()},
+                   "shells": "q scan 0.1-4.0 A^-1, 40 points, interior points rounded by single-precision "
+                             "powf as the reference's scan builder does",
+                   "NA": T.NA, "NF": T.NF, "nshells": T.nshells, "vectors": T.NM, "terms_per_step": T.terms(),
+                   "stepped": stepped, "residual_corrected": residual},
         "kernel_ms_per_step": {"amplitude": amp_ms / args.steps, "correlate": corr_ms / args.steps,
                                "finalize": fin_ms / args.steps, "device_total": dev_ms / args.steps},
         "staging_s": stage_s,
+        "staging_GBps": 12.0 * T.NA * T.NF / stage_s / 1e9,
         "pcie_inclusive_value": T.terms() / (wall / args.steps + stage_s),
         "roofline": dict(roof, traffic=AMP_TRAFFIC_BYTES if not (args.generic or args.vector_block or
                                                                  args.atom_split) else None,
                          traffic_unit="bytes/launch (rocprofv3 FETCH_SIZE+WRITE_SIZE, copy-calibrated)",
                          kernel="amp_all_kernel" if args.generic else "amp_oriented_kernel"),
     }
-    dev.close()
     if not args.no_cpu:
-        threads = min(16, os.cpu_count() or 1)
-        out["cpu_baseline"] = cpu_baseline(T, threads)
+        threads = host_threads()
+        rec, ref, k = cpu_baseline(T, threads)
+        out["cpu_baseline"] = rec
+        # the GPU on the same vectors of the same shell (all 40 shells with the
+        # baseline's orientations, so the same stepped path): error against the FP64 oracle
+        gfqt, gfq, _ = dev.compute_oriented(T.qlen, T.orient[:k])
+        err = float(np.max(np.abs(gfqt[-1] - ref[0])) / abs(ref[0][0]))
+        out["error_vs_fp64"] = {"shell_error": err, "fq_error": float(abs(gfq[-1] - ref[1]) / abs(ref[0][0])),
+                                "definition": "max_tau |F - F_oracle| / |F_oracle(0)| over the cpu_baseline's "
+                                              f"{k} vectors of the last shell",
+                                "stepped": bool(dev.timing().get("shell_progression", 0))}
+    dev.close()
     print(json.dumps(out), flush=True)
 
 

@@ -89,6 +89,9 @@ typedef struct sx_timing {
     int tile_shape;        /* oriented shells: tile shape of the last amplitude launch (1..5), else 0 */
     int shell_progression; /* oriented shells: 1 if the last launch stepped phasors across shells in
                               arithmetic progression, else 0 */
+    int shell_residuals;   /* oriented shells: 1 if that progression held only up to small residuals
+                              (a reference q scan rounded by single-precision powf,
+                              parameters.cpp:1151), corrected to first order per shell, else 0 */
 } sx_timing;
 
 const char *sx_version(void);

@@ -64,6 +64,9 @@ def lib():
         L.sxo_ylm.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double, _dp, _dp]
         L.sxo_cyl_bessel_j.argtypes = [ctypes.c_int, ctypes.c_double]
         L.sxo_cyl_bessel_j.restype = ctypes.c_double
+        L.sxo_scatter_mpsphere.argtypes = [_fp, sz, sz, _dp, ctypes.c_double, ctypes.c_int, ctypes.c_int, _dp]
+        L.sxo_scatter_mpcylinder.argtypes = [_fp, sz, sz, _dp, _dp, _dp, ctypes.c_int, ctypes.c_int, _dp]
+        L.sxo_cylinder_base.argtypes = [_dp, _dp]
         L.sxo_compute_multipole.argtypes = [ctypes.c_int, _fp, sz, sz, _dp, _dp, _dp, ip, sz, ctypes.c_int,
                                             ctypes.c_int, _dp, _dp, _dp]
         _lib = L
@@ -208,6 +211,31 @@ def sphere_moments(L):
 def cylinder_moments(L):
     """parameters.cpp:1068-1075: (0,0), then l = 1..L with m = 0..3."""
     return [(0, 0)] + [(l, m) for l in range(1, L + 1) for m in range(4)]
+
+
+def scatter_multipole(kind, coords_frames, b, q, l, m, axis=(0.0, 0.0, 1.0)):
+    """A_lm(t) of one moment (multipole_scatter_device.cpp:428-498 sphere,
+    900-987 cylinder); coords frames-major [NF][NA][3]; q: the q vector
+    (the sphere uses its length only)."""
+    c = np.ascontiguousarray(coords_frames, dtype=np.float32)
+    NF, NA = c.shape[0], c.shape[1]
+    bb = np.ascontiguousarray(b, dtype=np.float64)
+    qq = np.ascontiguousarray(q, dtype=np.float64)
+    out = np.zeros(2 * NF)
+    if kind == "sphere":
+        lib().sxo_scatter_mpsphere(_f(c), NA, NF, _d(bb), float(np.linalg.norm(qq)), int(l), int(m), _d(out))
+    else:
+        ax = np.ascontiguousarray(axis, dtype=np.float64)
+        lib().sxo_scatter_mpcylinder(_f(c), NA, NF, _d(bb), _d(qq), _d(ax), int(l), int(m), _d(out))
+    return _interleaved_to_cplx(out)
+
+
+def cylinder_base(axis):
+    """CartesianVectorBase(axis) (coor3d.cpp:278-296): rows e_r, e_phi, e_z."""
+    ax = np.ascontiguousarray(axis, dtype=np.float64)
+    out = np.zeros(9)
+    lib().sxo_cylinder_base(_d(ax), _d(out))
+    return out.reshape(3, 3)
 
 
 def compute_multipole(kind, coords_frames, b, q, moments, axis=(0.0, 0.0, 1.0), dsp="autocorrelate",

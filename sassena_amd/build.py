@@ -71,10 +71,7 @@ def build_host(verbose: bool = False) -> str:
     return SX_RUN
 
 
-def build(verbose: bool = False) -> str:
-    hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
-    tmp = LIB + ".tmp"
-    cmd = [hipcc, *HIPCC_FLAGS, "-I", os.path.join(ROOT, "include"), "-o", tmp, *sources()]
+def _hipcc(cmd, verbose):
     if verbose:
         print(" ".join(cmd), flush=True)
     res = subprocess.run(cmd, capture_output=True, text=True)
@@ -84,6 +81,25 @@ def build(verbose: bool = False) -> str:
         raise RuntimeError(f"hipcc failed ({res.returncode}):\n{errs}")
     if verbose and errs.strip():
         print(errs, file=sys.stderr)
+
+
+def build(verbose: bool = False) -> str:
+    """Every .hip source compiled to an object in parallel (each holds its own
+    kernels and launchers; device code does not cross files), then linked."""
+    from concurrent.futures import ThreadPoolExecutor
+    hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+    objdir = os.path.join(PKG, "_build")
+    os.makedirs(objdir, exist_ok=True)
+    compile_flags = [f for f in HIPCC_FLAGS if f != "-shared"]
+    jobs = []
+    for src in sources():
+        obj = os.path.join(objdir, os.path.splitext(os.path.basename(src))[0] + ".o")
+        jobs.append((obj, [hipcc, *compile_flags, "-c", "-I", os.path.join(ROOT, "include"), "-o", obj, src]))
+    workers = max(1, min(len(jobs), os.cpu_count() or 1, 8))
+    with ThreadPoolExecutor(workers) as ex:
+        list(ex.map(lambda j: _hipcc(j[1], verbose), jobs))
+    tmp = LIB + ".tmp"
+    _hipcc([hipcc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *[o for o, _ in jobs]], verbose)
     os.replace(tmp, LIB)
     build_host(verbose)
     return LIB

@@ -79,6 +79,29 @@ hipError_t launch_stage_atoms(const float *src, size_t a0, size_t na, size_t NF,
     return hipGetLastError();
 }
 
+// Largest |x| over n floats, as the bit pattern of a non-negative float
+// (ordered like unsigned integers) in *out, which the caller zeroes first.
+__global__ __launch_bounds__(256) void absmax_kernel(const float *__restrict__ x, size_t n,
+                                                     unsigned *__restrict__ out) {
+    float m = 0.f;
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256)
+        m = fmaxf(m, fabsf(x[i]));
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off, 64));
+    if ((threadIdx.x & 63) == 0) atomicMax(out, __float_as_uint(m));
+}
+
+__global__ void zero_u32_kernel(unsigned *p) { *p = 0u; }
+
+hipError_t launch_absmax(const float *x, size_t n, unsigned *out, hipStream_t stream) {
+    hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(1), 0, stream, out);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    const size_t blocks = std::min<size_t>(4096, std::max<size_t>(1, (n + 255) / 256));
+    hipLaunchKernelGGL(absmax_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, x, n, out);
+    return hipGetLastError();
+}
+
 // -------------------------------------------------------------- amplitude --
 
 struct AmpKernelArgs {
@@ -195,6 +218,25 @@ __global__ __launch_bounds__(256, 2) void amp_oriented_kernel(OrientKernelArgs a
     oriented_tile<VM, VS, PER_ROW_B>(a, ft, sp, og, sg);
 }
 
+// Shells that are an arithmetic progression up to small residuals (mode 2 of
+// oriented_tile_body); one factor per atom.  The FP64 totals take the
+// dynamic LDS (kResidualLds<VM, VS> bytes).
+template <int VM, int VS>
+__global__ __launch_bounds__(256, 2) void amp_oriented_res_kernel(OrientKernelArgs a) {
+    extern __shared__ double tot_lds[];
+    const int nitems = a.ftiles * a.split * a.ogroups * a.sgroups;
+    const int per = (nitems + 7) >> 3;
+    const int item = (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
+    if (item >= nitems) return;
+    const int sg = item % a.sgroups;
+    int rest = item / a.sgroups;
+    const int og = rest % a.ogroups;
+    rest /= a.ogroups;
+    const int sp = rest % a.split;
+    const int ft = rest / a.split;
+    oriented_tile_body<VM, VS, false, 2, true>(a, ft, sp, og, sg, tot_lds);
+}
+
 // Oriented tile shapes (orientations x shells per wavefront), picked by the
 // host autotuner: 0: 2x8, 1: 4x4, 2: 1x8, 3: 1x16, 4: 1x20.
 static const int kShapeVM[kOrientShapes] = {2, 4, 1, 1, 1};
@@ -204,11 +246,19 @@ int orient_shape_vm(int shape) { return kShapeVM[shape]; }
 int orient_shape_vs(int shape) { return kShapeVS[shape]; }
 
 template <int VM, int VS>
-static void launch_oriented_shape(const OrientKernelArgs &k, bool per_row, int blocks, hipStream_t stream) {
-    if (per_row)
+static hipError_t launch_oriented_shape(const OrientKernelArgs &k, bool per_row, int blocks, hipStream_t stream) {
+    if (k.ap == 2) {
+        if (per_row) return hipErrorInvalidValue;
+        const void *fn = (const void *)amp_oriented_res_kernel<VM, VS>;
+        const size_t lds = kResidualLds<VM, VS>;
+        hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL((amp_oriented_res_kernel<VM, VS>), dim3(blocks), dim3(256), lds, stream, k);
+    } else if (per_row)
         hipLaunchKernelGGL((amp_oriented_kernel<VM, VS, true>), dim3(blocks), dim3(256), 0, stream, k);
     else
         hipLaunchKernelGGL((amp_oriented_kernel<VM, VS, false>), dim3(blocks), dim3(256), 0, stream, k);
+    return hipGetLastError();
 }
 
 OrientKernelArgs make_orient_args(const OrientArgs &a) {
@@ -238,19 +288,34 @@ OrientKernelArgs make_orient_args(const OrientArgs &a) {
     k.amp = a.amp;
     k.ap = a.ap;
     k.qstep = a.qstep;
+    k.q0 = a.q0;
+    k.qres = a.qres;
     return k;
 }
 
-bool shells_in_progression(const double *qlen, int ns, double *step) {
-    *step = 0.0;
-    if (ns < 2) return false;
+int shell_progression(const double *qlen, int ns, double rmax, ShellGrid *g) {
+    *g = ShellGrid{};
+    if (ns < 2) return 0;
     const double d = (qlen[ns - 1] - qlen[0]) / (double)(ns - 1);
-    double qmax = 0.0;
+    double qmax = 0.0, emax = 0.0;
     for (int s = 0; s < ns; ++s) qmax = std::max(qmax, std::fabs(qlen[s]));
-    for (int s = 0; s < ns; ++s)
-        if (std::fabs(qlen[s] - (qlen[0] + (double)s * d)) > 1e-12 * qmax) return false;
-    *step = d;
-    return true;
+    g->res.resize(ns);
+    for (int s = 0; s < ns; ++s) {
+        const double e = qlen[s] - (qlen[0] + (double)s * d);
+        g->res[s] = e;
+        emax = std::max(emax, std::fabs(e));
+    }
+    g->q0 = qlen[0];
+    g->step = d;
+    if (emax <= 1e-12 * qmax) {
+        g->mode = 1;
+    } else if (emax <= kMaxShellResidual * qmax && emax * rmax <= kMaxResidualPhase / kTwoPiTurns) {
+        g->mode = 2;
+    } else {
+        g->res.clear();
+        g->step = 0.0;
+    }
+    return g->mode;
 }
 
 hipError_t launch_amp_oriented(const OrientArgs &a, hipStream_t stream) {
@@ -260,30 +325,36 @@ hipError_t launch_amp_oriented(const OrientArgs &a, hipStream_t stream) {
     const int blocks = ((nitems + 7) / 8) * 8;
     const bool per_row = a.per_shell_b != 0;
     switch (a.shape) {
-        case 0: launch_oriented_shape<2, 8>(k, per_row, blocks, stream); break;
-        case 1: launch_oriented_shape<4, 4>(k, per_row, blocks, stream); break;
-        case 2: launch_oriented_shape<1, 8>(k, per_row, blocks, stream); break;
-        case 3: launch_oriented_shape<1, 16>(k, per_row, blocks, stream); break;
-        default: launch_oriented_shape<1, 20>(k, per_row, blocks, stream); break;
+        case 0: return launch_oriented_shape<2, 8>(k, per_row, blocks, stream);
+        case 1: return launch_oriented_shape<4, 4>(k, per_row, blocks, stream);
+        case 2: return launch_oriented_shape<1, 8>(k, per_row, blocks, stream);
+        case 3: return launch_oriented_shape<1, 16>(k, per_row, blocks, stream);
+        default: return launch_oriented_shape<1, 20>(k, per_row, blocks, stream);
     }
-    return hipGetLastError();
 }
 
 template <int VM, int VS>
-static hipError_t oriented_occupancy(bool per_row, int *blocks) {
+static hipError_t oriented_occupancy(bool per_row, int mode, int *blocks) {
+    if (mode == 2) {
+        const void *k = (const void *)amp_oriented_res_kernel<VM, VS>;
+        const size_t lds = kResidualLds<VM, VS>;
+        hipError_t e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+        return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, k, 256, lds);
+    }
     const void *k = per_row ? (const void *)amp_oriented_kernel<VM, VS, true>
                             : (const void *)amp_oriented_kernel<VM, VS, false>;
     return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, k, 256, 0);
 }
 
-hipError_t oriented_blocks_per_cu(int shape, bool per_row, int *blocks) {
+hipError_t oriented_blocks_per_cu(int shape, bool per_row, int mode, int *blocks) {
     *blocks = 0;
     switch (shape) {
-        case 0: return oriented_occupancy<2, 8>(per_row, blocks);
-        case 1: return oriented_occupancy<4, 4>(per_row, blocks);
-        case 2: return oriented_occupancy<1, 8>(per_row, blocks);
-        case 3: return oriented_occupancy<1, 16>(per_row, blocks);
-        case 4: return oriented_occupancy<1, 20>(per_row, blocks);
+        case 0: return oriented_occupancy<2, 8>(per_row, mode, blocks);
+        case 1: return oriented_occupancy<4, 4>(per_row, mode, blocks);
+        case 2: return oriented_occupancy<1, 8>(per_row, mode, blocks);
+        case 3: return oriented_occupancy<1, 16>(per_row, mode, blocks);
+        case 4: return oriented_occupancy<1, 20>(per_row, mode, blocks);
         default: return hipErrorInvalidValue;
     }
 }

@@ -16,17 +16,35 @@ struct OrientKernelArgs {
     size_t natoms;         // atoms summed (<= NAp)
     int NM, ns, V, ftiles, ogroups, sgroups, split, atoms_per_split, per_shell_b;
     float2 *amp;           // [split][V][NFp], v = s * NM + m
-    int ap;                // shells in arithmetic progression with step qstep (|q| / 2 pi)
+    // Shells stepped from phasor to phasor: 0 no (one phase per shell), 1 an
+    // exact arithmetic progression (step qstep in |q| / 2 pi), 2 a progression
+    // up to small residuals: shell s lies at q0 + s qstep plus qres[s] (radians
+    // per Angstrom), corrected to first order in qres[s] d.
+    int ap;
     double qstep;
+    double q0;             // |q_0| / 2 pi of the ideal grid (ap == 2)
+    const float *qres;     // [ns] residuals of the shells against the ideal grid (ap == 2)
 };
+
+// Workgroup LDS of the residual-corrected tile: the FP64 totals of the 4
+// wavefronts, [wave][VM * VS][2][64] doubles.
+template <int VM, int VS>
+constexpr size_t kResidualLds = sizeof(double) * 4 * VM * VS * 2 * kWave;
 
 OrientKernelArgs make_orient_args(const OrientArgs &a);
 
 // One tile of the oriented amplitude sum: frame tile ft (one frame per
 // lane), atom range sp, orientations 4*VM*og.. (VM per wavefront) and
-// shells VS*sg..; 256 threads.  AP: the shells of the call are in arithmetic
-// progression (step a.qstep), and the tile's shells follow from its middle one
-// by the three-term recurrence.
+// shells VS*sg..; 256 threads.  MODE 1: the shells of the call are in
+// arithmetic progression (step a.qstep), and the tile's shells follow from its
+// middle one by the three-term recurrence.  MODE 2: as 1 on the ideal grid
+// q0 + s qstep, with each shell's residual r_s = q_s - (q0 + s qstep) (the
+// single-precision rounding of a reference q scan, parameters.cpp:1151)
+// applied to first order: b e^{i q_s d} = b e^{i q~_s d} (1 + i r_s d), the
+// neglected term (r_s d)^2 / 2 being bounded by the host.  The correction
+// sums g_s = sum_j b_j d_j e^{i q~_s d_j} run beside the amplitude sums and
+// enter them, scaled by r_s, at every fold; the FP64 totals then live in LDS
+// (`tot`, kResidualLds bytes) to leave the registers to g_s.
 //
 // Atom sums run in FP32 over sub-chunks of CH atoms (coordinates loaded
 // together) and are folded into FP64 totals every kFold atoms.
@@ -34,8 +52,12 @@ OrientKernelArgs make_orient_args(const OrientArgs &a);
 // float data format, no index stride.
 constexpr int kBufferDword3 = 0x00020000;
 
-template <int VM, int VS, bool PER_ROW_B, bool AP, bool BUF>
-__device__ __forceinline__ void oriented_tile_body(const OrientKernelArgs &a, int ft, int sp, int og, int sg) {
+template <int VM, int VS, bool PER_ROW_B, int MODE, bool BUF>
+__device__ __forceinline__ void oriented_tile_body(const OrientKernelArgs &a, int ft, int sp, int og, int sg,
+                                                   double *__restrict__ tot) {
+    constexpr bool AP = MODE >= 1;
+    constexpr bool RES = MODE == 2;
+    static_assert(!(RES && PER_ROW_B), "residual-corrected tiles take one factor per atom");
     constexpr int CH = 16;     // atoms per coordinate batch
     constexpr int kFold = 64;  // atoms per FP32 partial sum
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -52,7 +74,13 @@ __device__ __forceinline__ void oriented_tile_body(const OrientKernelArgs &a, in
         uy[i] = a.u[3 * m + 1];
         uz[i] = a.u[3 * m + 2];
     }
-    if constexpr (AP) {
+    float qr[RES ? VS : 1];
+    if constexpr (RES) {
+        // the tile's middle shell on the ideal grid, and the residuals
+        ql[0] = a.q0 + (double)(sbase + VS / 2) * a.qstep;
+#pragma unroll
+        for (int k = 0; k < VS; ++k) qr[k] = a.qres[min(sbase + k, a.ns - 1)];
+    } else if constexpr (AP) {
         // the tile's middle shell, which need not exist in a partial last tile
         ql[0] = a.qlen[min(sbase, a.ns - 1)] + (double)(VS / 2) * a.qstep;
     } else {
@@ -60,11 +88,20 @@ __device__ __forceinline__ void oriented_tile_body(const OrientKernelArgs &a, in
         for (int k = 0; k < VS; ++k) ql[k] = a.qlen[min(sbase + k, a.ns - 1)];
     }
 
-    double tr[VM][VS], ti[VM][VS];
+    // FP64 totals: registers, or this wavefront's slice of LDS ([VM*VS][2][64])
+    double tr[RES ? 1 : VM][RES ? 1 : VS], ti[RES ? 1 : VM][RES ? 1 : VS];
+    double *wt = RES ? tot + (size_t)wave * (VM * VS * 2 * kWave) + lane : nullptr;
 #pragma unroll
     for (int i = 0; i < VM; ++i)
 #pragma unroll
-        for (int k = 0; k < VS; ++k) tr[i][k] = ti[i][k] = 0.0;
+        for (int k = 0; k < VS; ++k) {
+            if constexpr (RES) {
+                wt[((i * VS + k) * 2) * kWave] = 0.0;
+                wt[((i * VS + k) * 2 + 1) * kWave] = 0.0;
+            } else {
+                tr[i][k] = ti[i][k] = 0.0;
+            }
+        }
 
     const size_t j0 = (size_t)sp * a.atoms_per_split;
     const size_t j1 = min(a.natoms, j0 + (size_t)a.atoms_per_split);
@@ -82,10 +119,14 @@ __device__ __forceinline__ void oriented_tile_body(const OrientKernelArgs &a, in
     for (size_t jf = j0; jf < j1; jf += kFold) {
         const size_t jend = min(j1, jf + (size_t)kFold);
         float cr[VM][VS], ci[VM][VS];
+        float gr[RES ? VM : 1][RES ? VS : 1], gi[RES ? VM : 1][RES ? VS : 1];
 #pragma unroll
         for (int i = 0; i < VM; ++i)
 #pragma unroll
-            for (int k = 0; k < VS; ++k) cr[i][k] = ci[i][k] = 0.f;
+            for (int k = 0; k < VS; ++k) {
+                cr[i][k] = ci[i][k] = 0.f;
+                if constexpr (RES) gr[i][k] = gi[i][k] = 0.f;
+            }
 #pragma unroll 1
         for (size_t j = jf; j < jend; j += CH) {
             float xs[CH], ys[CH], zs[CH];
@@ -152,8 +193,14 @@ __device__ __forceinline__ void oriented_tile_body(const OrientKernelArgs &a, in
                         const float sw = ((sh + sh) * ch) * rn;
                         const float c2 = cw + cw;
                         const float xc = __builtin_amdgcn_cosf(fc), yc = __builtin_amdgcn_sinf(fc);
+                        // b_j d_j of the correction sums (one factor per atom)
+                        const float bd = RES ? bk[0] * (float)d : 0.f;
                         cr[i][C] = fmaf(bk[C], xc, cr[i][C]);
                         ci[i][C] = fmaf(bk[C], yc, ci[i][C]);
+                        if constexpr (RES) {
+                            gr[i][C] = fmaf(bd, yc, gr[i][C]);
+                            gi[i][C] = fmaf(bd, xc, gi[i][C]);
+                        }
                         {  // shells above the middle
                             float xp = xc, yp = yc;
                             float xk = fmaf(xc, cw, -yc * sw), yk = fmaf(yc, cw, xc * sw);
@@ -161,6 +208,10 @@ __device__ __forceinline__ void oriented_tile_body(const OrientKernelArgs &a, in
                             for (int k = C + 1; k < VS; ++k) {
                                 cr[i][k] = fmaf(bk[k], xk, cr[i][k]);
                                 ci[i][k] = fmaf(bk[k], yk, ci[i][k]);
+                                if constexpr (RES) {
+                                    gr[i][k] = fmaf(bd, yk, gr[i][k]);
+                                    gi[i][k] = fmaf(bd, xk, gi[i][k]);
+                                }
                                 if (k + 1 < VS) {
                                     const float xn = fmaf(c2, xk, -xp), yn = fmaf(c2, yk, -yp);
                                     xp = xk;
@@ -177,6 +228,10 @@ __device__ __forceinline__ void oriented_tile_body(const OrientKernelArgs &a, in
                             for (int k = C - 1; k >= 0; --k) {
                                 cr[i][k] = fmaf(bk[k], xk, cr[i][k]);
                                 ci[i][k] = fmaf(bk[k], yk, ci[i][k]);
+                                if constexpr (RES) {
+                                    gr[i][k] = fmaf(bd, yk, gr[i][k]);
+                                    gi[i][k] = fmaf(bd, xk, gi[i][k]);
+                                }
                                 if (k > 0) {
                                     const float xn = fmaf(c2, xk, -xp), yn = fmaf(c2, yk, -yp);
                                     xp = xk;
@@ -201,8 +256,14 @@ __device__ __forceinline__ void oriented_tile_body(const OrientKernelArgs &a, in
         for (int i = 0; i < VM; ++i)
 #pragma unroll
             for (int k = 0; k < VS; ++k) {
-                tr[i][k] += (double)cr[i][k];
-                ti[i][k] += (double)ci[i][k];
+                if constexpr (RES) {
+                    // sum b e^{i q~ d} (1 + i r d) = (cr - r gr) + i (ci + r gi)
+                    wt[((i * VS + k) * 2) * kWave] += (double)fmaf(-qr[k], gr[i][k], cr[i][k]);
+                    wt[((i * VS + k) * 2 + 1) * kWave] += (double)fmaf(qr[k], gi[i][k], ci[i][k]);
+                } else {
+                    tr[i][k] += (double)cr[i][k];
+                    ti[i][k] += (double)ci[i][k];
+                }
             }
     }
 
@@ -213,17 +274,27 @@ __device__ __forceinline__ void oriented_tile_body(const OrientKernelArgs &a, in
             const int m = mbase + i, sh = sbase + k;
             if (m < a.NM && sh < a.ns) {
                 const size_t v = (size_t)sh * a.NM + m;
-                a.amp[((size_t)sp * a.V + v) * a.NFp + t] = make_float2((float)tr[i][k], (float)ti[i][k]);
+                double xr, xi;
+                if constexpr (RES) {
+                    xr = wt[((i * VS + k) * 2) * kWave];
+                    xi = wt[((i * VS + k) * 2 + 1) * kWave];
+                } else {
+                    xr = tr[i][k];
+                    xi = ti[i][k];
+                }
+                a.amp[((size_t)sp * a.V + v) * a.NFp + t] = make_float2((float)xr, (float)xi);
             }
         }
 }
 
+// Modes 0 and 1 (the mode is uniform over a launch or a trajectory); mode 2
+// has kernels of its own, since it needs `tot` in LDS.
 template <int VM, int VS, bool PER_ROW_B, bool BUF = true>
 __device__ __forceinline__ void oriented_tile(const OrientKernelArgs &a, int ft, int sp, int og, int sg) {
     if (a.ap)
-        oriented_tile_body<VM, VS, PER_ROW_B, true, BUF>(a, ft, sp, og, sg);
+        oriented_tile_body<VM, VS, PER_ROW_B, 1, BUF>(a, ft, sp, og, sg, nullptr);
     else
-        oriented_tile_body<VM, VS, PER_ROW_B, false, BUF>(a, ft, sp, og, sg);
+        oriented_tile_body<VM, VS, PER_ROW_B, 0, BUF>(a, ft, sp, og, sg, nullptr);
 }
 
 }  // namespace sx

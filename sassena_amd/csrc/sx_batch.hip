@@ -24,11 +24,16 @@
 
 namespace sx {
 
+// Trajectories whose shells step with residual corrections (mode 2) keep
+// their FP64 totals in the dynamic LDS, which the host sizes to
+// kResidualLds<VM, VS> when the batch holds any such trajectory; each
+// wavefront touches only its own slice, so tiles need no barrier between them.
 template <int VM, int VS>
 __global__ __launch_bounds__(256, 2) void amp_persistent_kernel(const OrientKernelArgs *__restrict__ targs,
                                                              const int4 *__restrict__ items,
                                                              const int *__restrict__ qstart,
                                                              unsigned *__restrict__ counters) {
+    extern __shared__ double tot_lds[];
     __shared__ int cur;
     const int x = blockIdx.x & 7;
     const int begin = qstart[x], n = qstart[x + 1] - begin;
@@ -39,7 +44,11 @@ __global__ __launch_bounds__(256, 2) void amp_persistent_kernel(const OrientKern
         __syncthreads();  // every thread has read cur before it is overwritten
         if (it >= n) break;
         const int4 w = items[begin + it];
-        oriented_tile<VM, VS, false, false>(targs[w.x], w.y, w.w, w.z & 0xffff, w.z >> 16);
+        const OrientKernelArgs &ta = targs[w.x];
+        if (ta.ap == 2)
+            oriented_tile_body<VM, VS, false, 2, false>(ta, w.y, w.w, w.z & 0xffff, w.z >> 16, tot_lds);
+        else
+            oriented_tile<VM, VS, false, false>(ta, w.y, w.w, w.z & 0xffff, w.z >> 16);
     }
 }
 
@@ -65,6 +74,16 @@ __global__ __launch_bounds__(256) void final_batch_kernel(const FinalArgs *__res
 
 using PersistentFn = void (*)(const OrientKernelArgs *, const int4 *, const int *, unsigned *);
 using CorrBatchFn = void (*)(const CorrArgs *, const int4 *);
+
+static size_t persistent_res_lds(int shape) {
+    switch (shape) {
+        case 0: return kResidualLds<2, 8>;
+        case 1: return kResidualLds<4, 4>;
+        case 2: return kResidualLds<1, 8>;
+        case 3: return kResidualLds<1, 16>;
+        default: return kResidualLds<1, 20>;
+    }
+}
 
 static PersistentFn persistent_fn(int shape) {
     switch (shape) {
@@ -97,9 +116,10 @@ constexpr int kXcds = 8;
 
 struct TrajWork {
     sx_ctx *ctx = nullptr;  // staged coordinates, factors and FFT tables
-    DevBuf qlen, u, amp, ppart, apart, fqt, fq, fq2;
-    bool ap = false;       // shells in arithmetic progression
-    double qstep = 0.0;    // their step in |q| / 2 pi
+    DevBuf qlen, u, qres, amp, ppart, apart, fqt, fq, fq2;
+    int mode = 0;          // shell stepping mode (sx::shell_progression)
+    double qstep = 0.0;    // step and origin of the shells in |q| / 2 pi
+    double q0 = 0.0;
     int chunks = 1;
 };
 
@@ -131,6 +151,7 @@ struct sx_batch {
     std::vector<TrajWork> work;
     int shape = 0;
     int max_resident = 0;  // persistent blocks that fit the GPU at once
+    bool residuals = false;  // some trajectory steps its shells with residual corrections
     std::unique_ptr<Plan> all;
     std::vector<std::unique_ptr<Plan>> one;
     sx_batch_timing timing{};
@@ -171,8 +192,10 @@ sx::OrientKernelArgs orient_args(const TrajWork &w, const sx_trajectory &t, int 
     oa.per_shell_b = 0;
     oa.shape = shape;
     oa.amp = w.amp.as<float2>();
-    oa.ap = w.ap ? 1 : 0;
+    oa.ap = w.mode;
     oa.qstep = w.qstep;
+    oa.q0 = w.q0;
+    oa.qres = w.qres.as<float>();
     return sx::make_orient_args(oa);
 }
 
@@ -284,8 +307,9 @@ int build_plan(sx_batch *b, const std::vector<int> &ks, int shape, int stride, P
     P.groups.clear();
     for (auto &kv : by_len) {
         const sx::FftPlan &pl = b->work[kv.second[0].x].ctx->plan;
-        Plan::CorrGroup g{pl.E, pl.T, (int)citems.size(), (int)kv.second.size(),
-                          sizeof(double2) * pl.Lh + sizeof(double2) * (pl.T / 64)};
+        // corr_all_block reduces a_m in the start of the FFT buffer, so the
+        // dynamic LDS is exactly the transform data (160 KB at Lh = 10240)
+        Plan::CorrGroup g{pl.E, pl.T, (int)citems.size(), (int)kv.second.size(), sx::corr_all_lds_bytes(pl)};
         P.groups.push_back(g);
         citems.insert(citems.end(), kv.second.begin(), kv.second.end());
     }
@@ -308,7 +332,9 @@ int launch_amplitude(sx_batch *b, Plan &P, int shape) {
     hipLaunchKernelGGL(sx::reset_counters_kernel, dim3(1), dim3(kXcds), 0, b->stream, P.counters.as<unsigned>(),
                        kXcds);
     SXB_HIP(b, hipGetLastError());
-    hipLaunchKernelGGL(sx::persistent_fn(shape), dim3(P.persistent_blocks), dim3(256), 0, b->stream,
+    // (the dynamic-LDS attribute of every shape is set once, in sx_batch_stage)
+    const size_t lds = b->residuals ? sx::persistent_res_lds(shape) : 0;
+    hipLaunchKernelGGL(sx::persistent_fn(shape), dim3(P.persistent_blocks), dim3(256), lds, b->stream,
                        P.oka.as<sx::OrientKernelArgs>(), P.items.as<int4>(), P.qstart.as<int>(),
                        P.counters.as<unsigned>());
     SXB_HIP(b, hipGetLastError());
@@ -397,7 +423,7 @@ void sx_batch_destroy(sx_batch *b) {
     b->all.reset();
     b->one.clear();
     for (TrajWork &w : b->work) {
-        for (DevBuf *d : {&w.qlen, &w.u, &w.amp, &w.ppart, &w.apart, &w.fqt, &w.fq, &w.fq2}) d->release();
+        for (DevBuf *d : {&w.qlen, &w.u, &w.qres, &w.amp, &w.ppart, &w.apart, &w.fqt, &w.fq, &w.fq2}) d->release();
         sx_destroy(w.ctx);
     }
     if (b->e0) (void)hipEventDestroy(b->e0);
@@ -437,7 +463,17 @@ int sx_batch_stage(sx_batch *b, const sx_trajectory *trajs, size_t ntraj) {
         if (!w.ctx->plan_ok) return bfail(b, SX_ERR_UNSUPPORTED, "trajectory %zu: NF=%zu too long", k, t.NF);
         std::vector<double> ql(t.nshells);
         for (size_t s = 0; s < t.nshells; ++s) ql[s] = t.qlen[s] / kTwoPi;
-        w.ap = sx::shells_in_progression(ql.data(), (int)t.nshells, &w.qstep);
+        sx::ShellGrid g;
+        w.mode = sx::shell_progression(ql.data(), (int)t.nshells, w.ctx->rmax, &g);
+        w.qstep = g.step;
+        w.q0 = g.q0;
+        if (w.mode == 2) {
+            std::vector<float> rr(t.nshells);
+            for (size_t s = 0; s < t.nshells; ++s) rr[s] = (float)(kTwoPi * g.res[s]);  // radians per Angstrom
+            SXB_HIP(b, w.qres.ensure(sizeof(float) * t.nshells));
+            SXB_HIP(b, hipMemcpy(w.qres.p, rr.data(), sizeof(float) * t.nshells, hipMemcpyHostToDevice));
+            b->residuals = true;
+        }
         w.chunks = (int)std::min<size_t>(t.NM, 8);
         SXB_HIP(b, w.qlen.ensure(sizeof(double) * t.nshells));
         SXB_HIP(b, hipMemcpy(w.qlen.p, ql.data(), sizeof(double) * t.nshells, hipMemcpyHostToDevice));
@@ -455,7 +491,12 @@ int sx_batch_stage(sx_batch *b, const sx_trajectory *trajs, size_t ntraj) {
     SXB_HIP(b, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
     int min_per_cu = 1 << 30;
     for (int sh = 0; sh < sx::kOrientShapes; ++sh) {
-        SXB_HIP(b, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)sx::persistent_fn(sh), 256, 0));
+        const size_t lds = b->residuals ? sx::persistent_res_lds(sh) : 0;
+        if (lds > 0)
+            SXB_HIP(b, hipFuncSetAttribute((const void *)sx::persistent_fn(sh),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        SXB_HIP(b, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)sx::persistent_fn(sh), 256,
+                                                               lds));
         min_per_cu = std::min(min_per_cu, std::max(per_cu, 1));
     }
     b->max_resident = cus * min_per_cu;

@@ -72,12 +72,12 @@ int pick_split_blocks(const sx_ctx *c, long base) {
 // rounds per unit of work; ties go to the smaller split.
 int round_filling_split(long base, int first, long resident, size_t NAp, size_t split_bytes);
 
-int pick_oriented_split(sx_ctx *c, long base, int shape, bool per_row, size_t V, int *split) {
+int pick_oriented_split(sx_ctx *c, long base, int shape, bool per_row, int mode, size_t V, int *split) {
     base = std::max<long>(base, 1);
     *split = pick_split_blocks(c, base);
     if (c->cfg.atom_split > 0) return SX_OK;
     int per_cu = 0, cus = 0;
-    SX_HIP(c, sx::oriented_blocks_per_cu(shape, per_row, &per_cu));
+    SX_HIP(c, sx::oriented_blocks_per_cu(shape, per_row, mode, &per_cu));
     SX_HIP(c, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->cfg.device));
     const size_t split_bytes = sizeof(float2) * c->NFp * std::max<size_t>(V, 1);
     *split = round_filling_split(base, *split, (long)std::max(per_cu, 1) * std::max(cus, 1), c->NAp, split_bytes);
@@ -211,8 +211,31 @@ constexpr double kTuneTerms = 2e10;
 constexpr double kTuneSampleTerms = 5e10;
 
 int default_shape(const sx_ctx *c, int ns, bool per_shell) {
-    if (!c->q_ap || per_shell) return 0;
+    if (!c->q_mode || per_shell) return 0;
     return ns > 16 ? 4 : ns > 8 ? 3 : 0;
+}
+
+// OrientArgs of the current call (stepping mode and tables from run_batch).
+sx::OrientArgs oriented_args(const sx_ctx *c, int ns, int NM, const float *bptr, bool per_shell, int shape,
+                             int split) {
+    sx::OrientArgs oa;
+    oa.coords = c->coords.as<float>();
+    oa.NAp = c->NAp;
+    oa.NFp = c->NFp;
+    oa.qlen = c->qlen.as<double>();
+    oa.u = c->uvec.as<double>();
+    oa.b = bptr;
+    oa.NM = NM;
+    oa.ns = ns;
+    oa.split = split;
+    oa.per_shell_b = per_shell ? 1 : 0;
+    oa.shape = shape;
+    oa.amp = c->amp.as<float2>();
+    oa.ap = c->q_mode;
+    oa.qstep = c->q_step;
+    oa.q0 = c->q_origin;
+    oa.qres = c->qres.as<float>();
+    return oa;
 }
 
 int pick_shape(sx_ctx *c, int ns, int NM, const float *bptr, bool per_shell, int *shape) {
@@ -226,7 +249,7 @@ int pick_shape(sx_ctx *c, int ns, int NM, const float *bptr, bool per_shell, int
     const double terms = (double)ns * NM * (double)c->NA * (double)c->NF;
     if (terms < kTuneTerms) return SX_OK;
     const std::array<size_t, 6> key = {c->NFp, c->NAp, (size_t)NM, (size_t)ns, per_shell ? 1u : 0u,
-                                       c->q_ap ? 1u : 0u};
+                                       (size_t)c->q_mode};
     auto it = c->tuned.find(key);
     if (it != c->tuned.end()) {
         *shape = it->second;
@@ -239,24 +262,10 @@ int pick_shape(sx_ctx *c, int ns, int NM, const float *bptr, bool per_shell, int
     for (int sh = 0; sh < sx::kOrientShapes; ++sh) {
         int split = 1;
         const int src = pick_oriented_split(c, sx::oriented_base_blocks(c->NFp, NM, ns, sh), sh, per_shell,
-                                            (size_t)ns * NM, &split);
+                                            c->q_mode, (size_t)ns * NM, &split);
         if (src != SX_OK) return src;
         SX_HIP(c, c->amp.ensure(sizeof(float2) * (size_t)ns * NM * c->NFp * split));
-        sx::OrientArgs oa;
-        oa.coords = c->coords.as<float>();
-        oa.NAp = c->NAp;
-        oa.NFp = c->NFp;
-        oa.qlen = c->qlen.as<double>();
-        oa.u = c->uvec.as<double>();
-        oa.b = bptr;
-        oa.NM = NM;
-        oa.ns = ns;
-        oa.split = split;
-        oa.per_shell_b = per_shell ? 1 : 0;
-        oa.shape = sh;
-        oa.amp = c->amp.as<float2>();
-        oa.ap = c->q_ap ? 1 : 0;
-        oa.qstep = c->q_step;
+        sx::OrientArgs oa = oriented_args(c, ns, NM, bptr, per_shell, sh, split);
         // atom prefix: whole 64-atom rows, at least one per atom range
         const double frac = std::min(1.0, kTuneSampleTerms / terms);
         size_t na = (size_t)std::ceil(frac * (double)c->NAp / 64.0) * 64;
@@ -311,7 +320,18 @@ int run_batch(sx_ctx *c, int ns, size_t NM, const double *qv, const double *qlen
     if (oriented) {
         std::vector<double> ql(ns);
         for (int s = 0; s < ns; ++s) ql[s] = qlen[s] / kTwoPi;
-        c->q_ap = sx::shells_in_progression(ql.data(), ns, &c->q_step);
+        sx::ShellGrid g;
+        c->q_mode = sx::shell_progression(ql.data(), ns, c->rmax, &g);
+        // residual-corrected steps take one factor per atom
+        if (c->q_mode == 2 && b_shell) c->q_mode = 0;
+        c->q_step = g.step;
+        c->q_origin = g.q0;
+        if (c->q_mode == 2) {
+            std::vector<float> rr(ns);
+            for (int s = 0; s < ns; ++s) rr[s] = (float)(kTwoPi * g.res[s]);  // radians per Angstrom
+            SX_HIP(c, c->qres.ensure(sizeof(float) * ns));
+            SX_HIP(c, hipMemcpyAsync(c->qres.p, rr.data(), sizeof(float) * ns, hipMemcpyHostToDevice, c->stream));
+        }
         SX_HIP(c, c->qlen.ensure(sizeof(double) * ns));
         SX_HIP(c, hipMemcpyAsync(c->qlen.p, ql.data(), sizeof(double) * ns, hipMemcpyHostToDevice, c->stream));
         SX_HIP(c, c->uvec.ensure(sizeof(double) * 3 * NM));
@@ -371,7 +391,7 @@ int run_batch(sx_ctx *c, int ns, size_t NM, const double *qv, const double *qlen
         }
         if (oriented && !mp) {
             int rc = pick_oriented_split(c, sx::oriented_base_blocks(c->NFp, (int)NM, ns, shape), shape,
-                                         b_shell != nullptr, (size_t)ns * NM, &split);
+                                         b_shell != nullptr, c->q_mode, (size_t)ns * NM, &split);
             if (rc != SX_OK) return rc;
         }
         if (mp) {
@@ -413,24 +433,11 @@ int run_batch(sx_ctx *c, int ns, size_t NM, const double *qv, const double *qlen
             }
             SX_HIP(c, sx::launch_multipole(ma, c->stream));
         } else if (oriented) {
-            sx::OrientArgs oa;
-            oa.coords = c->coords.as<float>();
-            oa.NAp = c->NAp;
-            oa.NFp = c->NFp;
-            oa.qlen = c->qlen.as<double>();
-            oa.u = c->uvec.as<double>();
-            oa.b = bptr;
-            oa.NM = (int)NM;
-            oa.ns = ns;
-            oa.split = split;
-            oa.per_shell_b = b_shell ? 1 : 0;
-            oa.shape = shape;
-            oa.amp = c->amp.as<float2>();
-            oa.ap = c->q_ap ? 1 : 0;
-            oa.qstep = c->q_step;
+            const sx::OrientArgs oa = oriented_args(c, ns, (int)NM, bptr, b_shell != nullptr, shape, split);
             SX_HIP(c, sx::launch_amp_oriented(oa, c->stream));
             c->timing.tile_shape = shape + 1;
-            c->timing.shell_progression = oa.ap;
+            c->timing.shell_progression = oa.ap ? 1 : 0;
+            c->timing.shell_residuals = oa.ap == 2 ? 1 : 0;
         } else {
             sx::AmpArgs aa;
             aa.coords = c->coords.as<float>();
@@ -678,7 +685,7 @@ void sx_destroy(sx_ctx *c) {
     if (!c) return;
     (void)hipSetDevice(c->cfg.device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
-    for (DevBuf *d : {&c->coords, &c->b, &c->twL, &c->twL64, &c->twp, &c->twp64, &c->tw64, &c->w, &c->aidx, &c->apair, &c->tail, &c->qrev, &c->qlen, &c->uvec, &c->brow, &c->bshell, &c->mpmom, &c->mpq, &c->mpsph, &c->amp,
+    for (DevBuf *d : {&c->coords, &c->b, &c->twL, &c->twL64, &c->twp, &c->twp64, &c->tw64, &c->w, &c->aidx, &c->apair, &c->tail, &c->qrev, &c->qlen, &c->uvec, &c->brow, &c->bshell, &c->mpmom, &c->mpq, &c->mpsph, &c->qres, &c->redbuf, &c->amp,
                       &c->ppart, &c->apart, &c->fqt, &c->fq, &c->fq2, &c->stage})
         d->release();
     for (auto &e : c->ev)
@@ -733,7 +740,16 @@ int sx_stage(sx_ctx *c, const float *xyz, size_t NA, size_t NF, int layout) {
             SX_HIP(c, hipStreamSynchronize(c->stream));
         }
     }
+    // largest |coordinate|: |d| = |u . r| <= sqrt(3) of it bounds the residual phases
+    SX_HIP(c, c->redbuf.ensure(sizeof(unsigned)));
+    SX_HIP(c, sx::launch_absmax(dst, 3 * plane, c->redbuf.as<unsigned>(), c->stream));
+    unsigned bits = 0;
+    SX_HIP(c, hipMemcpyAsync(&bits, c->redbuf.p, sizeof(unsigned), hipMemcpyDeviceToHost, c->stream));
     SX_HIP(c, hipStreamSynchronize(c->stream));
+    float amax = 0.f;
+    std::memcpy(&amax, &bits, sizeof amax);
+    c->rmax = std::sqrt(3.0) * (double)amax;
+    if (!std::isfinite(c->rmax)) return fail(c, SX_ERR_ARG, "sx_stage: non-finite coordinates");
     c->stage.release();
     int rc = prepare_tables(c);
     if (rc != SX_OK) return rc;

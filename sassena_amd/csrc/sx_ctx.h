@@ -62,12 +62,17 @@ struct sx_ctx {
     sxh::DevBuf coords, b, twL, twL64, twp, twp64, tw64, w, aidx, apair, tail;
     std::vector<int> aidx_h;  // "self": atoms with a nonzero length in the current call
     std::vector<float> b_h;   // host copy of the float factors of sx_set_factors
-    sxh::DevBuf qrev, qlen, uvec, brow, bshell, amp, ppart, apart, fqt, fq, fq2, stage, mpmom, mpq, mpsph;
+    sxh::DevBuf qrev, qlen, uvec, brow, bshell, amp, ppart, apart, fqt, fq, fq2, stage, mpmom, mpq, mpsph, qres,
+        redbuf;
     sx_timing timing{};
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
-    // oriented shells of the current call in arithmetic progression (step in |q| / 2 pi)
-    bool q_ap = false;
-    double q_step = 0.0;
+    // stepping mode of the oriented shells of the current call (sx::shell_progression:
+    // 0 none, 1 exact progression, 2 progression with first-order residual corrections);
+    // origin and step in |q| / 2 pi
+    int q_mode = 0;
+    double q_step = 0.0, q_origin = 0.0;
+    // largest |r| of the staged coordinates (Angstrom): bounds the residual phases
+    double rmax = 0.0;
     // autotuned oriented tile shape per (NFp, NAp, NM, shells, per-shell b, progression)
     std::map<std::array<size_t, 6>, int> tuned;
 };

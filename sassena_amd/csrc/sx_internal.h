@@ -16,6 +16,7 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <vector>
 
 namespace sx {
 
@@ -52,6 +53,10 @@ hipError_t launch_stage_frames(const float *src, size_t NA, size_t NF0, size_t n
 hipError_t launch_stage_atoms(const float *src, size_t a0, size_t na, size_t NF,
                               float *coords, size_t NAp, size_t NFp, hipStream_t stream);
 
+// Largest |x| over n floats into *out (bits of a non-negative float); a
+// fill kernel clears *out first, so the pair may be captured in a graph.
+hipError_t launch_absmax(const float *x, size_t n, unsigned *out, hipStream_t stream);
+
 struct AmpArgs {
     const float *coords;   // [3][NAp][NFp]
     size_t NAp, NFp;
@@ -81,21 +86,40 @@ struct OrientArgs {
     float2 *amp;           // [split][ns*NM][NFp]
     // Shells in arithmetic progression (qlen[s] = qlen[0] + s qstep): the
     // tile evaluates the phasor of one shell and steps to the others by the
-    // three-term recurrence with exp(2 pi i qstep d).
+    // three-term recurrence with exp(2 pi i qstep d).  ap = 2: the same on the
+    // ideal grid q0 + s qstep, each shell's residual qres[s] (radians per
+    // Angstrom, |q| units) corrected to first order; one factor per atom.
     int ap = 0;
     double qstep = 0.0;
+    double q0 = 0.0;
+    const float *qres = nullptr;
     // Atoms summed (the first natoms of the NAp rows); 0: all NAp.  The
     // autotuner times shapes on a prefix of the atoms.
     size_t natoms = 0;
 };
-// True if |q| (any units) is an arithmetic progression to 1e-12 relative;
-// *step receives the common difference.
-bool shells_in_progression(const double *qlen, int ns, double *step);
+// How shells |q_s| (in |q| / 2 pi, turns per Angstrom) may be stepped by the
+// oriented kernel.  mode 1: an arithmetic progression to 1e-12 relative;
+// mode 2: a progression up to residuals r_s = q_s - (q_0 + s step) of at most
+// kMaxShellResidual relative, which the kernel corrects to first order in
+// 2 pi r_s d, provided that phase stays below kMaxResidualPhase for every
+// coordinate (|d| <= rmax, Angstrom), so that the neglected second-order term
+// is below 5e-7; else mode 0 (one phase per shell).  The single-precision
+// powf of a reference q scan (parameters.cpp:1151) leaves residuals of about
+// 6e-8 relative.
+struct ShellGrid {
+    int mode = 0;
+    double q0 = 0.0, step = 0.0;    // turns per Angstrom
+    std::vector<double> res;        // [ns], turns per Angstrom (mode 2)
+};
+constexpr double kMaxShellResidual = 1e-6;
+constexpr double kMaxResidualPhase = 1e-3;
+constexpr double kTwoPiTurns = 6.283185307179586476925286766559;
+int shell_progression(const double *qlen, int ns, double rmax, ShellGrid *g);
 constexpr int kOrientShapes = 5;
 hipError_t launch_amp_oriented(const OrientArgs &a, hipStream_t stream);
 int oriented_base_blocks(size_t NFp, int NM, int ns, int shape);
-// Workgroups of the oriented kernel of this shape that fit one CU at once.
-hipError_t oriented_blocks_per_cu(int shape, bool per_row, int *blocks);
+// Workgroups of the oriented kernel of this shape (and stepping mode) that fit one CU at once.
+hipError_t oriented_blocks_per_cu(int shape, bool per_row, int mode, int *blocks);
 int orient_shape_vm(int shape);
 int orient_shape_vs(int shape);
 
@@ -117,6 +141,9 @@ struct CorrArgs {
     const double *qrev; const float *b; int brow_per_shell;
 };
 hipError_t launch_corr_all(const CorrArgs &a, hipStream_t stream);
+// Dynamic LDS of one coherent correlation block for plan P (the FP64
+// transform data; the block reduction reuses its start).
+size_t corr_all_lds_bytes(const FftPlan &P);
 // halves: 1 transforms the two halves of a padded series one after the other
 // (half the LDS, twice the resident blocks), 2 both at once.
 hipError_t launch_corr_self(const CorrArgs &a, int halves, hipStream_t stream);

@@ -40,6 +40,8 @@ void Timer::stop(const std::string &label) {
     s.count++;
 }
 
+void Timer::add_count(const std::string &label, size_t n) { stats_[label].count += n; }
+
 // ------------------------------------------------------------ control -----
 
 void OrientationVectors::create() {
@@ -487,6 +489,15 @@ void SelfVectorsScatterDevice::stage_data() {
     timer_.stop("st:device");
 }
 
+// Shells of the last oriented call that stepped phasors from shell to shell
+// ("sd:stepped"; "sd:stepped-residual" when corrected for the single-precision
+// rounding of a q scan): counts only, for the run report.
+void AbstractVectorsScatterDevice::count_stepped(size_t ns) {
+    sx_timing tm{};
+    if (sx_get_timing(ctx_, &tm) != SX_OK || !tm.shell_progression) return;
+    timer_.add_count(tm.shell_residuals ? "sd:stepped-residual" : "sd:stepped", ns);
+}
+
 size_t AbstractVectorsScatterDevice::block_size(size_t first) const {
     // consecutive shells with the same number of subvectors share one call
     const size_t NM = init_subvectors(vectors_[first]).size();
@@ -511,6 +522,7 @@ void AbstractVectorsScatterDevice::compute_block(size_t first, size_t ns, const 
         for (size_t s = 0; s < ns; ++s) ql[s] = vectors_[first + s].length();
         check(sx_compute_shells_oriented(ctx_, ns, ql.data(), uu.data(), u.size(), bshell, fqt, fq, fq2),
               "sx_compute_shells_oriented");
+        count_stepped(ns);
         return;
     }
     const size_t NM = init_subvectors(vectors_[first]).size();
@@ -543,6 +555,7 @@ void AbstractVectorsScatterDevice::compute_block(size_t first, size_t ns, const 
     if (shared) {
         check(sx_compute_shells_oriented(ctx_, ns, ql.data(), uu.data(), NM, bshell, fqt, fq, fq2),
               "sx_compute_shells_oriented");
+        count_stepped(ns);
         return;
     }
     check(sx_compute_shells(ctx_, ns, qv.data(), NM, bshell, fqt, fq, fq2), "sx_compute_shells");

@@ -60,6 +60,8 @@ class Timer {
     };
     void start(const std::string &label);
     void stop(const std::string &label);
+    // adds n to the count of `label` without a time (events, not intervals)
+    void add_count(const std::string &label, size_t n);
     const std::map<std::string, Stat> &stats() const { return stats_; }
 
    private:
@@ -311,6 +313,7 @@ class AbstractVectorsScatterDevice : public AbstractScatterDevice {
     bool oriented() const;  // subvectors are |q| u_m for a shared set u
     size_t block_size(size_t first) const override;
     void compute_block(size_t first, size_t ns, const double *bshell, double *fqt, double *fq, double *fq2) override;
+    void count_stepped(size_t ns);
 };
 
 class AllVectorsScatterDevice : public AbstractVectorsScatterDevice {

@@ -15,6 +15,8 @@ from __future__ import annotations
 
 import dataclasses
 import math
+import os
+from concurrent.futures import ThreadPoolExecutor
 
 import numpy as np
 
@@ -83,27 +85,72 @@ def orientations(rng, NM):
 
 
 def linear_shells(q0, q1, n):
-    """A linear |q| scan (scans with exponent 1, parameters.cpp:1142-1162)."""
+    """|q| of a linear scan as the reference builds it (a ``scans`` entry with
+    exponent 1, parameters.cpp:1142-1162): the end points exactly, interior
+    point j at from + powf(j / (n - 1), 1) * (to - from), where the
+    single-precision powf rounds the fraction to float (parameters.cpp:1151),
+    so the interior points sit about 6e-8 relative off the exact grid."""
+    if n == 1:
+        return np.array([(q0 + q1) / 2.0])
+    out = []
+    for j in range(n):
+        if j == 0:
+            out.append(q0)
+        elif j == n - 1:
+            out.append(q1)
+        else:
+            out.append(q0 + float(np.float32(j * 1.0 / (n - 1))) * (q1 - q0))
+    return np.array(out, dtype=np.float64)
+
+
+def exact_linear_shells(q0, q1, n):
+    """|q| on the exact arithmetic grid q0 + j (q1 - q0) / (n - 1) (a q list
+    given point by point, e.g. a vectors file)."""
     if n == 1:
         return np.array([(q0 + q1) / 2.0])
     return np.array([q0 + (j / (n - 1)) * (q1 - q0) if 0 < j < n - 1 else (q0 if j == 0 else q1)
                      for j in range(n)], dtype=np.float64)
 
 
+# Atoms per independently seeded block of random_walk: the result depends
+# only on the seed, not on how many threads generate it.
+_WALK_BLOCK = 2048
+
+
+def _threads():
+    try:
+        return max(1, min(16, len(os.sched_getaffinity(0))))
+    except AttributeError:
+        return max(1, min(16, os.cpu_count() or 1))
+
+
 def random_walk(rng, NA, NF, box, sigma, chunk=256):
-    """r(0) ~ U[0,box)^3, r(t+1) = r(t) + N(0, sigma^2) per component."""
+    """r(0) ~ U[0,box)^3, r(t+1) = r(t) + N(0, sigma^2) per component, in FP64,
+    stored as float32.  One seed is drawn from rng; atoms are generated in
+    blocks of _WALK_BLOCK, each from its own child of that seed, on up to 16
+    threads (numpy releases the GIL while it fills arrays)."""
     xyz = np.empty((3, NA, NF), np.float32)
-    pos = rng.uniform(0.0, box, size=(3, NA))
-    t = 0
-    while t < NF:
-        n = min(chunk, NF - t)
-        steps = rng.normal(0.0, sigma, size=(3, NA, n))
-        if t == 0:
-            steps[:, :, 0] = 0.0
-        path = pos[:, :, None] + np.cumsum(steps, axis=2)
-        xyz[:, :, t:t + n] = path
-        pos = path[:, :, -1]
-        t += n
+    seed = int(rng.integers(0, 2 ** 63))
+    nblk = (NA + _WALK_BLOCK - 1) // _WALK_BLOCK
+    children = np.random.SeedSequence(seed).spawn(nblk)
+
+    def block(k):
+        g = np.random.default_rng(children[k])
+        a0, a1 = k * _WALK_BLOCK, min(NA, (k + 1) * _WALK_BLOCK)
+        pos = g.uniform(0.0, box, size=(3, a1 - a0))
+        t = 0
+        while t < NF:
+            n = min(chunk, NF - t)
+            steps = g.normal(0.0, sigma, size=(3, a1 - a0, n))
+            if t == 0:
+                steps[:, :, 0] = 0.0
+            path = pos[:, :, None] + np.cumsum(steps, axis=2)
+            xyz[:, a0:a1, t:t + n] = path
+            pos = path[:, :, -1]
+            t += n
+
+    with ThreadPoolExecutor(_threads()) as ex:
+        list(ex.map(block, range(nblk)))
     return xyz
 
 
@@ -170,13 +217,16 @@ def case(name, seed=None, scale=1.0):
     return Trajectory(name, xyz, b, linear_shells(q0, q1, ns), orient, kind, box)
 
 
-def c4_batch(seed=404, count=64, scale=1.0):
-    """The uneven batch: count trajectories of log-uniform size."""
+def c4_batch(seed=404, count=64, scale=1.0, frame_scale=None):
+    """The uneven batch: count trajectories of log-uniform size.  scale
+    shrinks the atom counts (and the frame counts unless frame_scale is
+    given, which shrinks the frame counts alone)."""
     rng = np.random.default_rng(seed)
+    fs = scale if frame_scale is None else frame_scale
     out = []
     for i in range(count):
         NA = int(round(math.exp(rng.uniform(math.log(500), math.log(50000))) * scale))
-        NF = int(rng.integers(256, 4097) * scale)
+        NF = int(rng.integers(256, 4097) * fs)
         NA, NF = max(NA, 1), max(NF, 2)
         box = (NA / 0.1) ** (1.0 / 3.0)
         ns = int(rng.integers(10, 61))

@@ -13,7 +13,9 @@ from util import shell_error, small_trajectory, unit_vectors  # noqa: E402
 pytestmark = pytest.mark.gpu
 
 
-def uneven(n, seed):
+def uneven(n, seed, long_frames=()):
+    """n small trajectories of random size, then one of few atoms, shells and
+    orientations for each frame count in long_frames."""
     rng = np.random.default_rng(seed)
     out = []
     for k in range(n):
@@ -23,13 +25,28 @@ def uneven(n, seed):
         ns = int(rng.integers(1, 12))
         NM = int(rng.integers(1, 24))
         out.append((frames, b, np.sort(rng.uniform(0.1, 4.0, ns)), unit_vectors(NM, k + seed)))
+    for k, NF in enumerate(long_frames):
+        NA = int(rng.integers(20, 60))
+        frames, b = small_trajectory(NA, NF, seed=seed * 1000 + k)
+        ns = int(rng.integers(1, 3))
+        out.append((frames, b, np.sort(rng.uniform(0.1, 4.0, ns)), unit_vectors(2, k + seed)))
     return out
+
+
+# One frame count per FFT plan above Lh = 1024: half lengths 1536 ... 10240,
+# with 8, 16, 20 and 24 points per thread and 64 to 512 threads, up to the
+# largest single-block plan (160 KB of LDS).
+LONG_FRAMES = (1100, 1800, 2100, 2600, 3500, 5001, 6000, 8000, 9000)
 
 
 @pytest.mark.parametrize("shape", [0, 1, 2, 3, 4, 5])
 def test_batch_against_oracle(oracle_mod, shape):
+    """Six small trajectories and one per long FFT plan class against the
+    FP64 oracle, every tile shape (0: autotuned)."""
     import sassena_amd
-    trajs = uneven(6, 17)
+    assert [sassena_amd.pad_length(nf) // 2 for nf in LONG_FRAMES] == [1536, 2048, 2560, 3072, 4096, 5120, 6144,
+                                                                          8192, 10240]
+    trajs = uneven(6, 17, LONG_FRAMES if shape in (0, 5) else LONG_FRAMES[2:4])
     with sassena_amd.Batch(tile_shape=shape) as bt:
         bt.stage([(np.ascontiguousarray(f.transpose(2, 1, 0)), b, ql, u) for f, b, ql, u in trajs])
         res = bt.run()
@@ -74,6 +91,48 @@ def test_batch_errors():
         with pytest.raises(sassena_amd.SxError) as e:
             bt.stage([(np.ascontiguousarray(frames.transpose(2, 1, 0)), b, [1.0], unit_vectors(2, 1))])
         assert e.value.code == -5
+
+
+def test_c4_batch_at_full_frame_counts(oracle_mod):
+    """The 64-trajectory batch of the uneven workload with its frame counts
+    (256-4,096: every FFT plan class the workload reaches), shells
+    (10-60, a reference q scan, so residual-corrected steps) and orientations
+    (50-300), atom counts scaled by 0.1: every trajectory of the batch graph
+    equals its own graph (sx_batch_run_one) bit for bit and the
+    single-trajectory entry point to 1e-6; the trajectories of the longest
+    frame counts match the FP64 oracle on their first, middle and last
+    shells (two orientations)."""
+    from sassena_amd import Batch, ScatterDevice, synth
+    trajs = synth.c4_batch(scale=0.1, frame_scale=1.0)
+    assert len(trajs) == 64
+    bt = Batch()
+    bt.stage([(T.xyz, T.b, T.qlen, T.orient) for T in trajs])
+    res = bt.run()
+    t = bt.timing()
+    assert t["terms"] == sum(T.terms() for T in trajs)
+    for k, T in enumerate(trajs):
+        one, _ = bt.run_one(k)
+        assert np.array_equal(one[0], res[k][0]) and np.array_equal(one[1], res[k][1])
+        with ScatterDevice("all") as dev:
+            dev.stage(T.xyz, layout="soa")
+            dev.set_factors(T.b)
+            sfqt, sfq, sfq2 = dev.compute_oriented(T.qlen, T.orient)
+            assert dev.timing()["shell_residuals"] == 1
+        assert np.max(shell_error(res[k][0], sfqt)) < 1e-6, k
+    bt.close()
+    longest = sorted(range(64), key=lambda k: trajs[k].NF)[-3:]
+    for k in longest:
+        T = trajs[k]
+        frames = synth.to_frames_aos(T.xyz)
+        u = T.orient[:2]
+        with ScatterDevice("all") as dev:
+            dev.stage(T.xyz, layout="soa")
+            dev.set_factors(T.b)
+            fqt, fq, fq2 = dev.compute_oriented(T.qlen, u)
+        for s in (0, T.nshells // 2, T.nshells - 1):
+            rfqt, rfq, rfq2 = oracle_mod.compute("all", frames, T.b, T.qlen[s] * u, nthreads=8)
+            assert shell_error(fqt[s], rfqt)[0] < 1e-5, (k, s)
+            assert abs(fq[s] - rfq) < 1e-5 * abs(rfqt[0])
 
 
 def test_time_amplitude_leaves_results():

@@ -102,6 +102,26 @@ def test_sx_run_scans_without_orientations(tmp_path, oracle_mod):
     check_shells(oracle_mod, "all", frames, b, None, q, fqt, fq, fq2)
 
 
+def test_sx_run_scan_with_orientations_steps_shells(tmp_path, oracle_mod):
+    """A scatter.xml q scan (40 points, 0.1-4.0 A^-1, interior points rounded
+    by single-precision powf as parameters.cpp:1151 does) with a file of
+    orientations: the host hands the shells to the oriented entry point,
+    which steps phasors across them with first-order residual corrections
+    (reported as "sd:stepped-residual"); every shell against the oracle."""
+    frames, b, u = setup_case(tmp_path, NA=200, NF=150, NM=12)
+    xml = config("all", "unused").replace(
+        "<vectors><type>file</type><file>unused</file></vectors>",
+        "<vectors><type>scans</type><scans><scan><from>0.1</from><to>4.0</to><points>40</points>"
+        "<base><x>1</x><y>0</y><z>0</z></base></scan></scans></vectors>")
+    (tmp_path / "s.xml").write_text(xml)
+    res = sx_run("--config", str(tmp_path / "s.xml"), cwd=tmp_path)
+    line = next(l for l in res.stderr.splitlines() if "sd:stepped-residual" in l)
+    assert int(line.split("count")[1].split()[0]) == 40
+    q, fqt, fq, fq2 = read_signal(tmp_path)
+    assert np.array_equal(q, oracle_mod.scan(0.1, 4.0, 40))
+    check_shells(oracle_mod, "all", frames, b, u, q, fqt, fq, fq2)
+
+
 @pytest.mark.parametrize("mp", ["sphere", "cylinder"])
 def test_sx_run_multipole(tmp_path, oracle_mod, mp):
     """scattering.average.orientation.type=multipole through the factory."""

@@ -328,6 +328,75 @@ def test_shell_progression_recurrence(sx, oracle_mod, shape):
             check_close(fqt[s], fq[s], fq2[s], *refs[key])
 
 
+@pytest.mark.parametrize("shape", [1, 2, 3, 4, 5])
+def test_scan_shells_residual_steps(sx, oracle_mod, shape):
+    """The headline shells as the reference builds a q scan (interior points
+    rounded by single-precision powf, about 6e-8 relative off the exact
+    grid): the stepped-phasor path with first-order residual corrections,
+    on every tile shape, full-size headline trajectory, two orientations;
+    and a list perturbed by 4e-7 relative (residual phases 20x larger) on
+    the same path."""
+    from sassena_amd import synth
+    T = synth.case("c2")
+    frames = synth.to_frames_aos(T.xyz)
+    u = T.orient[:2]
+    scan = synth.linear_shells(0.1, 4.0, 40)
+    assert np.max(np.abs(scan - synth.exact_linear_shells(0.1, 4.0, 40))) > 1e-8
+    bent = synth.exact_linear_shells(0.1, 4.0, 40) * (1 + 4e-7 * np.cos(np.arange(40)))
+    refs = {}
+    for qlen in (scan, bent):
+        with sx.ScatterDevice("all", tile_shape=shape) as dev:
+            dev.stage(T.xyz, layout="soa")
+            dev.set_factors(T.b)
+            fqt, fq, fq2 = dev.compute_oriented(qlen, u)
+            tm = dev.timing()
+            assert tm["shell_progression"] == 1 and tm["shell_residuals"] == 1
+        for s in (1, 5, 9, 13, 20, 27, 33, 38):
+            key = float(qlen[s])
+            if key not in refs:
+                refs[key] = oracle_mod.compute("all", frames, T.b, qlen[s] * u, nthreads=8)
+            check_close(fqt[s], fq[s], fq2[s], *refs[key])
+
+
+def test_residual_bound_falls_back_to_direct(sx, oracle_mod):
+    """Residuals whose phase over the coordinates could reach 1e-3 rad take
+    the direct path (one phase per shell)."""
+    from sassena_amd import synth
+    frames, b = small_trajectory(60, 80, seed=21, box=20.0)
+    frames = frames + np.float32(40000.0)   # |r| ~ 7e4 A: residual phases near 1e-2 rad
+    u = unit_vectors(3, 21)
+    qlen = synth.linear_shells(0.1, 2.0, 10)
+    with sx.ScatterDevice("all") as dev:
+        dev.stage(frames, layout="frames")
+        dev.set_factors(b)
+        fqt, fq, fq2 = dev.compute_oriented(qlen, u)
+        assert dev.timing()["shell_progression"] == 0
+    for s in (0, 4, 9):
+        check_close(fqt[s], fq[s], fq2[s], *oracle_mod.compute("all", frames, b, qlen[s] * u))
+
+
+def test_c5_full_size_subset_against_oracle(sx, oracle_mod):
+    """The control-variate workload's trajectory at full size (100,000 atoms
+    x 10,000 frames in a 100 A box, pad 20,480) through the oriented entry
+    point: the first, a middle and the last of its 60 scan shells x two
+    orientations against the FP64 oracle."""
+    from sassena_amd import synth
+    T = synth.case("c5")
+    assert (T.NA, T.NF, T.nshells) == (100000, 10000, 60) and sx.pad_length(T.NF) == 20480
+    shells = [0, 29, 59]
+    u = T.orient[:2]
+    with sx.ScatterDevice("all") as dev:
+        dev.stage(T.xyz, layout="soa")
+        dev.set_factors(T.b)
+        fqt, fq, fq2 = dev.compute_oriented(T.qlen[shells], u)
+    frames = synth.to_frames_aos(T.xyz)
+    del T.xyz
+    for i, s in enumerate(shells):
+        ref = oracle_mod.compute("all", frames, T.b, T.qlen[s] * u, nthreads=16)
+        print(f"c5 shell {s}: error {shell_error(fqt[i], ref[0])[0]:.3e}")
+        check_close(fqt[i], fq[i], fq2[i], *ref)
+
+
 def test_c3_full_size_self_properties(sx, oracle_mod):
     """The incoherent workload at full size (50000 atoms x 5000 frames, two of
     its shells x 100 orientations): F_s(q, 0) = sum_j b_j^2 at every q, since

@@ -208,3 +208,107 @@ def test_sphere_multipole_reaches_debye_limit(oracle_mod):
     assert abs(fqt[0].real - debye) < 2e-7 * abs(debye)
     assert abs(fq.real - debye) < 2e-7 * abs(debye)
     assert abs(fqt[0].imag) < 1e-12 * abs(debye)
+
+
+def _moving_cluster(seed, NA=9, NF=3, lo=0.4, hi=3.5):
+    """A few atoms near the origin, slightly moving, every coordinate in
+    [lo, hi] (so z > 0 along a z-like axis)."""
+    rng = np.random.default_rng(seed)
+    r0 = rng.uniform(lo, hi, size=(1, NA, 3))
+    return (r0 + np.cumsum(rng.normal(0, 0.05, size=(NF, NA, 3)), axis=0)).astype(np.float32), rng.uniform(-2, 4, NA)
+
+
+def test_sphere_moments_rebuild_the_plane_wave_amplitude(oracle_mod):
+    """Plane-wave expansion e^{iq.r} = 4 pi sum_lm i^l j_l(qr) Y*_lm(r^) Y_lm(q^):
+    the sphere moments A_lm(t) = 4 pi i^l sum_j b_j j_l(q r_j) Y*_lm(r^_j)
+    (multipole_scatter_device.cpp:428-498) rebuild the plain amplitude
+    A(q, t) = sum_j b_j e^{i q.r_j(t)} (all_vectors_scatter_device.cpp:418-438)
+    at every direction q^ as sum_lm A_lm(t) Y_lm(q^).  A wrong power of i,
+    a missing conjugate or a wrong sign of m breaks the identity (checked
+    below on the moments themselves)."""
+    frames, b = _moving_cluster(11, lo=-1.8, hi=1.8)
+    ql, L = 1.6, 22
+    A = {(l, m): oracle_mod.scatter_multipole("sphere", frames, b, [0, 0, ql], l, m)
+         for l in range(L + 1) for m in range(-l, l + 1)}
+    scale = np.sum(np.abs(b))
+    for th, ph in [(0.3, 1.1), (1.9, 4.4), (2.8, 0.2)]:
+        q = ql * np.array([np.sin(th) * np.cos(ph), np.sin(th) * np.sin(ph), np.cos(th)])
+        direct = oracle_mod.scatter_all(frames, b, q)
+        Y = {k: oracle_mod.ylm(k[0], k[1], th, ph) for k in A}
+        rebuilt = sum(A[k] * Y[k] for k in A)
+        # the reference rounds r, theta, phi to float: ~1e-7 of sum |b|
+        assert np.max(np.abs(rebuilt - direct)) < 2e-6 * scale
+        # planted mistakes: conj(Y) on the q side, or i^-l instead of i^l
+        wrong1 = sum(A[k] * np.conj(Y[k]) for k in A)
+        wrong2 = sum(A[k] * Y[k] * (-1) ** k[0] for k in A)
+        assert np.max(np.abs(wrong1 - direct)) > 1e-3 * scale
+        assert np.max(np.abs(wrong2 - direct)) > 1e-3 * scale
+
+
+@pytest.mark.parametrize("axis", [(0.0, 0.0, 1.0), (0.3, -0.2, 0.9)])
+def test_cylinder_moments_rebuild_the_plane_wave_amplitude(oracle_mod, axis):
+    """Jacobi-Anger in the cylinder frame of the axis (e_r, e_phi, e_z of
+    coor3d.cpp:278-296): e^{i q.r} = e^{i q_z z} [J_0(q_r rho) + 2 sum_n i^n
+    J_n(q_r rho) cos(n (phi - phi'))] for q at azimuth phi'.  With the
+    reference's moments of q (azimuth phi_q, multipole_scatter_device.cpp:
+    900-987: m = 0, 1 the cos / sin parts of order 2l with (-1)^l, m = 2, 3
+    those of order 2l - 1 with i (-1)^(l-1), all times sqrt(2 pi) sqrt(2)),
+    the plain amplitude at the rotated vector q' (same q_r, q_z, azimuth
+    phi_q + D) is
+      A(q', t) = [A_00 + sqrt(2) sum_l (A_l0 cos 2lD + A_l1 sin 2lD
+                  + A_l2 cos (2l-1)D + A_l3 sin (2l-1)D)] / sqrt(2 pi).
+    Every atom has z > 0 and q_z > 0 (parallel_sign +1).  The check pins each
+    of the four moment kinds, their signs, orders and the frame; mistakes
+    planted in the combination (m = 2 and 3 swapped, (-1)^l dropped, the
+    sign of phi - phi_q flipped) fail it."""
+    frames, b = _moving_cluster(12 if axis[2] == 1.0 else 13)
+    base = oracle_mod.cylinder_base(axis)
+    er, ep, ez = base
+    qr, qz, phq = 1.3, 0.8, 0.9
+    q = qr * (np.cos(phq) * er + np.sin(phq) * ep) + qz * ez
+    L = 18
+    mom = [(0, 0)] + [(l, m) for l in range(1, L + 1) for m in range(4)]
+    A = {k: oracle_mod.scatter_multipole("cylinder", frames, b, q, k[0], k[1], axis=axis) for k in mom}
+    scale = np.sum(np.abs(b))
+
+    def rebuild(D, swap=False, nosign=False, flip=False):
+        tot = A[(0, 0)].copy()
+        for l in range(1, L + 1):
+            s = (-1) ** l if nosign else 1
+            d = -D if flip else D
+            a2, a3 = (A[(l, 3)], A[(l, 2)]) if swap else (A[(l, 2)], A[(l, 3)])
+            tot += np.sqrt(2) * (s * A[(l, 0)] * np.cos(2 * l * d) + s * A[(l, 1)] * np.sin(2 * l * d)
+                                 + a2 * np.cos((2 * l - 1) * d) + a3 * np.sin((2 * l - 1) * d))
+        return tot / np.sqrt(2 * np.pi)
+
+    for D in (0.0, 0.7, 2.3, 4.1):
+        qd = qr * (np.cos(phq + D) * er + np.sin(phq + D) * ep) + qz * ez
+        direct = oracle_mod.scatter_all(frames, b, qd)
+        assert np.max(np.abs(rebuild(D) - direct)) < 2e-6 * scale, D
+        if D != 0.0:
+            for bad in ({"swap": True}, {"nosign": True}, {"flip": True}):
+                assert np.max(np.abs(rebuild(D, **bad) - direct)) > 1e-3 * scale, (D, bad)
+
+
+@pytest.mark.parametrize("axis", [(0.0, 0.0, 1.0), (0.3, -0.2, 0.9)])
+def test_cylinder_multipole_sum_is_grafs_theorem(oracle_mod, axis):
+    """Static cluster: (1 / 2 pi) sum over l <= L, m = 0..3 of |A_lm|^2, the
+    cylinder device's fqt[0] (multipole_scatter_device.cpp:866), is the
+    average of |A(q)|^2 over the azimuth of q about the axis, which Graf's
+    addition theorem gives in closed form:
+      sum_ij b_i b_j e^{i q_z (z_i - z_j)} J_0(q_r |rho_i - rho_j|)."""
+    import scipy.special as sp
+    frames, b = _moving_cluster(14, NA=7, NF=1)
+    frames = np.repeat(frames, 2, axis=0)
+    base = oracle_mod.cylinder_base(axis)
+    er, ep, ez = base
+    qr, qz = 1.1, 0.6
+    q = qr * (np.cos(0.4) * er + np.sin(0.4) * ep) + qz * ez
+    fqt, fq, fq2 = oracle_mod.compute_multipole("cylinder", frames, b, q, oracle_mod.cylinder_moments(30), axis=axis)
+    r = frames[0].astype(np.float64)
+    z = r @ ez
+    rho = r - z[:, None] * ez[None]
+    dr = np.linalg.norm(rho[:, None] - rho[None], axis=2)
+    graf = np.sum(b[:, None] * b[None] * np.exp(1j * qz * (z[:, None] - z[None])) * sp.jv(0, qr * dr))
+    assert abs(fqt[0] - graf) < 1e-6 * np.sum(np.abs(b)) ** 2
+    assert abs(fqt[1] - graf) < 1e-6 * np.sum(np.abs(b)) ** 2
