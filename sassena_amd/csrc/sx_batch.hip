@@ -301,7 +301,7 @@ int build_plan(sx_batch *b, const std::vector<int> &ks, int shape, int stride, P
         const int tiles = (int)((t.NF + 255) / 256);
         for (int s = 0; s < (int)t.nshells; ++s)
             for (int ti = 0; ti < tiles; ++ti) fitems.push_back(make_int4(k, s, ti, 0));
-        final_lds = std::max(final_lds, sizeof(double) * (size_t)c->plan.L);
+        final_lds = std::max(final_lds, sx::final_lds_bytes(c->plan.L));
     }
     std::vector<int4> citems;
     P.groups.clear();
@@ -460,7 +460,10 @@ int sx_batch_stage(sx_batch *b, const sx_trajectory *trajs, size_t ntraj) {
         rc = sx_stage(w.ctx, t.xyz, t.NA, t.NF, t.layout);
         if (rc == SX_OK) rc = sx_set_factors(w.ctx, t.b, t.NA);
         if (rc != SX_OK) return bfail(b, rc, "trajectory %zu: %s", k, sx_last_error(w.ctx));
-        if (!w.ctx->plan_ok) return bfail(b, SX_ERR_UNSUPPORTED, "trajectory %zu: NF=%zu too long", k, t.NF);
+        if (!w.ctx->plan_ok || w.ctx->plan.N1 > 1)
+            return bfail(b, SX_ERR_UNSUPPORTED,
+                         "trajectory %zu: NF=%zu above the batch's one-block FFT lengths (at most 10240 frames); "
+                         "run it through sx_compute_shells_oriented", k, t.NF);
         std::vector<double> ql(t.nshells);
         for (size_t s = 0; s < t.nshells; ++s) ql[s] = t.qlen[s] / kTwoPi;
         sx::ShellGrid g;

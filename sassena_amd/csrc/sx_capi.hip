@@ -172,7 +172,7 @@ int prepare_tables(sx_ctx *c) {
     SX_HIP(c, c->w.ensure(sizeof(double2) * L));
     SX_HIP(c, hipMemcpyAsync(c->twL.p, tw.data(), sizeof(float2) * L, hipMemcpyHostToDevice, c->stream));
     SX_HIP(c, hipMemcpyAsync(c->tw64.p, tw64.data(), sizeof(double2) * L, hipMemcpyHostToDevice, c->stream));
-    SX_HIP(c, sx::launch_weights(c->plan, (int)c->NF, c->w.as<double2>(), c->stream));
+    SX_HIP(c, sx::launch_weights(c->plan, (int)c->NF, c->tw64.as<double2>(), c->w.as<double2>(), c->stream));
     // packed pass twiddles: exp(-2 pi i q kk / (Ns R)) at toff[p] + kk R + q
     const sx::FftPlan &P = c->plan;
     std::vector<float2> tp(P.ntw);
@@ -479,7 +479,21 @@ int run_batch(sx_ctx *c, int ns, size_t NM, const double *qv, const double *qlen
             ca.NFp = c->NFp;
             ca.split = split;
             ca.split_stride = split_stride;
-            SX_HIP(c, sx::launch_corr_all(ca, c->stream));
+            if (P.N1 > 1) {
+                // rows of a long plan: per-row parts of a, summed per vector
+                const int wpp = 2 * P.N1;
+                const int rblocks = (int)std::max<size_t>(1, std::min<size_t>(256, NM / 256));
+                SX_HIP(c, c->apair.ensure(sizeof(double2) * wpp * (size_t)ns * NM));
+                SX_HIP(c, c->apart.ensure(sizeof(double) * 4 * (size_t)ns * std::max(chunks, rblocks)));
+                ca.apart = c->apart.as<double>();
+                SX_HIP(c, sx::launch_corr_rows(ca, false, c->apair.as<double2>(), c->stream));
+                SX_HIP(c, sx::launch_self_a_reduce(c->apair.as<double2>(), (long)NM, wpp, ns, rblocks,
+                                                   c->apart.as<double>(), c->stream));
+                fa.apart = c->apart.as<double>();
+                fa.achunks = rblocks;
+            } else {
+                SX_HIP(c, sx::launch_corr_all(ca, c->stream));
+            }
             fa.ppart = c->ppart.as<double>();
         } else {
             SX_HIP(c, sx::launch_series_all(dsp == SX_DSP_SQUARE ? 1 : 2, c->amp.as<float2>(), c->NFp, split,
@@ -507,10 +521,20 @@ int run_batch(sx_ctx *c, int ns, size_t NM, const double *qv, const double *qlen
         ca.qrev = c->qrev.as<double>();
         ca.b = bptr;
         ca.brow_per_shell = b_shell ? 1 : 0;
+        const int tiles = sx::series_tiles(NF);
+        if (tiles > 1) SX_HIP(c, c->apair.ensure(sizeof(double2) * (size_t)tiles * ns * (size_t)pairs));
         SX_HIP(c, sx::launch_series_self(dsp == SX_DSP_SQUARE ? 1 : 2, ca, c->ppart.as<double2>(),
-                                         c->fqt.as<double>(), c->stream));
-        c->timing.correlate_launches++;
+                                         c->fqt.as<double>(), tiles > 1 ? c->apair.as<double2>() : nullptr,
+                                         c->stream));
         fa.chunks = chunks;
+        if (tiles > 1) {
+            const int rblocks = (int)std::max<long>(1, std::min<long>(256, pairs / 256));
+            SX_HIP(c, c->apart.ensure(sizeof(double) * 4 * (size_t)ns * std::max(chunks, rblocks)));
+            SX_HIP(c, sx::launch_self_a_reduce(c->apair.as<double2>(), pairs, tiles, ns, rblocks,
+                                               c->apart.as<double>(), c->stream));
+            fa.achunks = rblocks;
+        }
+        c->timing.correlate_launches++;
         fa.apart = c->apart.as<double>();
         fa.ppart = nullptr;
     } else {
@@ -565,9 +589,19 @@ int run_batch(sx_ctx *c, int ns, size_t NM, const double *qv, const double *qlen
         const size_t apair_bytes = sizeof(double2) * wpp * (size_t)ns * (size_t)pairs;
         int halves = c->cfg.fft_halves;
         if (halves == 0) halves = (P.T <= 256 && apair_bytes <= kSplitBudgetBytes) ? 3 : 2;
-        if (halves == 3 && P.T > 256)
+        if (halves == 3 && P.T > 256 && P.N1 == 1)
             return fail(c, SX_ERR_UNSUPPORTED, "fft_halves=3 needs an FFT plan of at most 256 threads (NF=%zu)", NF);
-        if (halves == 3) {
+        if (P.N1 > 1) {
+            // rows of a long plan (the halves always in different blocks):
+            // one part of a per (pair, half, row), summed per pair
+            const int rpp = 2 * P.N1;
+            SX_HIP(c, c->apair.ensure(std::max<size_t>(sizeof(double2) * rpp * (size_t)ns * (size_t)pairs,
+                                                       sizeof(double2))));
+            SX_HIP(c, sx::launch_corr_rows(ca, true, c->apair.as<double2>(), c->stream));
+            SX_HIP(c, sx::launch_self_a_reduce(c->apair.as<double2>(), pairs, rpp, ns, rblocks,
+                                               c->apart.as<double>(), c->stream));
+            fa.achunks = rblocks;
+        } else if (halves == 3) {
             // Equal blocks, 2 ns chunks of them: a count that is not a multiple of
             // the blocks resident at once leaves a last round with the GPU mostly
             // idle (4,120 blocks on 1,024 slots ran 5 rounds for 4.02 of work).
@@ -776,8 +810,12 @@ int sx_set_factors(sx_ctx *c, const double *b, size_t NA) {
 
 namespace {
 
-int compute_impl(sx_ctx *c, size_t nshells, const double *qv, const double *qlen, const double *u, size_t NM,
-                 const double *b_shell, double *fqt, double *fq, double *fq2) {
+bool needs_fft_rows(const sx_ctx *c) {
+    return c->cfg.dsp_type == SX_DSP_AUTOCORRELATE && c->plan_ok && c->plan.N1 > 1;
+}
+
+int check_compute_args(sx_ctx *c, size_t nshells, const double *qv, const double *qlen, const double *u, size_t NM,
+                       const double *b_shell, double *fqt, double *fq, double *fq2) {
     if (!c) return SX_ERR_ARG;
     if (!c->staged) return fail(c, SX_ERR_STATE, "compute: no staged coordinates");
     if (!b_shell && !c->have_b) return fail(c, SX_ERR_STATE, "compute: scattering factors not set");
@@ -788,8 +826,6 @@ int compute_impl(sx_ctx *c, size_t nshells, const double *qv, const double *qlen
     const bool needs_fft = c->cfg.dsp_type == SX_DSP_AUTOCORRELATE;
     if (needs_fft && !c->plan_ok)
         return fail(c, SX_ERR_UNSUPPORTED, "compute: NF=%zu exceeds the supported correlation length", c->NF);
-    if (!needs_fft && c->cfg.scattering_type == SX_SCATTERING_SELF && c->NF > (size_t)sx::kMaxSeriesFrames)
-        return fail(c, SX_ERR_UNSUPPORTED, "compute: self square/plain supports NF <= %d", sx::kMaxSeriesFrames);
     if (qv) {
         for (size_t i = 0; i < 3 * nshells * NM; ++i)
             if (!std::isfinite(qv[i])) return fail(c, SX_ERR_ARG, "compute: non-finite q vector component");
@@ -799,6 +835,13 @@ int compute_impl(sx_ctx *c, size_t nshells, const double *qv, const double *qlen
         for (size_t i = 0; i < 3 * NM; ++i)
             if (!std::isfinite(u[i])) return fail(c, SX_ERR_ARG, "compute: non-finite orientation component");
     }
+    return SX_OK;
+}
+
+int compute_impl(sx_ctx *c, size_t nshells, const double *qv, const double *qlen, const double *u, size_t NM,
+                 const double *b_shell, double *fqt, double *fq, double *fq2) {
+    int rc = check_compute_args(c, nshells, qv, qlen, u, NM, b_shell, fqt, fq, fq2);
+    if (rc != SX_OK) return rc;
     SX_HIP(c, hipSetDevice(c->cfg.device));
     c->timing = sx_timing{};
     c->timing.pad_length = c->plan_ok ? (size_t)c->plan.L : 0;
@@ -807,12 +850,15 @@ int compute_impl(sx_ctx *c, size_t nshells, const double *qv, const double *qlen
     if (c->cfg.scattering_type == SX_SCATTERING_ALL) {
         const size_t per_shell = sizeof(float2) * NM * c->NFp * 16;  // worst-case split
         per = std::max<size_t>(1, std::min(nshells, kAmpBudgetBytes / std::max<size_t>(per_shell, 1)));
+    } else if (needs_fft_rows(c)) {
+        // per-(pair, half, row) parts of a of a long plan within kSplitBudgetBytes
+        const size_t per_shell = sizeof(double2) * 2 * c->plan.N1 * c->NA * NM;
+        per = std::max<size_t>(1, std::min(nshells, kSplitBudgetBytes / std::max<size_t>(per_shell, 1)));
     }
     for (size_t s0 = 0; s0 < nshells; s0 += per) {
         const int ns = (int)std::min(per, nshells - s0);
-        int rc = run_batch(c, ns, NM, qv ? qv + 3 * NM * s0 : nullptr, qlen ? qlen + s0 : nullptr, u,
-                           b_shell ? b_shell + c->NA * s0 : nullptr, fqt + 2 * c->NF * s0, fq + 2 * s0,
-                           fq2 + 2 * s0);
+        rc = run_batch(c, ns, NM, qv ? qv + 3 * NM * s0 : nullptr, qlen ? qlen + s0 : nullptr, u,
+                       b_shell ? b_shell + c->NA * s0 : nullptr, fqt + 2 * c->NF * s0, fq + 2 * s0, fq2 + 2 * s0);
         if (rc != SX_OK) return rc;
     }
     return SX_OK;

@@ -20,6 +20,7 @@
 // products, would otherwise reach 1e-5 of C(0)); the incoherent path, whose
 // cost is dominated by NA x NM FFTs, transforms in FP32, where the rounding
 // averages out over the pairs.
+#include <cmath>
 #include <type_traits>
 
 #include "sx_fft_device.h"
@@ -28,51 +29,76 @@ namespace sx {
 
 // ------------------------------------------------------------ planning ----
 
-bool make_fft_plan(size_t NF, FftPlan *plan) {
-    struct Cand {
-        int Lh, T, E, odd;
-    };
-    static const Cand cands[] = {
-        {256, 64, 4, 1},    {512, 64, 8, 1},     {1024, 128, 8, 1},  {1536, 64, 24, 3},
-        {2048, 256, 8, 1},  {2560, 128, 20, 5},  {3072, 128, 24, 3}, {4096, 256, 16, 1},
-        {5120, 256, 20, 5}, {6144, 256, 24, 3},  {8192, 512, 16, 1}, {10240, 512, 20, 5},
-    };
-    if (NF == 0) return false;
-    for (const Cand &c : cands) {
-        if ((size_t)c.Lh < NF) continue;
-        FftPlan p;
-        p.Lh = c.Lh;
-        p.L = 2 * c.Lh;
-        p.T = c.T;
-        p.E = c.E;
-        int rem = c.Lh / c.odd;
-        int np = 0;
-        // powers of two: largest radix dividing both the remainder and E
-        while (rem > 1) {
-            int r = 8;
-            while (r > 1 && (rem % r != 0 || c.E % r != 0)) r >>= 1;
-            if (r == 1) return false;
-            p.radix[np++] = r;
-            rem /= r;
-        }
-        if (c.odd > 1) p.radix[np++] = c.odd;
-        p.npass = np;
-        int ns = 1;
-        for (int i = 0; i < np; ++i) {
-            p.ns[i] = ns;
-            ns *= p.radix[i];
-        }
-        if (ns != c.Lh || np < 2) return false;
-        int off = 0;
-        for (int i = 1; i < np; ++i) {
-            p.toff[i] = off;
-            off += p.ns[i] * p.radix[i];
-        }
-        p.ntw = off;
-        *plan = p;
-        return true;
+namespace {
+struct Cand {
+    int Lh, T, E, odd;
+};
+// One-block half-lengths: 2^k, 3 2^k, 5 2^k up to 10240 (160 KB of FP64 in LDS).
+const Cand kCands[] = {
+    {256, 64, 4, 1},    {512, 64, 8, 1},     {1024, 128, 8, 1},  {1536, 64, 24, 3},
+    {2048, 256, 8, 1},  {2560, 128, 20, 5},  {3072, 128, 24, 3}, {4096, 256, 16, 1},
+    {5120, 256, 20, 5}, {6144, 256, 24, 3},  {8192, 512, 16, 1}, {10240, 512, 20, 5},
+};
+
+bool plan_from(const Cand &c, int N1, FftPlan *plan) {
+    FftPlan p;
+    p.Lh = c.Lh;
+    p.N1 = N1;
+    p.L = 2 * N1 * c.Lh;
+    p.T = c.T;
+    p.E = c.E;
+    int rem = c.Lh / c.odd;
+    int np = 0;
+    // powers of two: largest radix dividing both the remainder and E
+    while (rem > 1) {
+        int r = 8;
+        while (r > 1 && (rem % r != 0 || c.E % r != 0)) r >>= 1;
+        if (r == 1) return false;
+        p.radix[np++] = r;
+        rem /= r;
     }
-    return false;
+    if (c.odd > 1) p.radix[np++] = c.odd;
+    p.npass = np;
+    int ns = 1;
+    for (int i = 0; i < np; ++i) {
+        p.ns[i] = ns;
+        ns *= p.radix[i];
+    }
+    if (ns != c.Lh || np < 2) return false;
+    int off = 0;
+    for (int i = 1; i < np; ++i) {
+        p.toff[i] = off;
+        off += p.ns[i] * p.radix[i];
+    }
+    p.ntw = off;
+    *plan = p;
+    return true;
+}
+}  // namespace
+
+bool make_fft_plan(size_t NF, FftPlan *plan) {
+    if (NF == 0) return false;
+    for (const Cand &c : kCands)
+        if ((size_t)c.Lh >= NF) return plan_from(c, 1, plan);
+    // Longer series: N1 rows of a one-block length Lh, N1 * Lh >= NF.  Each
+    // row evaluates N1 inputs per point (the small DFT that forms it), so the
+    // cost per half is about N1 Lh (4 N1 + 1.5 log2 Lh) operations: fewer rows
+    // win over a slightly shorter length.
+    const Cand *best = nullptr;
+    int bestN1 = 0;
+    double bestCost = 0.0;
+    for (int N1 = 2; N1 <= kMaxRows; ++N1)
+        for (const Cand &c : kCands) {
+            const size_t len = (size_t)N1 * c.Lh;
+            if (len < NF) continue;
+            const double cost = (double)len * (4.0 * N1 + 1.5 * std::log2((double)c.Lh));
+            if (!best || cost < bestCost) {
+                best = &c;
+                bestN1 = N1;
+                bestCost = cost;
+            }
+        }
+    return best && plan_from(*best, bestN1, plan);
 }
 
 template <typename C, int NH, int E>
@@ -310,6 +336,118 @@ __global__ __launch_bounds__(256) void self_a_reduce_kernel(const double2 *__res
     }
 }
 
+// Plans of N1 > 1 rows: block (shell s, chunk c of series, half h, row k1)
+// transforms that row of every series of its chunk (one Lh-point FFT each),
+// accumulates its bins' power and writes the series' part of a (block sum of
+// the weighted power) to apart_rows[s][series][2 N1]; the parts of the 2 N1
+// rows of a series are summed by self_a_reduce.  SELF: the series are the
+// (atom, vector) pairs of the shell with phases formed here
+// (self_vectors_scatter_device.cpp:306-319), else the coherent amplitudes of
+// its vectors.
+template <typename C, int E, bool SELF>
+__global__ __launch_bounds__(512) void corr_rows_kernel(CorrArgs a, double2 *__restrict__ apart_rows) {
+    using Acc = typename Cx<C>::R;
+    extern __shared__ double2 lds_d[];
+    C *buf = reinterpret_cast<C *>(lds_d);
+    double2 *red = reinterpret_cast<double2 *>(buf);  // after the last pass, as in corr_all_block
+    const FftPlan &P = a.plan;
+    const int rows = 2 * P.N1;
+    int bid = blockIdx.x;
+    const int hk = bid % rows;
+    bid /= rows;
+    const int c = bid % a.chunks, s = bid / a.chunks;
+    const int h = hk / P.N1, k1 = hk % P.N1;
+    const int tid = threadIdx.x;
+    const long npairs = SELF ? (long)a.NA * a.NM : (long)a.NM;
+    const long p0 = npairs * c / a.chunks, p1 = npairs * (c + 1) / a.chunks;
+    const C *tw = pre_twiddles<C>(a);
+    const C *twp = pass_twiddles<C>(a);
+    const double2 *w = a.w + (size_t)hk * P.Lh;
+    C v[1][E];
+    Acc acc[2][E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) acc[0][e] = acc[1][e] = Acc(0);
+    double *row = a.ppart + ((size_t)s * a.chunks + c) * P.L;
+    const size_t plane = a.NAp * a.NFp;
+    bool first = true;
+    int since = 0;
+    for (long pr = p0; pr < p1; ++pr) {
+        double ar = 0.0, ai = 0.0;
+        int t = tid;
+        asm volatile("" : "+v"(t));
+        if constexpr (SELF) {
+            const int ja = (int)(pr / a.NM), m = (int)(pr % a.NM);
+            const int j = a.aidx ? a.aidx[ja] : ja;
+            const double *q = a.qrev + ((size_t)s * a.NM + m) * 3;
+            RowInput<C, SelfRowSource> in;
+            in.src.X = a.coords + (size_t)j * a.NFp;
+            in.src.Y = in.src.X + plane;
+            in.src.Z = in.src.X + 2 * plane;
+            in.src.qx = q[0];
+            in.src.qy = q[1];
+            in.src.qz = q[2];
+            in.src.hinvL = (double)h / (double)P.L;
+            in.src.b = a.b[(size_t)(a.brow_per_shell ? s : 0) * a.NAp + j];
+            in.tw = tw;
+            in.L = (unsigned)P.L;
+            in.NF = (unsigned)a.NF;
+            in.Lh = (unsigned)P.Lh;
+            in.N1 = P.N1;
+            in.k1 = k1;
+            fft_sweep<C, 1, E>(P, 0, v, acc, buf, twp, tw, w, P.Lh, in, ar, ai, t);
+        } else {
+            RowInput<C, AmpRowSource<C>> in;
+            in.src.p = a.amp + ((size_t)s * a.NM + pr) * a.NFp;
+            in.src.split_stride = a.split_stride;
+            in.src.split = a.split;
+            in.src.h = h;
+            in.src.tw = tw;
+            in.tw = tw;
+            in.L = (unsigned)P.L;
+            in.NF = (unsigned)a.NF;
+            in.Lh = (unsigned)P.Lh;
+            in.N1 = P.N1;
+            in.k1 = k1;
+            fft_sweep<C, 1, E>(P, 0, v, acc, buf, twp, tw, w, P.Lh, in, ar, ai, t);
+        }
+        const double2 r = block_sum2(ar, ai, red, tid, P.T);
+        if (tid == 0) apart_rows[((size_t)s * npairs + pr) * rows + hk] = r;
+        if (sizeof(Acc) == 4 && ++since == kFlushEvery) {
+            flush_power_row<E>(P, acc, row, first, tid, h, k1);
+            first = false;
+            since = 0;
+        }
+    }
+    if (since > 0 || first || sizeof(Acc) != 4) flush_power_row<E>(P, acc, row, first, tid, h, k1);
+}
+
+using RowsFn = void (*)(CorrArgs, double2 *);
+
+template <typename C, bool SELF>
+static RowsFn rows_fn(int E) {
+    switch (E) {
+        case 4: return corr_rows_kernel<C, 4, SELF>;
+        case 8: return corr_rows_kernel<C, 8, SELF>;
+        case 16: return corr_rows_kernel<C, 16, SELF>;
+        case 20: return corr_rows_kernel<C, 20, SELF>;
+        case 24: return corr_rows_kernel<C, 24, SELF>;
+        default: return nullptr;
+    }
+}
+
+hipError_t launch_corr_rows(const CorrArgs &a, bool self, double2 *apart_rows, hipStream_t stream) {
+    if (a.plan.N1 < 2) return hipErrorInvalidValue;
+    const RowsFn k = self ? rows_fn<float2, true>(a.plan.E) : rows_fn<double2, false>(a.plan.E);
+    if (!k) return hipErrorInvalidValue;
+    // the transform data of one row; the block sum reuses its start
+    const size_t lds = (self ? sizeof(float2) : sizeof(double2)) * (size_t)a.plan.Lh;
+    hipError_t e = hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k, dim3((unsigned)(a.nshells * a.chunks * 2 * a.plan.N1)), dim3(a.plan.T), lds, stream, a,
+                       apart_rows);
+    return hipGetLastError();
+}
+
 // The block reduction reuses the FFT buffer (see corr_all_block), so the
 // dynamic LDS is exactly the transform data: 160 KB for 10240 FP64 points.
 template <typename C, int NH>
@@ -405,30 +543,43 @@ hipError_t launch_corr_self(const CorrArgs &a, int halves, hipStream_t stream) {
 
 // ------------------------------------------------------ FP64 transforms ----
 
-// w_f = sum_{tau<NF} c_tau e^{+2 pi i f tau / L}, c_tau = 1 / (NF L (NF - tau))
-__global__ __launch_bounds__(256) void weights_kernel(int L, int NF, int R, double2 *__restrict__ w) {
+// w_f = sum_{tau<NF} c_tau e^{+2 pi i f tau / L}, c_tau = 1 / (NF L (NF - tau)),
+// the phasor stepped by e^{+2 pi i f / L} and re-anchored on the FP64 table
+// tw64 every 64 lags.  Stored in the order the last FFT pass of the plan
+// visits the bins: bin f = 2 (k1 + N1 k2) + h, k2 = j + q Lh / R, at
+// (h N1 + k1) Lh + q Lh / R + j (N1 = 1: the one-block halves).
+__global__ __launch_bounds__(256) void weights_kernel(int L, int NF, int R, int N1, int Lh,
+                                                      const double2 *__restrict__ tw64, double2 *__restrict__ w) {
     const int f = blockIdx.x * 256 + threadIdx.x;
     if (f >= L) return;
     double wr = 0.0, wi = 0.0;
-    const double base = 2.0 * 3.14159265358979323846 / (double)L;
-    long idx = 0;
-    for (int tau = 0; tau < NF; ++tau) {
-        const double c = 1.0 / ((double)NF * (double)L * (double)(NF - tau));
-        double sn, cs;
-        sincos(base * (double)idx, &sn, &cs);
-        wr = fma(c, cs, wr);
-        wi = fma(c, sn, wi);
-        idx += f;
-        if (idx >= L) idx -= L;
+    const double2 st = tw64[f];
+    long anchor = 0;
+    const long jump = (64L * f) % L;
+    for (int t0 = 0; t0 < NF; t0 += 64) {
+        double2 z = tw64[anchor];
+        anchor += jump;
+        if (anchor >= L) anchor -= L;
+        const int t1 = min(t0 + 64, NF);
+        for (int tau = t0; tau < t1; ++tau) {
+            const double c = 1.0 / ((double)NF * (double)L * (double)(NF - tau));
+            wr = fma(c, z.x, wr);
+            wi = fma(c, z.y, wi);
+            const double zr = z.x * st.x - z.y * st.y;
+            z.y = z.x * st.y + z.y * st.x;
+            z.x = zr;
+        }
     }
-    const int stride = L / (2 * R);  // Lh / R
-    const int fh = f >> 1, q = fh / stride, j = fh - q * stride;
-    w[((f & 1) * R + q) * stride + j] = make_double2(wr, wi);
+    const int stride = Lh / R;
+    const int h = f & 1, fh = f >> 1;
+    const int k1 = fh % N1, k2 = fh / N1;
+    const int q = k2 / stride, j = k2 - q * stride;
+    w[((size_t)(h * N1 + k1)) * Lh + (size_t)q * stride + j] = make_double2(wr, wi);
 }
 
-hipError_t launch_weights(const FftPlan &P, int NF, double2 *w, hipStream_t stream) {
+hipError_t launch_weights(const FftPlan &P, int NF, const double2 *tw64, double2 *w, hipStream_t stream) {
     hipLaunchKernelGGL(weights_kernel, dim3((unsigned)((P.L + 255) / 256)), dim3(256), 0, stream, P.L, NF,
-                       P.radix[P.npass - 1], w);
+                       P.radix[P.npass - 1], P.N1, P.Lh, tw64, w);
     return hipGetLastError();
 }
 
@@ -442,7 +593,7 @@ __global__ __launch_bounds__(256) void final_kernel(FinalArgs a) {
 }
 
 hipError_t launch_final(const FinalArgs &a, hipStream_t stream) {
-    const size_t lds = a.ppart ? sizeof(double) * (size_t)a.L : 0;
+    const size_t lds = a.ppart ? final_lds_bytes(a.L) : 0;
     if (lds > 0) {
         hipError_t e =
             hipFuncSetAttribute((const void *)final_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
@@ -541,16 +692,22 @@ hipError_t launch_series_all(int mode, const float2 *amp, size_t NFp, int split,
 
 // "self" with dsp.type square or plain (self_vectors_scatter_device.cpp:115-135):
 // per pair the series S(t) = |b e^{i phi}|^2 or b e^{i phi}; one block per
-// (shell, chunk of pairs) keeps its partial sum over pairs of S(t) in LDS
-// (each frame owned by one thread) and the per-pair a = mean_t S.
-__global__ __launch_bounds__(256) void self_series_kernel(int mode, CorrArgs a, double2 *__restrict__ rows) {
+// (shell, chunk of pairs, frame tile of at most kMaxSeriesFrames frames) keeps
+// its partial sum over pairs of S(t) in LDS (each frame owned by one thread).
+// One tile: the block also forms a = mean_t S per pair and its chunk's
+// sums of a and |a|^2; several tiles: the tile's part of a per pair goes to
+// parts[shell][pair][tile] for self_a_reduce.
+__global__ __launch_bounds__(256) void self_series_kernel(int mode, CorrArgs a, double2 *__restrict__ rows,
+                                                          int tiles, double2 *__restrict__ parts) {
     extern __shared__ double2 row[];
     __shared__ double2 red[4];
     const int tid = threadIdx.x;
-    const int s = blockIdx.x / a.chunks, c = blockIdx.x % a.chunks;
+    const int tile = blockIdx.x % tiles, sc = blockIdx.x / tiles;
+    const int s = sc / a.chunks, c = sc % a.chunks;
+    const int t0 = tile * kMaxSeriesFrames, t1 = min(a.NF, t0 + kMaxSeriesFrames), nt = t1 - t0;
     const long npairs = (long)a.NA * a.NM;
     const long p0 = npairs * c / a.chunks, p1 = npairs * (c + 1) / a.chunks;
-    for (int t = tid; t < a.NF; t += 256) row[t] = make_double2(0.0, 0.0);
+    for (int t = tid; t < nt; t += 256) row[t] = make_double2(0.0, 0.0);
     double sa_r = 0.0, sa_i = 0.0, sa2 = 0.0;
     const size_t plane = a.NAp * a.NFp;
     for (long pr = p0; pr < p1; ++pr) {
@@ -565,8 +722,8 @@ __global__ __launch_bounds__(256) void self_series_kernel(int mode, CorrArgs a, 
         in.qz = q[2];
         in.b = a.b[(size_t)(a.brow_per_shell ? s : 0) * a.NAp + j];
         double ar = 0.0, ai = 0.0;
-        for (int t = tid; t < a.NF; t += 256) {
-            const float2 x = in(t);
+        for (int t = tid; t < nt; t += 256) {
+            const float2 x = in(t0 + t);
             double sr, si;
             if (mode == 1) {
                 sr = (double)x.x * x.x + (double)x.y * x.y;
@@ -583,14 +740,18 @@ __global__ __launch_bounds__(256) void self_series_kernel(int mode, CorrArgs a, 
         const double2 r = block_sum2(ar, ai, red, tid, 256);
         if (tid == 0) {
             const double xr = r.x / a.NF, xi = r.y / a.NF;
-            sa_r += xr;
-            sa_i += xi;
-            sa2 += xr * xr + xi * xi;
+            if (tiles > 1) {
+                parts[((size_t)s * npairs + pr) * tiles + tile] = make_double2(xr, xi);
+            } else {
+                sa_r += xr;
+                sa_i += xi;
+                sa2 += xr * xr + xi * xi;
+            }
         }
     }
-    double2 *dst = rows + ((size_t)s * a.chunks + c) * a.NF;
-    for (int t = tid; t < a.NF; t += 256) dst[t] = row[t];
-    if (tid == 0) {
+    double2 *dst = rows + ((size_t)s * a.chunks + c) * a.NF + t0;
+    for (int t = tid; t < nt; t += 256) dst[t] = row[t];
+    if (tid == 0 && tiles == 1) {
         double *ap = a.apart + ((size_t)s * a.chunks + c) * 4;
         ap[0] = sa_r;
         ap[1] = sa_i;
@@ -617,13 +778,15 @@ __global__ __launch_bounds__(256) void series_rows_final_kernel(int NF, int NM, 
     fqt[((size_t)s * NF + t) * 2 + 1] = im * f;
 }
 
-hipError_t launch_series_self(int mode, const CorrArgs &a, double2 *rows, double *fqt, hipStream_t stream) {
-    const size_t lds = sizeof(double2) * (size_t)a.NF;
+hipError_t launch_series_self(int mode, const CorrArgs &a, double2 *rows, double *fqt, double2 *parts,
+                              hipStream_t stream) {
+    const int tiles = series_tiles(a.NF);
+    const size_t lds = sizeof(double2) * (size_t)std::min(a.NF, kMaxSeriesFrames);
     hipError_t e =
         hipFuncSetAttribute((const void *)self_series_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(self_series_kernel, dim3((unsigned)(a.nshells * a.chunks)), dim3(256), lds, stream, mode, a,
-                       rows);
+    hipLaunchKernelGGL(self_series_kernel, dim3((unsigned)(a.nshells * a.chunks * tiles)), dim3(256), lds, stream,
+                       mode, a, rows, tiles, parts);
     e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(series_rows_final_kernel, dim3((unsigned)((a.NF + 255) / 256), (unsigned)a.nshells), dim3(256),

@@ -505,49 +505,57 @@ struct SelfInputShifted {
 };
 
 // One block of the shell back-transform: shell s, frame tile `tile` (256
-// frames); Psh holds L doubles of LDS.  fqt(tau) = R(tau) / (L (NF - tau) NM)
-// with R = sum_f P_f e^{+2 pi i f tau / L} in FP64, P the sum of the chunk
-// partials; tile 0 also writes fq = sum a / NM and fq2 = sum |a|^2 / NM.
+// frames); Psh holds final_lds_bytes(L) of LDS.  fqt(tau) = R(tau) / (L (NF -
+// tau) NM) with R = sum_f P_f e^{+2 pi i f tau / L} in FP64, P the sum of the
+// chunk partials, staged through LDS kFinalChunk bins at a time; tile 0 also
+// writes fq = sum a / NM and fq2 = sum |a|^2 / NM.
 __device__ __forceinline__ void final_block(const FinalArgs &a, int s, int tile, double *Psh) {
     const int tid = threadIdx.x;
     if (a.ppart != nullptr) {
         const double *base = a.ppart + (size_t)s * a.chunks * a.L;
-        for (int f = tid; f < a.L; f += 256) {
-            double acc = 0.0;
-            for (int c = 0; c < a.chunks; ++c) acc += base[(size_t)c * a.L + f];
-            Psh[f] = acc;
-        }
-        __syncthreads();
         const int tau = tile * 256 + tid;
-        if (tau < a.NF) {
-            // e^{+2 pi i f tau / L} by repeated products with e^{+2 pi i tau / L},
-            // re-anchored on the FP64 table every 64 bins
-            const double2 st = a.tw64[tau];
-            const long L = a.L;
-            long anchor = 0;  // (f * tau) mod L at the anchors
-            const long jump = (64L * tau) % L;
-            double rr0 = 0.0, ri0 = 0.0, rr1 = 0.0, ri1 = 0.0;
-            for (int f0 = 0; f0 < a.L; f0 += 64) {
-                double2 z = a.tw64[anchor];
-                anchor += jump;
-                if (anchor >= L) anchor -= L;
-                const int f1 = min(f0 + 64, a.L);
-                int f = f0;
-                for (; f + 1 < f1; f += 2) {
-                    const double p0 = Psh[f], p1 = Psh[f + 1];
-                    rr0 = fma(p0, z.x, rr0);
-                    ri0 = fma(p0, z.y, ri0);
-                    double zr = z.x * st.x - z.y * st.y, zi = z.x * st.y + z.y * st.x;
-                    rr1 = fma(p1, zr, rr1);
-                    ri1 = fma(p1, zi, ri1);
-                    z.x = zr * st.x - zi * st.y;
-                    z.y = zr * st.y + zi * st.x;
-                }
-                if (f < f1) {
-                    rr0 = fma(Psh[f], z.x, rr0);
-                    ri0 = fma(Psh[f], z.y, ri0);
+        const bool live = tau < a.NF;
+        // e^{+2 pi i f tau / L} by repeated products with e^{+2 pi i tau / L},
+        // re-anchored on the FP64 table every 64 bins
+        const long L = a.L;
+        const double2 st = a.tw64[live ? tau : 0];
+        long anchor = 0;  // (f * tau) mod L at the anchors
+        const long jump = (64L * (live ? tau : 0)) % L;
+        double rr0 = 0.0, ri0 = 0.0, rr1 = 0.0, ri1 = 0.0;
+        for (int fc = 0; fc < a.L; fc += kFinalChunk) {
+            const int nc = min(kFinalChunk, a.L - fc);
+            __syncthreads();  // every thread is done with the previous chunk
+            for (int f = tid; f < nc; f += 256) {
+                double acc = 0.0;
+                for (int c = 0; c < a.chunks; ++c) acc += base[(size_t)c * a.L + fc + f];
+                Psh[f] = acc;
+            }
+            __syncthreads();
+            if (live) {
+                for (int f0 = 0; f0 < nc; f0 += 64) {
+                    double2 z = a.tw64[anchor];
+                    anchor += jump;
+                    if (anchor >= L) anchor -= L;
+                    const int f1 = min(f0 + 64, nc);
+                    int f = f0;
+                    for (; f + 1 < f1; f += 2) {
+                        const double p0 = Psh[f], p1 = Psh[f + 1];
+                        rr0 = fma(p0, z.x, rr0);
+                        ri0 = fma(p0, z.y, ri0);
+                        double zr = z.x * st.x - z.y * st.y, zi = z.x * st.y + z.y * st.x;
+                        rr1 = fma(p1, zr, rr1);
+                        ri1 = fma(p1, zi, ri1);
+                        z.x = zr * st.x - zi * st.y;
+                        z.y = zr * st.y + zi * st.x;
+                    }
+                    if (f < f1) {
+                        rr0 = fma(Psh[f], z.x, rr0);
+                        ri0 = fma(Psh[f], z.y, ri0);
+                    }
                 }
             }
+        }
+        if (live) {
             const double norm = a.norm != 0.0 ? a.norm : 1.0 / (double)a.NM;
             double scale = norm / ((double)a.L * (double)(a.NF - tau));
             double re = (rr0 + rr1) * scale, im = (ri0 + ri1) * scale;
@@ -581,6 +589,89 @@ __device__ __forceinline__ void final_block(const FinalArgs &a, int s, int tile,
         a.fq[2 * s + 1] = (a.conj ? -si : si) * factor;
         a.fq2[2 * s] = s2 * factor;
         a.fq2[2 * s + 1] = 0.0;
+    }
+}
+
+// ------------------------------------------------- series of N1 > 1 rows ---
+
+// Input of row k1 of half h of a series x (plan with N1 > 1 rows of Lh
+// points, M = N1 Lh, L = 2 M):
+//   in(n2) = W_M^{n2 k1} sum_{n1 < N1} y(n2 + Lh n1) W_{N1}^{n1 k1},
+//   y(n) = x(n) W_L^{h n} for n < NF, else 0,
+// the column DFT and twiddle of the four-step factorisation
+// Y[k1 + N1 k2] = sum_{n2} W_Lh^{n2 k2} in(n2).  tw: [L] exp(-2 pi i e / L),
+// so W_{N1} = W_L^{2 Lh} and W_M = W_L^2.  Src(n) returns y(n) (its
+// pre-twiddle applied).
+template <typename C, typename Src>
+struct RowInput {
+    static constexpr bool kPretwiddled = true;
+    Src src;
+    const C *tw;
+    unsigned L, NF, Lh;
+    int N1, k1;
+    __device__ __forceinline__ C operator()(unsigned n2) const {
+        C acc = Cx<C>::make(0, 0);
+        for (int n1 = 0; n1 < N1; ++n1) {
+            const unsigned n = n2 + Lh * (unsigned)n1;
+            if (n >= NF) break;
+            const C y = src(n);
+            const int e = (n1 * k1) % N1;
+            acc = cadd(acc, e == 0 ? y : cmul(y, tw[2u * Lh * (unsigned)e]));
+        }
+        return k1 == 0 ? acc : cmul(acc, tw[(2u * n2 * (unsigned)k1) % L]);
+    }
+};
+
+// Row source of a coherent series: the sum of the atom-range partial
+// amplitudes at frame n, times W_L^{h n}.
+template <typename C>
+struct AmpRowSource {
+    const float2 *p;
+    size_t split_stride;
+    int split, h;
+    const C *tw;
+    __device__ __forceinline__ C operator()(unsigned n) const {
+        float2 s = p[n];
+        for (int k = 1; k < split; ++k) {
+            const float2 t = p[(size_t)k * split_stride + n];
+            s.x += t.x;
+            s.y += t.y;
+        }
+        const C x = Cx<C>::make(s.x, s.y);
+        return h ? cmul(x, tw[n]) : x;
+    }
+};
+
+// Row source of a self series: b exp(2 pi i (q.r_n - h n / L)), the phase
+// and its pre-twiddle formed and reduced in FP64.
+struct SelfRowSource {
+    const float *X, *Y, *Z;
+    double qx, qy, qz, hinvL;
+    float b;
+    __device__ __forceinline__ float2 operator()(unsigned n) const {
+        double p = fma(-(double)n, hinvL, qx * (double)X[n]);
+        p = fma(qy, (double)Y[n], p);
+        p = fma(qz, (double)Z[n], p);
+        const float f = (float)__builtin_amdgcn_fract(p);
+        return make_float2(b * __builtin_amdgcn_cosf(f), b * __builtin_amdgcn_sinf(f));
+    }
+};
+
+// As flush_power_half for row k1 of half h of an N1-row plan: bin k2 of the
+// row is bin 2 (k1 + N1 k2) + h of the power-spectrum row.
+template <int E, typename Acc>
+__device__ __forceinline__ void flush_power_row(const FftPlan &P, Acc (&acc)[2][E], double *row, bool first, int tid,
+                                                int h, int k1) {
+    const int R = P.radix[P.npass - 1];
+    const int stride = P.Lh / R;
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        const int i = e / R, q = e % R;
+        const int k2 = tid + i * P.T + q * stride;
+        const double add = (double)acc[0][e];
+        double *dst = row + 2 * ((size_t)k1 + (size_t)P.N1 * k2) + h;
+        *dst = first ? add : *dst + add;
+        acc[0][e] = Acc(0);
     }
 }
 

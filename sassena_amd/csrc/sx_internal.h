@@ -25,12 +25,17 @@ constexpr int kAtomChunk = 8;     // atoms per coordinate batch of amp_all_kerne
 constexpr int kMaxPasses = 16;
 
 // Stockham plan of one half-length FFT (Lh points); the full correlation
-// length is L = 2*Lh >= 2*NF.  Radices are ordered powers of two first and at
+// length is L = 2*N1*Lh >= 2*NF.  Radices are ordered powers of two first and at
 // most one odd radix last, so every stride before the last pass is a power of
-// two.
+// two.  N1 = 1: each half of a padded series is one Lh-point FFT in one
+// workgroup's LDS.  N1 > 1 (series longer than the largest such FFT): each
+// half is N1 * Lh points, transformed as N1 rows of Lh points whose inputs a
+// small N1-point DFT forms on the fly (row k1 of half h holds the bins
+// 2 (k1 + N1 k2) + h, k2 < Lh).
 struct FftPlan {
     int Lh = 0;
     int L = 0;
+    int N1 = 1;
     int T = 0;      // threads per block
     int E = 0;      // elements per thread per half
     int npass = 0;
@@ -45,6 +50,9 @@ struct FftPlan {
 
 // Returns false if NF cannot be served by a supported FFT length.
 bool make_fft_plan(size_t NF, FftPlan *plan);
+// Largest frame count a plan serves (16 rows of the largest one-block FFT).
+constexpr int kMaxRows = 16;
+constexpr size_t kMaxCorrFrames = (size_t)kMaxRows * 10240;
 
 // ---- launchers (all asynchronous on `stream`; return hipError_t) ----
 
@@ -141,6 +149,13 @@ struct CorrArgs {
     const double *qrev; const float *b; int brow_per_shell;
 };
 hipError_t launch_corr_all(const CorrArgs &a, hipStream_t stream);
+// Plans of N1 > 1 rows: one block per (shell, chunk of series, half, row);
+// each wavefront's part of a per series goes to apart_rows[shell][series][2 N1][wavefronts],
+// which self_a_reduce sums per series (wpp = 2 N1 x wavefronts).  self: the
+// series are (atom, vector) pairs with phases formed in the kernel, else the
+// coherent amplitudes of the vectors.
+hipError_t launch_corr_rows(const CorrArgs &a, bool self, double2 *apart_rows, hipStream_t stream);
+inline int corr_rows_parts(const FftPlan &P) { return 2 * P.N1 * (P.T / 64); }
 // Dynamic LDS of one coherent correlation block for plan P (the FP64
 // transform data; the block reduction reuses its start).
 size_t corr_all_lds_bytes(const FftPlan &P);
@@ -164,10 +179,11 @@ hipError_t launch_self_tail(const CorrArgs &a, double2 *tail, int tblocks, hipSt
 hipError_t launch_self_a_reduce(const double2 *apair, long npairs, int wpp, int nshells, int rblocks,
                                 double *apart, hipStream_t stream);
 
-// w_f = (1/NF) sum_tau e^{+2 pi i f tau / L} / (L (NF - tau)), stored at
-// ((f & 1) R + q) Lh/R + j for f/2 = j + q Lh/R, R the last radix of the plan:
-// the order in which the last FFT pass visits the bins.
-hipError_t launch_weights(const FftPlan &P, int NF, double2 *w, hipStream_t stream);
+// w_f = (1/NF) sum_tau e^{+2 pi i f tau / L} / (L (NF - tau)), stored in the
+// order in which the last FFT pass visits the bins: for f = 2 (k1 + N1 k2) + h,
+// k2 = j + q Lh/R (R the last radix of the plan), at (h N1 + k1) Lh + q Lh/R + j.
+// tw64: [L] exp(+2 pi i e / L).
+hipError_t launch_weights(const FftPlan &P, int NF, const double2 *tw64, double2 *w, hipStream_t stream);
 
 struct FinalArgs {
     int L, NF, NM, nshells, chunks;
@@ -184,6 +200,9 @@ struct FinalArgs {
     double *fqt, *fq, *fq2;
 };
 hipError_t launch_final(const FinalArgs &a, hipStream_t stream);
+// Bins of the summed power spectrum one block of final_kernel holds in LDS at a time.
+constexpr int kFinalChunk = 8192;
+inline size_t final_lds_bytes(int L) { return sizeof(double) * (size_t)(L < kFinalChunk ? L : kFinalChunk); }
 
 // dsp.type square (mode 1) / plain (mode 2) on coherent amplitudes; norm is
 // the final factor of fqt (1/NM for vector averages)
@@ -217,8 +236,13 @@ int multipole_base_blocks(size_t NFp, int ns, int nmo);
 constexpr size_t kSphereTermBytes = sizeof(float) + 4 * sizeof(double);
 hipError_t launch_multipole_sphere_terms(const MultipoleArgs &m, void *sph, hipStream_t stream);
 
-// "self" with dsp.type square (mode 1) / plain (mode 2); NF <= kMaxSeriesFrames
+// "self" with dsp.type square (mode 1) / plain (mode 2), frames in tiles of
+// kMaxSeriesFrames; with more than one tile each tile's part of a per pair
+// goes to parts[shell][pair][tile] (series_tiles(NF) parts per pair) for
+// launch_self_a_reduce, else a.apart receives the chunk sums directly.
 constexpr int kMaxSeriesFrames = 8192;
-hipError_t launch_series_self(int mode, const CorrArgs &a, double2 *rows, double *fqt, hipStream_t stream);
+inline int series_tiles(int NF) { return (NF + kMaxSeriesFrames - 1) / kMaxSeriesFrames; }
+hipError_t launch_series_self(int mode, const CorrArgs &a, double2 *rows, double *fqt, double2 *parts,
+                              hipStream_t stream);
 
 }  // namespace sx

@@ -185,7 +185,7 @@ def test_errors_are_reported(sx):
         with pytest.raises(sx.SxError):
             dev.set_factors(b[:5])
     with sx.ScatterDevice("all") as dev:
-        frames, b = small_trajectory(2, 10300, seed=1)
+        frames, b = small_trajectory(2, 163841, seed=1)
         dev.stage(frames, layout="frames")
         dev.set_factors(b)
         with pytest.raises(sx.SxError) as e:
@@ -458,3 +458,61 @@ def test_self_half_layouts(sx, oracle_mod, NF, halves):
         ref = oracle_mod.compute("self", atoms, b, qlen[s] * u, nthreads=8)
         print(f"NF={NF} halves={halves} shell {s}: error {shell_error(fqt[s], ref[0])[0]:.3e}")
         check_close(fqt[s], fq[s], fq2[s], *ref)
+
+
+@pytest.mark.parametrize("NF", [10241, 13000, 20000, 31000, 50000])
+@pytest.mark.parametrize("kind", ["all", "self"])
+def test_long_series_against_oracle(sx, oracle_mod, kind, NF):
+    """Frame counts above the largest one-block FFT (rows of N1 = 2, 3, 5
+    one-block transforms formed by an on-the-fly column DFT) against the
+    FP64 oracle's 2NF transform, coherent and incoherent, with the direct
+    method's conjugate on one case."""
+    NA = 9 if kind == "all" else 6
+    frames, b = small_trajectory(NA, NF, seed=NF + 3)
+    if kind == "self":
+        b[1] = 0.0
+    qv = 1.1 * unit_vectors(3, NF)
+    coords = frames if kind == "all" else np.ascontiguousarray(frames.transpose(1, 0, 2))
+    methods = ["fftw", "direct"] if NF == 20000 else ["fftw"]
+    for method in methods:
+        with sx.ScatterDevice(kind, "autocorrelate", method) as dev:
+            dev.stage(frames, layout="frames")
+            dev.set_factors(b)
+            fqt, fq, fq2 = dev.compute_shells(np.stack([qv, 2.3 * qv / 1.1]))
+        for s, qq in enumerate((qv, 2.3 * qv / 1.1)):
+            ref = oracle_mod.compute(kind, coords, b, qq, "autocorrelate", method, nthreads=8)
+            print(f"{kind} NF={NF} {method} shell {s}: error {shell_error(fqt[s], ref[0])[0]:.3e}")
+            check_close(fqt[s], fq[s], fq2[s], *ref)
+
+
+@pytest.mark.parametrize("dsp", ["square", "plain"])
+def test_self_series_over_frame_tiles(sx, oracle_mod, dsp):
+    """Incoherent square / plain series longer than one LDS tile of frames
+    (8,192): the per-pair means are summed over tiles."""
+    frames, b = small_trajectory(11, 20000, seed=44)
+    atoms = np.ascontiguousarray(frames.transpose(1, 0, 2))
+    qv = 0.9 * unit_vectors(4, 44)
+    with sx.ScatterDevice("self", dsp) as dev:
+        dev.stage(frames, layout="frames")
+        dev.set_factors(b)
+        fqt, fq, fq2 = dev.compute_shell(qv)
+    check_close(fqt, fq, fq2, *oracle_mod.compute("self", atoms, b, qv, dsp, nthreads=8))
+
+
+def test_long_series_multipole_and_oriented(sx, oracle_mod):
+    """A long series (NF = 12,000) through the oriented entry point (scan
+    shells, residual steps) and the sphere multipole."""
+    from sassena_amd import synth
+    frames, b = small_trajectory(15, 12000, seed=45, box=6.0)
+    frames = frames - np.float32(3.0)
+    u = unit_vectors(3, 45)
+    qlen = synth.linear_shells(0.3, 2.4, 5)
+    mom = oracle_mod.sphere_moments(3)
+    with sx.ScatterDevice("all") as dev:
+        dev.stage(frames, layout="frames")
+        dev.set_factors(b)
+        fqt, fq, fq2 = dev.compute_oriented(qlen, u)
+        mfqt, mfq, mfq2 = dev.compute_multipole("sphere", [[0, 0, 1.2]], mom)
+    for s in (0, 2, 4):
+        check_close(fqt[s], fq[s], fq2[s], *oracle_mod.compute("all", frames, b, qlen[s] * u, nthreads=8))
+    check_close(mfqt[0], mfq[0], mfq2[0], *oracle_mod.compute_multipole("sphere", frames, b, [0, 0, 1.2], mom))
