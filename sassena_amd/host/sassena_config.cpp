@@ -150,7 +150,58 @@ void collect(const XMLElement *e, const std::vector<std::string> &parts, size_t 
         if (c->name == parts[k]) collect(c.get(), parts, k + 1, out);
 }
 
+std::string encode(const std::string &t) {
+    std::string out;
+    for (char ch : t) {
+        switch (ch) {
+            case '<': out += "&lt;"; break;
+            case '>': out += "&gt;"; break;
+            case '&': out += "&amp;"; break;
+            case '"': out += "&quot;"; break;
+            default: out += ch;
+        }
+    }
+    return out;
+}
+
+void dump_element(const XMLElement &e, int depth, std::string &out) {
+    out.append(2 * depth, ' ');
+    out += "<" + e.name;
+    for (const auto &a : e.attributes) out += " " + a.first + "=\"" + encode(a.second) + "\"";
+    if (e.children.empty() && e.text.empty()) {
+        out += "/>\n";
+        return;
+    }
+    out += ">";
+    if (e.children.empty()) {
+        out += encode(e.text) + "</" + e.name + ">\n";
+        return;
+    }
+    out += "\n";
+    if (!e.text.empty()) {
+        out.append(2 * depth + 2, ' ');
+        out += encode(e.text) + "\n";
+    }
+    for (const auto &c : e.children) dump_element(*c, depth + 1, out);
+    out.append(2 * depth, ' ');
+    out += "</" + e.name + ">\n";
+}
+
 }  // namespace
+
+std::string dump_xml(const XMLElement &root) {
+    std::string out = "<?xml version=\"1.0\"?>\n";
+    dump_element(root, 0, out);
+    return out;
+}
+
+std::string dump_xml_text(const std::string &text) {
+    try {
+        return dump_xml(*parse_xml(text));
+    } catch (const terminate_request &) {
+        return std::string();
+    }
+}
 
 std::vector<const XMLElement *> XMLElement::find_all(const std::string &path) const {
     std::vector<std::string> parts;

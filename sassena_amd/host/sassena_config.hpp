@@ -32,6 +32,13 @@ struct XMLElement {
 
 std::unique_ptr<XMLElement> parse_xml(const std::string &text);
 std::unique_ptr<XMLElement> parse_xml_file(const std::string &path);
+// The parsed document serialised again (declaration, elements, attributes
+// and text; comments and processing instructions dropped, two-space
+// indentation): what the reference stores as meta/config and meta/database
+// through libxml2's document dump (xml_interface.cpp:121-128,
+// file_writer_service.cpp:58-62).  dump_xml_text: "" when the text does not parse.
+std::string dump_xml(const XMLElement &root);
+std::string dump_xml_text(const std::string &text);
 
 // Input and output files named in the configuration.
 struct RunFiles {

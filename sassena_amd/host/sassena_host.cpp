@@ -4,6 +4,8 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <fstream>
 #include <random>
 #include <sstream>
@@ -231,6 +233,38 @@ Database Database::neutron_coherent() {
         db.add(e.first, en);
     }
     return db;
+}
+
+namespace {
+// shortest decimal form that reads back as the same double
+std::string shortest(double v) {
+    char buf[40];
+    for (int p = 6; p <= 17; ++p) {
+        std::snprintf(buf, sizeof buf, "%.*g", p, v);
+        if (std::strtod(buf, nullptr) == v) break;
+    }
+    return buf;
+}
+}  // namespace
+
+std::string Database::to_xml() const {
+    std::ostringstream o;
+    o << "<?xml version=\"1.0\"?>\n<root>\n";
+    const char *groups[3] = {"scatterfactors", "sizes", "exclusionfactors"};
+    for (int g = 0; g < 3; ++g) {
+        o << "  <" << groups[g] << ">\n";
+        for (size_t k = 0; k < names_.size(); ++k) {
+            const DatabaseFunction &f = g == 0 ? entries_[k].sfactor : g == 1 ? entries_[k].volume
+                                                                              : entries_[k].exclusion;
+            o << "    <element>\n      <name>" << names_[k] << "</name>\n      <type>" << f.function_type
+              << "</type>\n";
+            for (double c : f.constants) o << "      <param>" << shortest(c) << "</param>\n";
+            o << "    </element>\n";
+        }
+        o << "  </" << groups[g] << ">\n";
+    }
+    o << "</root>\n";
+    return o.str();
 }
 
 ScatterFactors::ScatterFactors(const Database &db, std::vector<size_t> atom_ids, double background_factor,

@@ -148,6 +148,10 @@ class Database {
     // Neutron coherent scattering lengths (fm) of common elements, constant
     // in q (function type 0).
     static Database neutron_coherent();
+    // The entries as a database document in the layout Database::read_xml
+    // reads (database.cpp:31-145): scatterfactors, sizes and
+    // exclusionfactors, one <element> with <name>, <type> and <param>s each.
+    std::string to_xml() const;
 
    private:
     std::vector<std::string> names_;

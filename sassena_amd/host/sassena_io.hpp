@@ -64,7 +64,11 @@ class IResumableWriter : public ISignalWriter {
 // extendible, meta/{rawconfig,config,database}; appends to an existing file.
 class HDF5SignalWriter : public IResumableWriter {
    public:
-    HDF5SignalWriter(const std::string &file, size_t NF, const RunFiles &flags, const std::string &rawconfig);
+    // rawconfig: the configuration file's bytes (meta/rawconfig); config_doc
+    // and database_doc: the parsed configuration and database documents
+    // (meta/config, meta/database; file_writer_service.cpp:52-62)
+    HDF5SignalWriter(const std::string &file, size_t NF, const RunFiles &flags, const std::string &rawconfig,
+                     const std::string &config_doc, const std::string &database_doc);
     ~HDF5SignalWriter() override;
     void write(const CartesianCoor3D &q, const std::vector<std::complex<double>> &fqt, std::complex<double> fq,
                std::complex<double> fq2) override;
@@ -91,10 +95,16 @@ class RawSignalWriter : public IResumableWriter {
     size_t NF_;
 };
 
-std::unique_ptr<IResumableWriter> open_signal_writer(const RunFiles &files, size_t NF, const std::string &config);
+// config: path of the configuration file ("" for none); database_doc: the
+// database document when no database file is named (files.database_file,
+// whose document is stored otherwise).
+std::unique_ptr<IResumableWriter> open_signal_writer(const RunFiles &files, size_t NF, const std::string &config,
+                                                     const std::string &database_doc = std::string());
 std::vector<CartesianCoor3D> remaining_qvectors(const ISignalWriter &writer, const std::vector<CartesianCoor3D> &all);
 
 // HDF5 signal file -> raw signal layout (for tools without an HDF5 reader).
 void convert_signal_h5_to_raw(const std::string &h5, const std::string &raw);
+// The meta/<name> dataset of a signal file (string or character array).
+std::string read_signal_meta(const std::string &h5, const std::string &name);
 
 }  // namespace sassena_amd

@@ -6,8 +6,9 @@
 //   sx_run --config scatter.xml            run and write the signal file
 //   sx_run --config scatter.xml --vectors  print q vectors and orientations
 //   sx_run --h5-to-raw signal.h5 out.raw   convert a signal file
+//   sx_run --h5-meta signal.h5 NAME        print meta/NAME (rawconfig, config, database)
 //   sx_run --convert-trajectory in fmt out.raw   re-read a trajectory
-//   sx_run --write-test-signal file NF nq  exercise a signal writer
+//   sx_run --write-test-signal file NF nq [config.xml]  exercise a signal writer
 //   sx_run --structure file.pdb            print the elements of a PDB
 //   sx_run --database db.xml Q BG E1 E2 ...  print the scattering factors of the listed
 //                                            elements at |q| = Q, background factor BG
@@ -50,6 +51,11 @@ int main(int argc, char **argv) {
     std::string config = "scatter.xml";
     bool vectors_only = false;
     try {
+        if (argc == 4 && !std::strcmp(argv[1], "--h5-meta")) {
+            // prints the meta/<name> dataset of a signal file
+            std::fputs(read_signal_meta(argv[2], argv[3]).c_str(), stdout);
+            return 0;
+        }
         if (argc == 4 && !std::strcmp(argv[1], "--h5-to-raw")) {
             convert_signal_h5_to_raw(argv[2], argv[3]);
             return 0;
@@ -62,12 +68,14 @@ int main(int argc, char **argv) {
             RawTrajectory::write(argv[4], src->atoms(), src->frames(), all.data());
             return 0;
         }
-        if (argc == 5 && !std::strcmp(argv[1], "--write-test-signal")) {
-            // writes nq deterministic shells of NF frames through the signal writer
+        if ((argc == 5 || argc == 6) && !std::strcmp(argv[1], "--write-test-signal")) {
+            // writes nq deterministic shells of NF frames through the signal
+            // writer, with an optional configuration file for the meta group
             RunFiles files;
             files.signal_file = argv[2];
             const size_t NF = std::stoul(argv[3]), nq = std::stoul(argv[4]);
-            std::unique_ptr<IResumableWriter> w = open_signal_writer(files, NF, "");
+            std::unique_ptr<IResumableWriter> w = open_signal_writer(files, NF, argc == 6 ? argv[5] : "",
+                                                                     Database::neutron_coherent().to_xml());
             std::vector<CartesianCoor3D> qs;
             for (size_t i = 0; i < nq; ++i) qs.emplace_back(0.1 * (i + 1), 0.0, 0.5);
             for (const auto &q : remaining_qvectors(*w, qs)) {
@@ -135,7 +143,8 @@ int main(int argc, char **argv) {
         ScatterFactors sf(db, ids, params.scattering.background_factor);
 
         // resume: q vectors already in the signal file are skipped (sassena.cpp:274-305)
-        std::unique_ptr<ISignalWriter> writer = open_signal_writer(files, src->frames(), config);
+        std::unique_ptr<ISignalWriter> writer =
+            open_signal_writer(files, src->frames(), config, Database::neutron_coherent().to_xml());
         std::vector<CartesianCoor3D> todo = remaining_qvectors(*writer, params.scattering.qvectors);
         std::fprintf(stderr, "sx_run: %zu atoms, %zu frames, %zu of %zu q vectors to compute\n", src->atoms(),
                      src->frames(), todo.size(), params.scattering.qvectors.size());

@@ -125,6 +125,26 @@ def test_signal_writer_and_resume(tmp_path, ext):
     assert res.returncode == 1
 
 
+def test_signal_meta_group(tmp_path):
+    """meta/rawconfig holds the configuration file's bytes, meta/config and
+    meta/database the parsed configuration and database documents as string
+    datasets (file_writer_service.cpp:52-62)."""
+    xml = ('<?xml version="1.0"?>\n<!-- a comment -->\n<root><scattering><type>all</type>'
+           '<dsp><type>autocorrelate</type><method>fftw</method></dsp>'
+           '<signal><file>s.h5</file></signal></scattering></root>\n')
+    (tmp_path / "scatter.xml").write_text(xml)
+    f = str(tmp_path / "signal.h5")
+    sx_run("--write-test-signal", f, "4", "2", str(tmp_path / "scatter.xml"))
+    assert sx_run("--h5-meta", f, "rawconfig").stdout == xml
+    cfg = sx_run("--h5-meta", f, "config").stdout
+    assert cfg.startswith('<?xml version="1.0"?>') and "comment" not in cfg
+    for tag in ("<root>", "<type>all</type>", "<method>fftw</method>", "<file>s.h5</file>", "</root>"):
+        assert tag in cfg
+    db = sx_run("--h5-meta", f, "database").stdout
+    assert "<scatterfactors>" in db and "<name>H</name>" in db and "<param>-3.739" in db
+    assert "<exclusionfactors>" in db
+
+
 def test_pdb_structure(tmp_path):
     (tmp_path / "s.pdb").write_text(
         "ATOM      1  N   ALA A   1      11.104   6.134  -6.504  1.00  0.00           N\n"
