@@ -744,6 +744,14 @@ int sx_stage(sx_ctx *c, const float *xyz, size_t NA, size_t NF, int layout) {
     c->NFp = round_up(NF, sx::kWave);
     const size_t plane = c->NAp * c->NFp;
     c->coords.release();
+    // the block must fit the free device memory, with room for the FFT tables
+    size_t free_b = 0, total_b = 0;
+    SX_HIP(c, hipMemGetInfo(&free_b, &total_b));
+    const size_t need = sizeof(float) * 3 * plane + ((size_t)64 << 20);
+    if (need > free_b)
+        return fail(c, SX_ERR_NOMEM,
+                    "sx_stage: %zu atoms x %zu frames need %zu bytes of device memory, %zu of %zu are free", NA, NF,
+                    need, free_b, total_b);
     SX_HIP(c, c->coords.ensure(sizeof(float) * 3 * plane));
     SX_HIP(c, hipMemsetAsync(c->coords.p, 0, sizeof(float) * 3 * plane, c->stream));
     float *dst = c->coords.as<float>();

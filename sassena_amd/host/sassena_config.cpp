@@ -353,6 +353,10 @@ void read_config_text(const std::string &text, const std::string &basedir, Param
     params.stager_target = x.get("stager/target", params.stager_target);
     params.device = (int)x.get("limits/computation/device", (long)params.device);
     params.batch_shells = (size_t)x.get("limits/computation/shells", (long)params.batch_shells);
+    params.limits.stage_data = (size_t)x.get("limits/stage/memory/data", (long)params.limits.stage_data);
+    params.limits.result_buffer =
+        (size_t)x.get("limits/computation/memory/result_buffer", (long)params.limits.result_buffer);
+    params.limits.scale = (size_t)x.get("limits/computation/memory/scale", (long)params.limits.scale);
 
     // scattering (parameters.cpp:370-606)
     if (const XMLElement *s = x.find("scattering")) {

@@ -384,7 +384,11 @@ AbstractScatterDevice::AbstractScatterDevice(const Params &params, ICoordinateSo
 AbstractScatterDevice::~AbstractScatterDevice() { sx_destroy(ctx_); }
 
 void AbstractScatterDevice::check(int rc, const char *what) {
-    if (rc != SX_OK) throw device_error(rc, std::string(what) + ": " + sx_last_error(ctx_));
+    if (rc == SX_OK) return;
+    // device memory exhausted ends the run as the reference's memory checks
+    // do (abstract_scatter_device.cpp:111)
+    if (rc == SX_ERR_NOMEM) throw terminate_request(std::string(what) + ": " + sx_last_error(ctx_));
+    throw device_error(rc, std::string(what) + ": " + sx_last_error(ctx_));
 }
 
 void AbstractScatterDevice::run() {
@@ -680,10 +684,25 @@ MPCylinderScatterDevice::MPCylinderScatterDevice(const Params &params, ICoordina
                                                  ISignalWriter &writer, std::vector<CartesianCoor3D> qvectors)
     : MultipoleScatterDevice(params, src, sf, writer, std::move(qvectors), SX_MULTIPOLE_CYLINDER) {}
 
+void ram_check(const Params &params, size_t NA, size_t NF) {
+    const MemoryLimits &m = params.limits;
+    const size_t result = NF * 2 * sizeof(double);
+    if (result > m.scale * m.result_buffer)
+        throw terminate_request("limits.computation.memory.result_buffer too small: limits.computation.memory."
+                                "result_buffer=" + std::to_string(m.result_buffer) + ", limits.computation.memory."
+                                "scale=" + std::to_string(m.scale) + ", requested: " + std::to_string(result));
+    const size_t data = NA * NF * 3 * sizeof(float);
+    if (data > m.stage_data)
+        throw terminate_request("Insufficient Buffer size for coordinates (limits.stage.memory.data=" +
+                                std::to_string(m.stage_data) + "), requested (bytes): " + std::to_string(data));
+}
+
 std::unique_ptr<IScatterDevice> ScatterDeviceFactory::create(const Params &params, ICoordinateSource &src,
                                                              ScatterFactors &sf, ISignalWriter &writer,
                                                              std::vector<CartesianCoor3D> qvectors) {
     const ScatteringParameters &sp = params.scattering;
+    // memory limits before the device is created and anything is staged
+    ram_check(params, src.atoms(), src.frames());
     if (sp.type == "self") return std::make_unique<SelfVectorsScatterDevice>(params, src, sf, writer, qvectors);
     if (sp.type == "all") {
         if (sp.orientation_type == "vectors" || sp.orientation_type == "none")

@@ -116,12 +116,29 @@ struct ScatteringParameters {
     void create_from_scans();                // parameters.cpp:1125-1189
 };
 
+// limits.stage.memory.data and limits.computation.memory.{result_buffer,scale}
+// with the reference's defaults (parameters.cpp:611-624).
+struct MemoryLimits {
+    size_t stage_data = (size_t)500 * 1024 * 1024;
+    size_t result_buffer = (size_t)100 * 1024 * 1024;
+    size_t scale = 1;
+};
+
 struct Params {
     ScatteringParameters scattering;
     std::string stager_target = "system";
     int device = 0;
     size_t batch_shells = 64;  // q shells per device launch sequence
+    MemoryLimits limits;
 };
+
+// The memory checks that precede staging (AbstractScatterDevice::ram_check,
+// abstract_scatter_device.cpp:80-103, called from run() at :111, and the
+// stager's coordinate buffer check, data_stager.cpp:55-63): the result buffer
+// NF x 16 bytes against scale x limits.computation.memory.result_buffer and
+// the staged block NA x NF x 12 bytes against limits.stage.memory.data.
+// Throws terminate_request naming the limit, its value and the request.
+void ram_check(const Params &params, size_t NA, size_t NF);
 
 // ------------------------------------------------------------ database ----
 

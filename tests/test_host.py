@@ -91,6 +91,29 @@ def test_xml_errors_are_terminate_requests(tmp_path):
     assert res.returncode == 1 and "No q vectors" in res.stderr
 
 
+@pytest.mark.parametrize("limit,msg", [("<stage><memory><data>1000</data></memory></stage>",
+                                        "Insufficient Buffer size for coordinates"),
+                                       ("<computation><memory><result_buffer>500</result_buffer></memory>"
+                                        "</computation>", "result_buffer too small")])
+def test_memory_limits_end_the_run(tmp_path, limit, msg):
+    """The reference's memory checks before staging (data_stager.cpp:55-63,
+    abstract_scatter_device.cpp:80-111): a staged block or result buffer
+    above its configured limit ends the run with a terminate request, before
+    any device is created."""
+    rng = np.random.default_rng(2)
+    frames = rng.normal(size=(40, 30, 3)).astype(np.float32)   # 14,400 bytes, result 640 bytes
+    write_raw_trajectory(tmp_path / "t.raw", frames)
+    (tmp_path / "atoms.txt").write_text("H\n" * 30)
+    (tmp_path / "s.xml").write_text(
+        "<root><sample><structure><file>atoms.txt</file><format>elements</format></structure>"
+        "<framesets><frameset><file>t.raw</file><format>raw</format></frameset></framesets></sample>"
+        "<scattering><type>all</type><vectors><type>single</type><single><x>1</x><y>0</y><z>0</z></single>"
+        "</vectors><signal><file>out.h5</file></signal></scattering>"
+        f"<limits>{limit}</limits></root>")
+    res = sx_run("--config", str(tmp_path / "s.xml"), cwd=tmp_path, check=False)
+    assert res.returncode == 1 and "terminate request" in res.stderr and msg in res.stderr, res.stderr
+
+
 @pytest.mark.parametrize("cell", [False, True])
 def test_dcd_reader(tmp_path, cell):
     rng = np.random.default_rng(3)
