@@ -478,13 +478,16 @@ void sxo_sph_bessel_all(int L, double x, double *j) {
 /* Normalised associated Legendre term of Y_l^m for m >= 0 (Condon-Shortley
    phase): sqrt((2l+1)/(4 pi) (l-m)!/(l+m)!) P_l^m(x). */
 double sxo_ylm_theta(int l, int m, double x) {
-    double pmm = 1.0;
+    /* N_lm P_l^m(x), the theta part of boost::math::spherical_harmonic
+       (multipole_scatter_device.cpp:480), by the upward recurrence in l from
+       N_mm P_m^m = (-1)^m sqrt((2m+1)/(4 pi) prod_{i<=m} (2i-1)/(2i)) (1-x^2)^{m/2},
+       then the factor sqrt((2l+1)/(2m+1) (l-m)! (2m)! / (l+m)!) from log-gamma:
+       no factorial is formed, so it stays finite to high degree. */
     const double somx2 = sqrt((1.0 - x) * (1.0 + x));
-    double fact = 1.0;
-    for (int i = 1; i <= m; ++i) {
-        pmm *= -fact * somx2;
-        fact += 2.0;
-    }
+    double cm = (2.0 * m + 1.0) / (4.0 * SXO_PI);
+    for (int i = 1; i <= m; ++i) cm *= (2.0 * i - 1.0) / (2.0 * i);
+    double pmm = sqrt(cm);
+    for (int i = 1; i <= m; ++i) pmm *= -somx2;
     double plm = pmm;
     if (l > m) {
         double pmmp1 = x * (2.0 * m + 1.0) * pmm;
@@ -496,9 +499,9 @@ double sxo_ylm_theta(int l, int m, double x) {
             plm = pll;
         }
     }
-    double ratio = 1.0; /* (l-m)!/(l+m)! */
-    for (int k = l - m + 1; k <= l + m; ++k) ratio /= (double)k;
-    return sqrt((2.0 * l + 1.0) / (4.0 * SXO_PI) * ratio) * plm;
+    const double lf = 0.5 * (log((2.0 * l + 1.0) / (2.0 * m + 1.0)) + lgamma(l - m + 1.0) + lgamma(2.0 * m + 1.0) -
+                             lgamma(l + m + 1.0));
+    return exp(lf) * plm;
 }
 
 /* Y_l^m(theta, phi) for any |m| <= l; Y_l^{-m} = (-1)^m conj(Y_l^m). */

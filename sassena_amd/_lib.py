@@ -42,7 +42,7 @@ class sx_config(ctypes.Structure):
     _fields_ = [("device", ctypes.c_int), ("scattering_type", ctypes.c_int), ("dsp_type", ctypes.c_int),
                 ("dsp_method", ctypes.c_int), ("profile", ctypes.c_int), ("vector_block", ctypes.c_int),
                 ("atom_split", ctypes.c_int), ("tile_shape", ctypes.c_int),
-                ("fft_halves", ctypes.c_int), ("reserved", ctypes.c_int * 7)]
+                ("fft_halves", ctypes.c_int), ("multipole_terms", ctypes.c_int), ("reserved", ctypes.c_int * 6)]
 
 
 class sx_timing(ctypes.Structure):

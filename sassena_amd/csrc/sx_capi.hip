@@ -9,12 +9,14 @@
 #include <algorithm>
 #include <array>
 #include <map>
+#include <mutex>
 #include <cmath>
 #include <numeric>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "sx_ctx.h"
@@ -291,6 +293,10 @@ int pick_shape(sx_ctx *c, int ns, int NM, const float *bptr, bool per_shell, int
 struct MPSpec {
     int kind;                    // 0 sphere, 1 cylinder
     std::vector<int> moments;    // [nmo][2]
+    // sphere, per moment: c_|m| = sqrt((2|m|+1)/(4 pi) prod_{i<=|m|} (2i-1)/(2i)), the start of the
+    // Legendre recurrence, and 4 pi f_l|m|, f = sqrt((2l+1)/(2|m|+1) (l-|m|)! (2|m|)! / (l+|m|)!)
+    // from log-gamma: no factorial is formed (finite for l <= 200)
+    std::vector<double> ynorm;
     std::vector<double> qinfo;   // per shell: |q| (sphere) or q_r, phi_q, q_z (cylinder)
     double base[9];              // cylinder frame of the axis
 };
@@ -404,11 +410,15 @@ int run_batch(sx_ctx *c, int ns, size_t NM, const double *qv, const double *qlen
             SX_HIP(c, c->mpmom.ensure(sizeof(int) * mp->moments.size()));
             SX_HIP(c, hipMemcpyAsync(c->mpmom.p, mp->moments.data(), sizeof(int) * mp->moments.size(),
                                      hipMemcpyHostToDevice, c->stream));
-            SX_HIP(c, c->mpq.ensure(sizeof(double) * (mp->qinfo.size() + 9)));
+            // q data, the cylinder frame and the sphere's Y_lm factors in one buffer
+            SX_HIP(c, c->mpq.ensure(sizeof(double) * (mp->qinfo.size() + 9 + mp->ynorm.size())));
             SX_HIP(c, hipMemcpyAsync(c->mpq.p, mp->qinfo.data(), sizeof(double) * mp->qinfo.size(),
                                      hipMemcpyHostToDevice, c->stream));
             SX_HIP(c, hipMemcpyAsync(c->mpq.as<double>() + mp->qinfo.size(), mp->base, sizeof(double) * 9,
                                      hipMemcpyHostToDevice, c->stream));
+            if (!mp->ynorm.empty())
+                SX_HIP(c, hipMemcpyAsync(c->mpq.as<double>() + mp->qinfo.size() + 9, mp->ynorm.data(),
+                                         sizeof(double) * mp->ynorm.size(), hipMemcpyHostToDevice, c->stream));
             sx::MultipoleArgs ma;
             ma.coords = c->coords.as<float>();
             ma.NAp = c->NAp;
@@ -417,6 +427,7 @@ int run_batch(sx_ctx *c, int ns, size_t NM, const double *qv, const double *qlen
             ma.qinfo = c->mpq.as<double>();
             ma.base = c->mpq.as<double>() + mp->qinfo.size();
             ma.moments = c->mpmom.as<int>();
+            ma.ynorm = mp->ynorm.empty() ? nullptr : c->mpq.as<double>() + mp->qinfo.size() + 9;
             ma.nmo = (int)NM;
             ma.ns = ns;
             ma.b = bptr;
@@ -426,7 +437,8 @@ int run_batch(sx_ctx *c, int ns, size_t NM, const double *qv, const double *qlen
             // Sphere: the spherical terms of every (atom, frame) once per call
             // (36 B each) when they fit the budget; else the kernel forms them.
             const size_t sph_bytes = sx::kSphereTermBytes * c->NAp * c->NFp;
-            if (mp->kind == 0 && sph_bytes <= kSphereTermBudget) {
+            const int terms = c->cfg.multipole_terms;
+            if (mp->kind == 0 && (terms == 1 || (terms == 0 && sph_bytes <= kSphereTermBudget))) {
                 SX_HIP(c, c->mpsph.ensure(sph_bytes));
                 SX_HIP(c, sx::launch_multipole_sphere_terms(ma, c->mpsph.p, c->stream));
                 ma.sph = c->mpsph.p;
@@ -696,6 +708,7 @@ int sx_create(const sx_config *cfg, sx_ctx **out) {
     if (cfg->dsp_type < SX_DSP_AUTOCORRELATE || cfg->dsp_type > SX_DSP_PLAIN) return SX_ERR_ARG;
     if (cfg->dsp_method != SX_METHOD_FFT && cfg->dsp_method != SX_METHOD_DIRECT) return SX_ERR_ARG;
     if (cfg->fft_halves < 0 || cfg->fft_halves > 3) return SX_ERR_ARG;
+    if (cfg->multipole_terms < 0 || cfg->multipole_terms > 2) return SX_ERR_ARG;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || cfg->device < 0 || cfg->device >= ndev) return SX_ERR_HIP;
     if (hipSetDevice(cfg->device) != hipSuccess) return SX_ERR_HIP;
@@ -720,13 +733,141 @@ void sx_destroy(sx_ctx *c) {
     (void)hipSetDevice(c->cfg.device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     for (DevBuf *d : {&c->coords, &c->b, &c->twL, &c->twL64, &c->twp, &c->twp64, &c->tw64, &c->w, &c->aidx, &c->apair, &c->tail, &c->qrev, &c->qlen, &c->uvec, &c->brow, &c->bshell, &c->mpmom, &c->mpq, &c->mpsph, &c->qres, &c->redbuf, &c->amp,
-                      &c->ppart, &c->apart, &c->fqt, &c->fq, &c->fq2, &c->stage})
+                      &c->ppart, &c->apart, &c->fqt, &c->fq, &c->fq2, &c->stage, &c->stage2})
         d->release();
     for (auto &e : c->ev)
         if (e) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
+
+}  // extern "C"
+
+namespace {
+
+// Host-to-device staging through two pinned bounce buffers shared by the
+// process (allocated once, kStageChunkFloats floats each, or one piece when
+// larger): while the DMA drains one, host threads copy the next piece of the
+// caller's block into the other.  Pieces are atom rows of one plane (SoA,
+// copied with a 2D copy straight into the padded rows), frames (frames-AoS)
+// or atoms (atoms-AoS), the AoS pieces transposed on the device from a
+// staging buffer per bounce buffer.
+struct PinPool {
+    std::mutex mu;
+    void *buf[2] = {nullptr, nullptr};
+    size_t bytes = 0;
+};
+PinPool g_pins;
+
+// copies n bytes on up to 8 host threads
+void parallel_copy(void *dst, const void *src, size_t n) {
+    const size_t kMin = (size_t)4 << 20;
+    const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+    const size_t parts = std::min<size_t>(std::min(8u, hw), std::max<size_t>(1, n / kMin));
+    if (parts <= 1) {
+        std::memcpy(dst, src, n);
+        return;
+    }
+    std::vector<std::thread> th;
+    const size_t per = (n + parts - 1) / parts;
+    for (size_t k = 0; k < parts; ++k) {
+        const size_t o = k * per;
+        if (o >= n) break;
+        const size_t len = std::min(per, n - o);
+        th.emplace_back([=] { std::memcpy((char *)dst + o, (const char *)src + o, len); });
+    }
+    for (auto &t : th) t.join();
+}
+
+int stage_upload(sx_ctx *c, const float *xyz, size_t NA, size_t NF, int layout, float *dst, size_t plane) {
+    struct Piece {
+        const float *src;
+        size_t floats;
+        size_t k, first, count;  // plane (SoA), first row / frame / atom, how many
+    };
+    std::vector<Piece> pieces;
+    size_t maxf = 0;
+    if (layout == SX_LAYOUT_SOA) {
+        const size_t per = std::max<size_t>(1, kStageChunkFloats / NF);
+        for (size_t k = 0; k < 3; ++k)
+            for (size_t a0 = 0; a0 < NA; a0 += per) {
+                const size_t na = std::min(per, NA - a0);
+                pieces.push_back({xyz + (k * NA + a0) * NF, na * NF, k, a0, na});
+            }
+    } else if (layout == SX_LAYOUT_FRAMES_AOS) {
+        // the transpose grid covers 64 frames per y block (<= 65535)
+        const size_t per = std::min<size_t>(std::max<size_t>(1, kStageChunkFloats / (3 * NA)), size_t(65535) * 64);
+        for (size_t f0 = 0; f0 < NF; f0 += per) {
+            const size_t nf = std::min(per, NF - f0);
+            pieces.push_back({xyz + f0 * NA * 3, 3 * NA * nf, 0, f0, nf});
+        }
+    } else {
+        // one y block per atom (<= 65535)
+        const size_t per = std::min<size_t>(std::max<size_t>(1, kStageChunkFloats / (3 * NF)), size_t(65535));
+        for (size_t a0 = 0; a0 < NA; a0 += per) {
+            const size_t na = std::min(per, NA - a0);
+            pieces.push_back({xyz + a0 * NF * 3, 3 * NF * na, 0, a0, na});
+        }
+    }
+    for (const Piece &p : pieces) maxf = std::max(maxf, p.floats);
+    const size_t bytes = sizeof(float) * std::max(maxf, kStageChunkFloats);
+    std::lock_guard<std::mutex> lock(g_pins.mu);
+    if (g_pins.bytes < bytes) {
+        for (void *&b : g_pins.buf) {
+            if (b) (void)hipHostFree(b);
+            b = nullptr;
+        }
+        g_pins.bytes = 0;
+        for (void *&b : g_pins.buf) SX_HIP(c, hipHostMalloc(&b, bytes, hipHostMallocDefault));
+        g_pins.bytes = bytes;
+    }
+    const bool aos = layout != SX_LAYOUT_SOA;
+    if (aos) {
+        SX_HIP(c, c->stage.ensure(sizeof(float) * maxf));
+        SX_HIP(c, c->stage2.ensure(sizeof(float) * maxf));
+    }
+    hipEvent_t done[2] = {nullptr, nullptr};
+    SX_HIP(c, hipEventCreateWithFlags(&done[0], hipEventDisableTiming));
+    SX_HIP(c, hipEventCreateWithFlags(&done[1], hipEventDisableTiming));
+    int rc = SX_OK;
+    for (size_t i = 0; i < pieces.size() && rc == SX_OK; ++i) {
+        const Piece &p = pieces[i];
+        const int bb = (int)(i & 1);
+        // the DMA (and transpose) of piece i - 2 has drained this bounce buffer
+        if (i >= 2) {
+            hipError_t e = hipEventSynchronize(done[bb]);
+            if (e != hipSuccess) {
+                rc = fail(c, SX_ERR_HIP, "sx_stage: %s", hipGetErrorString(e));
+                break;
+            }
+        }
+        parallel_copy(g_pins.buf[bb], p.src, sizeof(float) * p.floats);
+        hipError_t e = hipSuccess;
+        if (!aos) {
+            e = hipMemcpy2DAsync(dst + p.k * plane + p.first * c->NFp, sizeof(float) * c->NFp, g_pins.buf[bb],
+                                 sizeof(float) * NF, sizeof(float) * NF, p.count, hipMemcpyHostToDevice, c->stream);
+        } else {
+            float *stg = bb ? c->stage2.as<float>() : c->stage.as<float>();
+            e = hipMemcpyAsync(stg, g_pins.buf[bb], sizeof(float) * p.floats, hipMemcpyHostToDevice, c->stream);
+            if (e == hipSuccess)
+                e = layout == SX_LAYOUT_FRAMES_AOS
+                        ? sx::launch_stage_frames(stg, NA, p.first, p.count, dst, c->NAp, c->NFp, c->stream)
+                        : sx::launch_stage_atoms(stg, p.first, p.count, NF, dst, c->NAp, c->NFp, c->stream);
+        }
+        if (e == hipSuccess) e = hipEventRecord(done[bb], c->stream);
+        if (e != hipSuccess) rc = fail(c, SX_ERR_HIP, "sx_stage: %s", hipGetErrorString(e));
+    }
+    // the bounce buffers are free again once the stream has drained
+    hipError_t e = hipStreamSynchronize(c->stream);
+    (void)hipEventDestroy(done[0]);
+    (void)hipEventDestroy(done[1]);
+    if (rc == SX_OK && e != hipSuccess) rc = fail(c, SX_ERR_HIP, "sx_stage: %s", hipGetErrorString(e));
+    return rc;
+}
+
+}  // namespace
+
+extern "C" {
 
 int sx_stage(sx_ctx *c, const float *xyz, size_t NA, size_t NF, int layout) {
     if (!c) return SX_ERR_ARG;
@@ -743,45 +884,21 @@ int sx_stage(sx_ctx *c, const float *xyz, size_t NA, size_t NF, int layout) {
     c->NAp = round_up(NA, 64);
     c->NFp = round_up(NF, sx::kWave);
     const size_t plane = c->NAp * c->NFp;
-    c->coords.release();
-    // the block must fit the free device memory, with room for the FFT tables
+    const size_t block = sizeof(float) * 3 * plane;
+    // the block must fit the free device memory (beyond the allocation it
+    // reuses), with room for the FFT tables
     size_t free_b = 0, total_b = 0;
     SX_HIP(c, hipMemGetInfo(&free_b, &total_b));
-    const size_t need = sizeof(float) * 3 * plane + ((size_t)64 << 20);
+    const size_t need = (block > c->coords.cap ? block - c->coords.cap : 0) + ((size_t)64 << 20);
     if (need > free_b)
         return fail(c, SX_ERR_NOMEM,
                     "sx_stage: %zu atoms x %zu frames need %zu bytes of device memory, %zu of %zu are free", NA, NF,
-                    need, free_b, total_b);
-    SX_HIP(c, c->coords.ensure(sizeof(float) * 3 * plane));
-    SX_HIP(c, hipMemsetAsync(c->coords.p, 0, sizeof(float) * 3 * plane, c->stream));
+                    block, free_b, total_b);
+    SX_HIP(c, c->coords.ensure(block));  // kept when NA and NF need no more room than before
+    SX_HIP(c, hipMemsetAsync(c->coords.p, 0, block, c->stream));
     float *dst = c->coords.as<float>();
-    if (layout == SX_LAYOUT_SOA) {
-        for (int k = 0; k < 3; ++k)
-            SX_HIP(c, hipMemcpy2DAsync(dst + k * plane, sizeof(float) * c->NFp, xyz + k * NA * NF, sizeof(float) * NF,
-                                       sizeof(float) * NF, NA, hipMemcpyHostToDevice, c->stream));
-    } else if (layout == SX_LAYOUT_FRAMES_AOS) {
-        // frames per chunk; the transpose grid covers 64 frames per y block (<= 65535)
-        const size_t per = std::min<size_t>(std::max<size_t>(1, kStageChunkFloats / (3 * NA)), size_t(65535) * 64);
-        SX_HIP(c, c->stage.ensure(sizeof(float) * 3 * NA * std::min(per, NF)));
-        for (size_t f0 = 0; f0 < NF; f0 += per) {
-            const size_t nf = std::min(per, NF - f0);
-            SX_HIP(c, hipMemcpyAsync(c->stage.p, xyz + f0 * NA * 3, sizeof(float) * 3 * NA * nf,
-                                     hipMemcpyHostToDevice, c->stream));
-            SX_HIP(c, sx::launch_stage_frames(c->stage.as<float>(), NA, f0, nf, dst, c->NAp, c->NFp, c->stream));
-            SX_HIP(c, hipStreamSynchronize(c->stream));
-        }
-    } else {
-        // atoms per chunk; one y block per atom (<= 65535)
-        const size_t per = std::min<size_t>(std::max<size_t>(1, kStageChunkFloats / (3 * NF)), size_t(65535));
-        SX_HIP(c, c->stage.ensure(sizeof(float) * 3 * NF * std::min(per, NA)));
-        for (size_t a0 = 0; a0 < NA; a0 += per) {
-            const size_t na = std::min(per, NA - a0);
-            SX_HIP(c, hipMemcpyAsync(c->stage.p, xyz + a0 * NF * 3, sizeof(float) * 3 * NF * na,
-                                     hipMemcpyHostToDevice, c->stream));
-            SX_HIP(c, sx::launch_stage_atoms(c->stage.as<float>(), a0, na, NF, dst, c->NAp, c->NFp, c->stream));
-            SX_HIP(c, hipStreamSynchronize(c->stream));
-        }
-    }
+    int rc = stage_upload(c, xyz, NA, NF, layout, dst, plane);
+    if (rc != SX_OK) return rc;
     // largest |coordinate|: |d| = |u . r| <= sqrt(3) of it bounds the residual phases
     SX_HIP(c, c->redbuf.ensure(sizeof(unsigned)));
     SX_HIP(c, sx::launch_absmax(dst, 3 * plane, c->redbuf.as<unsigned>(), c->stream));
@@ -793,7 +910,8 @@ int sx_stage(sx_ctx *c, const float *xyz, size_t NA, size_t NF, int layout) {
     c->rmax = std::sqrt(3.0) * (double)amax;
     if (!std::isfinite(c->rmax)) return fail(c, SX_ERR_ARG, "sx_stage: non-finite coordinates");
     c->stage.release();
-    int rc = prepare_tables(c);
+    c->stage2.release();
+    rc = prepare_tables(c);
     if (rc != SX_OK) return rc;
     c->staged = true;
     return SX_OK;
@@ -924,6 +1042,18 @@ int sx_compute_multipole(sx_ctx *c, int kind, size_t nshells, const double *qv, 
     MPSpec mp;
     mp.kind = kind;
     mp.moments.assign(moments, moments + 2 * nmo);
+    if (kind == SX_MULTIPOLE_SPHERE) {
+        mp.ynorm.resize(2 * nmo);
+        for (size_t k = 0; k < nmo; ++k) {
+            const int l = moments[2 * k], am = std::abs(moments[2 * k + 1]);
+            double cm = (2.0 * am + 1.0) / (4.0 * M_PI);
+            for (int i = 1; i <= am; ++i) cm *= (2.0 * i - 1.0) / (2.0 * i);
+            const double lf = 0.5 * (std::log((2.0 * l + 1.0) / (2.0 * am + 1.0)) + std::lgamma(l - am + 1.0) +
+                                     std::lgamma(2.0 * am + 1.0) - std::lgamma(l + am + 1.0));
+            mp.ynorm[2 * k] = std::sqrt(cm);
+            mp.ynorm[2 * k + 1] = 4.0 * M_PI * std::exp(lf);
+        }
+    }
     for (double &x : mp.base) x = 0.0;
     if (kind == SX_MULTIPOLE_CYLINDER) {
         // CartesianVectorBase(axis) (coor3d.cpp:278-296)

@@ -16,7 +16,7 @@
 namespace sxh {
 
 constexpr double kTwoPi = 6.283185307179586476925286766559;
-constexpr size_t kStageChunkFloats = size_t(64) << 20;   // 256 MB staging chunks
+constexpr size_t kStageChunkFloats = size_t(16) << 20;   // 64 MB staging pieces
 constexpr size_t kAmpBudgetBytes = size_t(24) << 30;      // amplitude buffer per batch
 
 struct DevBuf {
@@ -62,8 +62,8 @@ struct sx_ctx {
     sxh::DevBuf coords, b, twL, twL64, twp, twp64, tw64, w, aidx, apair, tail;
     std::vector<int> aidx_h;  // "self": atoms with a nonzero length in the current call
     std::vector<float> b_h;   // host copy of the float factors of sx_set_factors
-    sxh::DevBuf qrev, qlen, uvec, brow, bshell, amp, ppart, apart, fqt, fq, fq2, stage, mpmom, mpq, mpsph, qres,
-        redbuf;
+    sxh::DevBuf qrev, qlen, uvec, brow, bshell, amp, ppart, apart, fqt, fq, fq2, stage, stage2, mpmom, mpq, mpsph,
+        qres, redbuf;
     sx_timing timing{};
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     // stepping mode of the oriented shells of the current call (sx::shell_progression:

@@ -218,6 +218,7 @@ struct MultipoleArgs {
     const double *qinfo;   // sphere: |q| per shell; cylinder: (q_r, phi_q, q_z) per shell
     const double *base;    // cylinder frame, 9 doubles
     const int *moments;    // [nmo][2]
+    const double *ynorm = nullptr;  // sphere: [nmo][2] (c_|m|, 4 pi f_l|m|) of plm_scaled
     int nmo, ns;
     const float *b;
     int per_shell_b;

@@ -30,6 +30,7 @@ struct MultipoleKernelArgs {
     const double *qinfo;    // per shell: sphere |q|; cylinder q_r, phi_q, q_z (3 doubles)
     const double *base;     // cylinder frame er, ephi, ez (9 doubles)
     const int *moments;     // [nmo][2] (l, m)
+    const double *ynorm;    // sphere: [nmo][2] start of the Legendre recurrence c_|m| and final 4 pi f_l|m|
     int nmo, ns, mblocks;
     const float *b;         // [nb][NAp]
     int per_shell_b;
@@ -175,14 +176,15 @@ __device__ __forceinline__ void cyl_bessel_orders(const int (&ns)[kMoments], int
         if (ns[q] >= 2) jv[q] = cap[q] * inorm;
 }
 
-// P_l^m(x) with the Condon-Shortley phase, m >= 0, somx2 = sqrt(1 - x^2) given
-// (the caller applies the normalisation ylm_norm).
-__device__ double plm_raw(int l, int m, double x, double somx2) {
-    double pmm = 1.0, fact = 1.0;
-    for (int i = 1; i <= m; ++i) {
-        pmm *= -fact * somx2;
-        fact += 2.0;
-    }
+// N_mm P_l^m(x) with the Condon-Shortley phase, m >= 0, somx2 = sqrt(1 - x^2):
+// the upward recurrence in l started from N_mm P_m^m = cm (-somx2)^m, with
+// cm = sqrt((2m + 1) / (4 pi) prod_{i<=m} (2i - 1) / (2i)) (host table), so
+// that no factorial is formed; the caller multiplies the finished sums by
+// f_lm = N_lm / N_mm = sqrt((2l+1)/(2m+1) (l-m)! (2m)! / (l+m)!) (host table).
+// Every intermediate stays finite for l <= 200.
+__device__ double plm_scaled(int l, int m, double x, double somx2, double cm) {
+    double pmm = cm;
+    for (int i = 1; i <= m; ++i) pmm *= -somx2;
     if (l == m) return pmm;
     double pmmp1 = x * (2.0 * m + 1.0) * pmm;
     double plm = pmmp1;
@@ -197,13 +199,6 @@ __device__ double plm_raw(int l, int m, double x, double somx2) {
         plm = pll;
     }
     return plm;
-}
-
-// sqrt((2l+1)/(4 pi) (l-m)!/(l+m)!), m >= 0
-__device__ double ylm_norm(int l, int m) {
-    double ratio = 1.0;
-    for (int k = l - m + 1; k <= l + m; ++k) ratio /= (double)k;
-    return sqrt((2.0 * l + 1.0) / (4.0 * kPi) * ratio);
 }
 
 // coor3d.cpp:168-202
@@ -295,11 +290,13 @@ __global__ __launch_bounds__(256, KIND == 0 ? 3 : 2) void multipole_kernel(Multi
     const int nk = min(kMoments, a.nmo - k0);
     const size_t t = (size_t)ftile * 64 + lane;
     int ls[kMoments], ms[kMoments];
+    double cms[kMoments];  // sphere: start of each moment's Legendre recurrence
 #pragma unroll
     for (int k = 0; k < kMoments; ++k) {
         const int kk = min(k0 + k, a.nmo - 1);
         ls[k] = a.moments[2 * kk];
         ms[k] = a.moments[2 * kk + 1];
+        cms[k] = kind == 0 ? a.ynorm[2 * kk] : 0.0;
     }
     double ar[kMoments], ai[kMoments];
     int lhi = 0;
@@ -373,7 +370,7 @@ __global__ __launch_bounds__(256, KIND == 0 ? 3 : 2) void multipole_kernel(Multi
                 const int l = ls[k], m = ms[k], am = m < 0 ? -m : m;
                 const double jl = jv[k];
                 // the Y_lm normalisation and 4 pi scale the finished sums
-                const double th = plm_raw(l, am, ct, somx2);
+                const double th = plm_scaled(l, am, ct, somx2, cms[k]);
                 if (am == aprev + 1) {
                     const double nr = er * cp - ei * sp;
                     ei = er * sp + ei * cp;
@@ -451,8 +448,7 @@ __global__ __launch_bounds__(256, KIND == 0 ? 3 : 2) void multipole_kernel(Multi
 #pragma unroll
     for (int k = 0; k < kMoments; ++k) {
         if (k >= nk) break;
-        const double norm =
-            kind == 1 ? sqrt(2.0 * kPi) : 4.0 * kPi * ylm_norm(ls[k], ms[k] < 0 ? -ms[k] : ms[k]);
+        const double norm = kind == 1 ? sqrt(2.0 * kPi) : a.ynorm[2 * (k0 + k) + 1];
         const size_t v = ((size_t)sp * a.ns + s) * a.nmo + k0 + k;
         a.amp[v * a.NFp + t] = make_float2((float)(norm * ar[k]), (float)(norm * ai[k]));
     }
@@ -469,6 +465,7 @@ hipError_t launch_multipole(const MultipoleArgs &m, hipStream_t stream) {
     k.qinfo = m.qinfo;
     k.base = m.base;
     k.moments = m.moments;
+    k.ynorm = m.ynorm;
     k.nmo = m.nmo;
     k.ns = m.ns;
     k.mblocks = (m.nmo + kMoments - 1) / kMoments;

@@ -66,6 +66,24 @@ def shell_errors(f, ref):
     return d.max(axis=1), d.mean(axis=1)
 
 
+def judge_sampling_error(xyz, b, qlen, u, device=0):
+    """The orientational sampling error of the many-orientation average
+    itself, by a split half: the two halves of the orientation set give
+    averages F_a, F_b whose difference has variance 4 / NM of the
+    per-orientation spread, so |F_a - F_b| / 2 estimates the error of the full
+    NM-orientation average.  Per shell: the mean over lags of that relative
+    to |F(q, 0)|."""
+    u = np.asarray(u, np.float64)
+    h = u.shape[0] // 2
+    with ScatterDevice("all", device=device) as dev:
+        dev.stage(np.ascontiguousarray(xyz), layout="soa")
+        dev.set_factors(b)
+        fa, _, _ = dev.compute_oriented(qlen, u[:h])
+        fb, _, _ = dev.compute_oriented(qlen, u[h:2 * h])
+    f = 0.5 * (fa + fb)
+    return np.mean(np.abs(fa - fb) / 2.0 / np.abs(f[:, :1]), axis=1)
+
+
 def run(xyz, b, qlen, u, ntrain_frames, n_small=16, tol=1e-3, steps=3000, batch=4096, lr=2e-3, seed=1,
         device=0, progress=None):
     """The full pipeline on one trajectory (xyz [3][NA][NF] float32).
@@ -147,10 +165,22 @@ def run(xyz, b, qlen, u, ntrain_frames, n_small=16, tol=1e-3, steps=3000, batch=
 
     bias = float(np.mean((f_net.real - f_many_ev.real) / np.abs(f_many_ev[:, :1].real)))
     cv_time = tm["small_eval"] + tm["predict"] + tm["fallback"]
+    # How much of the 16-orientation error persists from one training half to
+    # the other: per shell, the correlation over lags of Re D_A and Re D_B (a
+    # crystal's anisotropy: near 1; a diffusing gas: near 0 beyond the box
+    # form factor), and the part of D_B that D_A explains.
+    persistence = np.zeros(len(qlen))
+    explained = np.zeros(len(qlen))
+    for s_ in range(len(qlen)):
+        da, db = d_a[s_].real, d_b[s_].real
+        if np.std(da) > 0 and np.std(db) > 0:
+            persistence[s_] = float(np.corrcoef(da, db)[0, 1])
+        explained[s_] = 1.0 - float(np.sum((db - da) ** 2) / max(np.sum(db ** 2), 1e-300))
     return {
         "f_cv": f_cv, "f_net": f_net, "f_small": f_small_ev, "f_many": f_many_ev, "use_cv": use_cv,
         "validation_error": e_val, "train_loss": loss, "timing_s": tm,
         "error_small": summary(f_small_ev), "error_net": summary(f_net), "error_cv": summary(f_cv),
         "bias_net": bias, "cv_shells": int(np.count_nonzero(use_cv)),
+        "persistence": persistence, "explained": explained,
         "speedup_eval": tm["many_eval"] / cv_time if cv_time > 0 else None,
     }

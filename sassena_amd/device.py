@@ -32,7 +32,7 @@ class ScatterDevice:
     """One staged trajectory block on one GPU (not thread-safe)."""
 
     def __init__(self, scattering="all", dsp="autocorrelate", method="fftw", device=0, profile=False,
-                 vector_block=0, atom_split=0, tile_shape=0, fft_halves=0):
+                 vector_block=0, atom_split=0, tile_shape=0, fft_halves=0, multipole_terms=0):
         L = _lib.load()
         self._L = L
         cfg = _lib.sx_config()
@@ -45,6 +45,7 @@ class ScatterDevice:
         cfg.atom_split = atom_split
         cfg.tile_shape = tile_shape
         cfg.fft_halves = fft_halves
+        cfg.multipole_terms = multipole_terms
         h = ctypes.c_void_p()
         rc = L.sx_create(ctypes.byref(cfg), ctypes.byref(h))
         if rc != _lib.SX_OK:

@@ -10,7 +10,7 @@ import numpy as np
 import pytest
 
 sys.path.insert(0, os.path.dirname(__file__))
-from util import shell_error, small_trajectory, unit_vectors  # noqa: E402
+from util import fq2_bound, shell_error, small_trajectory, unit_vectors  # noqa: E402
 
 pytestmark = pytest.mark.gpu
 
@@ -25,13 +25,13 @@ def sx():
 
 
 def check_close(fqt, fq, fq2, rfqt, rfq, rfq2, tol=TOL):
-    """One shell: fqt by the per-shell error, fq against |F_ref(0)|, fq2
-    relative to itself (with a floor of tol * 1e-3 |F_ref(0)|^2)."""
+    """One shell: fqt by the per-shell error, fq against |F_ref(0)|, fq2 by
+    the bound the fq bound implies (util.fq2_bound)."""
     scale = abs(rfqt[0])
     err = shell_error(fqt, rfqt)[0]
     assert err < tol, f"fqt shell error {err:.3e}"
     assert abs(fq - rfq) <= tol * scale, (fq, rfq)
-    assert abs(fq2 - rfq2) <= tol * (abs(rfq2) + 1e-3 * scale ** 2), (fq2, rfq2)
+    assert abs(fq2 - rfq2) <= fq2_bound(rfq2, scale, tol), (fq2, rfq2)
 
 
 @pytest.mark.parametrize("path", GOLDEN, ids=[os.path.basename(p) for p in GOLDEN])

@@ -193,6 +193,30 @@ def test_special_functions_match_scipy(oracle_mod):
                 assert abs(oracle_mod.ylm(l, m, th, ph) - sp.sph_harm_y(l, m, th, ph)) < 1e-13
 
 
+def test_spherical_bessel_at_high_degree(oracle_mod):
+    """j_l up to l = 200 (the largest accepted moment degree) against scipy,
+    below and above the turning point x = l."""
+    import scipy.special as sp
+    for x in [0.5, 7.0, 30.0, 70.3, 150.0, 250.0]:
+        got = oracle_mod.sph_bessel_all(200, x)
+        ref = sp.spherical_jn(np.arange(201), x)
+        mask = np.abs(ref) > 1e-250
+        assert np.all(np.isfinite(got))
+        assert np.allclose(got[mask], ref[mask], rtol=1e-11, atol=0)
+
+
+def test_spherical_harmonics_at_high_degree(oracle_mod):
+    """Y_lm up to the largest accepted degree (l = 200), where unnormalised
+    associated Legendre values or (l+m)!/(l-m)! leave the double range:
+    against scipy's sph_harm_y."""
+    import scipy.special as sp
+    for l, m in [(60, 5), (90, 88), (120, -100), (150, 151 - 1), (200, 3), (200, -199), (200, 200)]:
+        for th, ph in [(0.3, 1.0), (1.2, 2.5), (2.9, 5.0)]:
+            ref = sp.sph_harm_y(l, m, th, ph)
+            got = oracle_mod.ylm(l, m, th, ph)
+            assert abs(got - ref) <= 1e-10 * max(abs(ref), 1e-300) + 1e-300, (l, m, th, got, ref)
+
+
 def test_sphere_multipole_reaches_debye_limit(oracle_mod):
     """(1/4 pi) sum_lm |A_lm|^2 of a static cluster tends to the Debye sum."""
     rng = np.random.default_rng(3)

@@ -22,3 +22,12 @@ def unit_vectors(n, seed):
     rng = np.random.default_rng(seed)
     u = rng.normal(size=(n, 3))
     return u / np.linalg.norm(u, axis=1)[:, None]
+
+
+def fq2_bound(rfq2, f0, tol=1e-5):
+    """Allowed |fq2 - fq2_ref|.  fq2 is the mean over series of |a|^2 with
+    a = <C(tau)>_tau; an error of at most tol |F(0)| in each a (the bound on
+    fq) moves |a|^2 by at most 2 |a| tol |F(0)| + (tol |F(0)|)^2, i.e. fq2 by
+    2 tol |F(0)| sqrt(fq2) + (tol F(0))^2; plus tol relative to fq2 itself."""
+    f0 = abs(f0)
+    return tol * abs(rfq2) + 2.0 * tol * f0 * abs(rfq2) ** 0.5 + (tol * f0) ** 2
