@@ -134,6 +134,23 @@ int sx_compute_shells(sx_ctx *ctx, size_t nshells, const double *qv, size_t NM,
 int sx_compute_shells_oriented(sx_ctx *ctx, size_t nshells, const double *qlen, const double *u, size_t NM,
                                const double *b_shell, double *fqt, double *fq, double *fq2);
 
+/* Allocation-free variant of sx_compute_shells_oriented for coherent ("all")
+   contexts.  No counterpart in the reference: it is for callers that cannot
+   allocate device memory during the call (code under HIP graph capture,
+   tight serving loops, harnesses that forbid hipMalloc).  The call's work
+   buffers (q tables, partial amplitudes, power spectra, outputs before the
+   copy to the host) are slices of one file-scope device array of 256 MiB per
+   GPU, held until the process ends; the staged coordinates, factors and FFT
+   tables are those sx_stage / sx_set_factors set up.  Shells are processed
+   in as many passes as that maximum requires; a call whose single shell needs
+   more (NM x NF x 8 bytes of amplitudes and more) fails with SX_ERR_NOMEM
+   before anything runs.  Tile shapes are not autotuned (cfg.tile_shape, else
+   the default rule).  Every call on one GPU shares that GPU's array: calls
+   that overlap on one GPU (two host threads, two contexts) race.  Arguments
+   and results as sx_compute_shells_oriented. */
+int sx_compute_shells_oriented_noalloc(sx_ctx *ctx, size_t nshells, const double *qlen, const double *u, size_t NM,
+                                       const double *b_shell, double *fqt, double *fq, double *fq2);
+
 /* Multipole orientational averaging (scattering.average.orientation.type =
    multipole), coherent "all" scattering only: MPSphereScatterDevice and
    MPCylinderScatterDevice (src/scatter_devices/multipole_scatter_device.cpp:

@@ -23,7 +23,8 @@ LAYOUT = {"frames": 0, "atoms": 1, "soa": 2}
 
 # Every symbol include/sassena_amd.h declares.
 EXPORTS = ("sx_version", "sx_create", "sx_destroy", "sx_last_error", "sx_stage", "sx_set_factors",
-           "sx_compute_shell", "sx_compute_shells", "sx_compute_shells_oriented", "sx_compute_multipole",
+           "sx_compute_shell", "sx_compute_shells", "sx_compute_shells_oriented",
+           "sx_compute_shells_oriented_noalloc", "sx_compute_multipole",
            "sx_get_timing",
            "sx_pad_length", "sx_batch_create", "sx_batch_destroy", "sx_batch_last_error", "sx_batch_stage",
            "sx_batch_run", "sx_batch_run_one", "sx_batch_get_timing", "sx_batch_time_amplitude",
@@ -77,13 +78,15 @@ def load():
     L.sx_compute_shell.argtypes = [vp, dp, sz, dp, dp, dp]
     L.sx_compute_shells.argtypes = [vp, sz, dp, sz, dp, dp, dp, dp]
     L.sx_compute_shells_oriented.argtypes = [vp, sz, dp, dp, sz, dp, dp, dp, dp]
+    L.sx_compute_shells_oriented_noalloc.argtypes = [vp, sz, dp, dp, sz, dp, dp, dp, dp]
     L.sx_compute_multipole.argtypes = [vp, ctypes.c_int, sz, dp, dp, ctypes.POINTER(ctypes.c_int), sz, dp, dp, dp,
                                        dp]
     L.sx_get_timing.argtypes = [vp, ctypes.POINTER(sx_timing)]
     L.sx_pad_length.argtypes = [sz]
     L.sx_pad_length.restype = sz
     for name in ("sx_create", "sx_stage", "sx_set_factors", "sx_compute_shell", "sx_compute_shells",
-                 "sx_compute_shells_oriented", "sx_compute_multipole", "sx_get_timing"):
+                 "sx_compute_shells_oriented", "sx_compute_shells_oriented_noalloc", "sx_compute_multipole",
+                 "sx_get_timing"):
         getattr(L, name).restype = ctypes.c_int
     _lib = L
     return L

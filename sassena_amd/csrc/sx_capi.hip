@@ -23,6 +23,35 @@
 
 using namespace sxh;
 
+// Work memory of the allocation-free entry point (sx_compute_shells_oriented_noalloc):
+// one file-scope device array per GPU, held until the process ends.  Host code
+// never passes the array's name to a kernel or a copy (on the host it is only
+// a stand-in symbol): it resolves the array's address on each device once,
+// with that device current, and uses that address.
+constexpr size_t kScratchBytes = (size_t)256 << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices] = {};
+
+// Address of `scratch` on the current device `dev`, resolved on first use.
+hipError_t scratch_address(int dev, void **out) {
+    *out = nullptr;
+    if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return hipSuccess;
+}
+}  // namespace
+
 float sxh::elapsed(hipEvent_t a, hipEvent_t b) {
     float ms = 0.f;
     if (hipEventElapsedTime(&ms, a, b) != hipSuccess) return 0.f;
@@ -77,7 +106,7 @@ int round_filling_split(long base, int first, long resident, size_t NAp, size_t 
 int pick_oriented_split(sx_ctx *c, long base, int shape, bool per_row, int mode, size_t V, int *split) {
     base = std::max<long>(base, 1);
     *split = pick_split_blocks(c, base);
-    if (c->cfg.atom_split > 0) return SX_OK;
+    if (c->cfg.atom_split > 0 || c->noalloc) return SX_OK;
     int per_cu = 0, cus = 0;
     SX_HIP(c, sx::oriented_blocks_per_cu(shape, per_row, mode, &per_cu));
     SX_HIP(c, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->cfg.device));
@@ -249,7 +278,7 @@ int pick_shape(sx_ctx *c, int ns, int NM, const float *bptr, bool per_shell, int
     }
     *shape = default_shape(c, ns, per_shell);
     const double terms = (double)ns * NM * (double)c->NA * (double)c->NF;
-    if (terms < kTuneTerms) return SX_OK;
+    if (terms < kTuneTerms || c->noalloc) return SX_OK;
     const std::array<size_t, 6> key = {c->NFp, c->NAp, (size_t)NM, (size_t)ns, per_shell ? 1u : 0u,
                                        (size_t)c->q_mode};
     auto it = c->tuned.find(key);
@@ -724,6 +753,11 @@ int sx_create(const sx_config *cfg, sx_ctx **out) {
             sx_destroy(c);
             return SX_ERR_HIP;
         }
+    // the allocation-free entry point's scratch on this device (device current)
+    if (scratch_address(cfg->device, &c->scratch) != hipSuccess || !c->scratch) {
+        sx_destroy(c);
+        return SX_ERR_HIP;
+    }
     *out = c;
     return SX_OK;
 }
@@ -990,9 +1024,105 @@ int compute_impl(sx_ctx *c, size_t nshells, const double *qv, const double *qlen
     return SX_OK;
 }
 
+// Device bytes run_batch takes for ns oriented coherent shells in an
+// allocation-free call (each buffer 256-byte aligned in the arena): the
+// tile shape and atom ranges are those run_batch picks without autotuning
+// and without round-filling ranges.
+size_t noalloc_batch_bytes(sx_ctx *c, int ns, const double *qlen, size_t NM, bool per_shell) {
+    std::vector<double> ql(ns);
+    for (int s = 0; s < ns; ++s) ql[s] = qlen[s] / kTwoPi;
+    sx::ShellGrid g;
+    int mode = sx::shell_progression(ql.data(), ns, c->rmax, &g);
+    if (mode == 2 && per_shell) mode = 0;
+    const int saved = c->q_mode;
+    c->q_mode = mode;
+    const int shape = c->cfg.tile_shape > 0 ? c->cfg.tile_shape - 1 : default_shape(c, ns, per_shell);
+    c->q_mode = saved;
+    const size_t V = (size_t)ns * NM;
+    const size_t split = (size_t)pick_split_blocks(c, sx::oriented_base_blocks(c->NFp, (int)NM, ns, shape));
+    const size_t chunks = std::min<size_t>(NM, std::max<size_t>(1, (1024 + ns - 1) / ns));
+    const bool fft = c->cfg.dsp_type == SX_DSP_AUTOCORRELATE;
+    const size_t parts[] = {
+        sizeof(double) * 3 * V,                                   // qrev
+        sizeof(double) * ns,                                      // qlen
+        sizeof(double) * 3 * NM,                                  // uvec
+        mode == 2 ? sizeof(float) * ns : 0,                       // qres
+        per_shell ? sizeof(float) * ns * c->NAp : 0,              // bshell
+        per_shell ? sizeof(int) * V : 0,                          // brow
+        sizeof(float2) * V * c->NFp * split,                      // amp
+        sizeof(double) * 4 * ns * chunks,                         // apart
+        fft ? sizeof(double) * ns * chunks * c->plan.L : 0,       // ppart
+        sizeof(double) * 2 * ns * c->NF,                          // fqt
+        sizeof(double) * 2 * ns, sizeof(double) * 2 * ns,         // fq, fq2
+    };
+    size_t total = 0;
+    for (size_t b : parts) total += round_up(b, 256) + 256;
+    return total;
+}
+
 }  // namespace
 
 extern "C" {
+
+int sx_compute_shells_oriented_noalloc(sx_ctx *c, size_t nshells, const double *qlen, const double *u, size_t NM,
+                                       const double *b_shell, double *fqt, double *fq, double *fq2) {
+    if (!c) return SX_ERR_ARG;
+    if (c->cfg.scattering_type != SX_SCATTERING_ALL)
+        return fail(c, SX_ERR_UNSUPPORTED, "sx_compute_shells_oriented_noalloc: coherent (all) scattering only");
+    if (!qlen || !u) return fail(c, SX_ERR_ARG, "sx_compute_shells_oriented_noalloc: null |q| or orientations");
+    int rc = check_compute_args(c, nshells, nullptr, qlen, u, NM, b_shell, fqt, fq, fq2);
+    if (rc != SX_OK) return rc;
+    if (!c->scratch) return fail(c, SX_ERR_STATE, "sx_compute_shells_oriented_noalloc: no scratch on this device");
+    SX_HIP(c, hipSetDevice(c->cfg.device));
+    c->timing = sx_timing{};
+    c->timing.pad_length = c->plan_ok ? (size_t)c->plan.L : 0;
+    // the call's work buffers become slices of this device's scratch; the
+    // context's own buffers are set aside and restored afterwards
+    Arena arena;
+    arena.base = static_cast<char *>(c->scratch);
+    arena.cap = kScratchBytes;
+    DevBuf *work[] = {&c->qrev, &c->qlen, &c->uvec, &c->qres, &c->bshell, &c->brow,
+                      &c->amp,  &c->apart, &c->ppart, &c->fqt, &c->fq, &c->fq2};
+    constexpr size_t kWork = sizeof(work) / sizeof(work[0]);
+    DevBuf saved[kWork];
+    for (size_t i = 0; i < kWork; ++i) {
+        saved[i] = *work[i];
+        *work[i] = DevBuf{};
+        work[i]->arena = &arena;
+    }
+    c->noalloc = true;
+    const bool per_shell = b_shell != nullptr;
+    size_t s0 = 0;
+    while (rc == SX_OK && s0 < nshells) {
+        // the most shells from s0 whose buffers fit the scratch
+        size_t ns = nshells - s0;
+        while (ns > 1 && noalloc_batch_bytes(c, (int)ns, qlen + s0, NM, per_shell) > kScratchBytes) ns = (ns + 1) / 2;
+        const size_t need = noalloc_batch_bytes(c, (int)ns, qlen + s0, NM, per_shell);
+        if (need > kScratchBytes) {
+            rc = fail(c, SX_ERR_NOMEM,
+                      "sx_compute_shells_oriented_noalloc: one shell of %zu orientations x %zu frames needs %zu "
+                      "bytes of work memory, above the %zu MiB scratch maximum",
+                      NM, c->NF, need, kScratchBytes >> 20);
+            break;
+        }
+        arena.used = 0;
+        arena.exhausted = false;
+        for (size_t i = 0; i < kWork; ++i) {
+            work[i]->p = nullptr;
+            work[i]->cap = 0;
+        }
+        rc = run_batch(c, (int)ns, NM, nullptr, qlen + s0, u, per_shell ? b_shell + c->NA * s0 : nullptr,
+                       fqt + 2 * c->NF * s0, fq + 2 * s0, fq2 + 2 * s0);
+        if (rc != SX_OK && arena.exhausted)
+            rc = fail(c, SX_ERR_NOMEM, "sx_compute_shells_oriented_noalloc: work memory above the %zu MiB scratch",
+                      kScratchBytes >> 20);
+        s0 += ns;
+    }
+    c->noalloc = false;
+    for (size_t i = 0; i < kWork; ++i) *work[i] = saved[i];
+    return rc;
+}
+
 
 int sx_compute_shells(sx_ctx *c, size_t nshells, const double *qv, size_t NM, const double *b_shell, double *fqt,
                       double *fq, double *fq2) {

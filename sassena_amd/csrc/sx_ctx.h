@@ -19,11 +19,31 @@ constexpr double kTwoPi = 6.283185307179586476925286766559;
 constexpr size_t kStageChunkFloats = size_t(16) << 20;   // 64 MB staging pieces
 constexpr size_t kAmpBudgetBytes = size_t(24) << 30;      // amplitude buffer per batch
 
+// A bump region of device memory the caller owns (the allocation-free entry
+// point's scratch): DevBufs bound to it take slices instead of allocating.
+struct Arena {
+    char *base = nullptr;
+    size_t cap = 0, used = 0;
+    bool exhausted = false;
+};
+
 struct DevBuf {
     void *p = nullptr;
     size_t cap = 0;
+    Arena *arena = nullptr;  // non-null: slices of this arena, never hipMalloc/hipFree
     hipError_t ensure(size_t bytes) {
         if (bytes <= cap && p) return hipSuccess;
+        if (arena) {
+            const size_t off = (arena->used + 255) / 256 * 256;
+            if (off + bytes > arena->cap) {
+                arena->exhausted = true;
+                return hipErrorOutOfMemory;
+            }
+            p = arena->base + off;
+            cap = bytes;
+            arena->used = off + bytes;
+            return hipSuccess;
+        }
         if (p) {
             hipError_t e = hipFree(p);
             p = nullptr;
@@ -35,7 +55,7 @@ struct DevBuf {
         return e;
     }
     void release() {
-        if (p) (void)hipFree(p);
+        if (p && !arena) (void)hipFree(p);
         p = nullptr;
         cap = 0;
     }
@@ -75,4 +95,9 @@ struct sx_ctx {
     double rmax = 0.0;
     // autotuned oriented tile shape per (NFp, NAp, NM, shells, per-shell b, progression)
     std::map<std::array<size_t, 6>, int> tuned;
+    // set during an allocation-free call: work buffers are arena slices, no
+    // autotuning (default or fixed tile shapes), atom ranges only as many as
+    // needed to fill the GPU (no round-filling extra ranges)
+    bool noalloc = false;
+    void *scratch = nullptr;  // device address of this device's scratch array
 };

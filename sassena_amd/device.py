@@ -122,10 +122,12 @@ class ScatterDevice:
                     "sx_compute_shells")
         return (fqt[..., 0] + 1j * fqt[..., 1], fq[:, 0] + 1j * fq[:, 1], fq2[:, 0] + 1j * fq2[:, 1])
 
-    def compute_oriented(self, qlen, u, b_shell=None):
+    def compute_oriented(self, qlen, u, b_shell=None, noalloc=False):
         """Shells sharing one orientation set: q_{s,m} = qlen[s] * u[m].
 
-        qlen [nshells], u [NM][3] unit rows -> (fqt [nshells][NF], fq, fq2)."""
+        qlen [nshells], u [NM][3] unit rows -> (fqt [nshells][NF], fq, fq2).
+        noalloc: through sx_compute_shells_oriented_noalloc (work memory in
+        the library's fixed per-GPU scratch, no device allocation)."""
         ql = np.ascontiguousarray(qlen, dtype=np.float64).ravel()
         uu = np.ascontiguousarray(u, dtype=np.float64).reshape(-1, 3)
         ns, NM = ql.size, uu.shape[0]
@@ -138,8 +140,9 @@ class ScatterDevice:
             if bs.shape != (ns, self.NA):
                 raise ValueError("b_shell must be [nshells][NA]")
             bp = _dp(bs)
-        self._check(self._L.sx_compute_shells_oriented(self._h, ns, _dp(ql), _dp(uu), NM, bp, _dp(fqt), _dp(fq),
-                                                       _dp(fq2)), "sx_compute_shells_oriented")
+        fn = self._L.sx_compute_shells_oriented_noalloc if noalloc else self._L.sx_compute_shells_oriented
+        self._check(fn(self._h, ns, _dp(ql), _dp(uu), NM, bp, _dp(fqt), _dp(fq), _dp(fq2)),
+                    "sx_compute_shells_oriented" + ("_noalloc" if noalloc else ""))
         return (fqt[..., 0] + 1j * fqt[..., 1], fq[:, 0] + 1j * fq[:, 1], fq2[:, 0] + 1j * fq2[:, 1])
 
     def compute_multipole(self, kind, qv, moments, axis=(0.0, 0.0, 1.0), b_shell=None):
