@@ -53,9 +53,10 @@ AMP_TRAFFIC_BYTES = 1.97e9 + 0.524e9
 
 
 # FP64 flops per (atom, frame, moment, shell) term of the multipole kernels
-# on the mp / mpc workloads, from rocprofv3 instruction counts (FMA counted
-# as 2 flops); None until measured.
-MP_FP64_PER_TERM = {"sphere": None, "cylinder": None}
+# on the mp / mpc workloads: rocprofv3 counts of FP64 FMA (2 flops), MUL and
+# ADD wave instructions x 64 lanes over the terms of one launch (sphere:
+# 2.94e12 / 4.84e10, cylinder: 2.12e12 / 1.64e10).
+MP_FP64_PER_TERM = {"sphere": 60.7, "cylinder": 129.3}
 
 
 def roofline_legs(terms_per_launch, launch_s):
@@ -305,8 +306,9 @@ def bench_c4(args):
     bt.close()
     terms = tm["terms"]
     amp_s = amp_ms * 1e-3
-    stepped = all(len(T.qlen) > 1 and np.allclose(np.diff(T.qlen), (T.qlen[-1] - T.qlen[0]) / (len(T.qlen) - 1),
-                                                   rtol=0, atol=1e-12 * np.max(np.abs(T.qlen))) for T in trajs)
+    legs = roofline_legs(terms, amp_s)
+    ex_flops = (EXEC_FLOPS_STEPPED * tm["terms_stepped"] + EXEC_FLOPS_RESIDUAL * tm["terms_residual"]
+                + ALG_FP32_FLOPS * (terms - tm["terms_stepped"] - tm["terms_residual"]))
     out = {
         "metric": "atom*frame*q-vector terms/s (64 uneven coherent trajectories, one graph launch)",
         "value": terms * args.steps / wall,
@@ -317,20 +319,18 @@ def bench_c4(args):
         "dtype": "f32 sincos / f64 phase", "data": "synthetic",
         "config": {"workload": "64 uneven coherent trajectories (batched graph, persistent amplitude kernel)",
                    "trajectories": len(trajs), "terms_per_step": terms, "tiles": tm["tiles"],
-                   "tile_shape": tm["tile_shape"], "scale": args.c4_scale, "generation_s": gen_s},
+                   "tile_shape": tm["tile_shape"], "scale": args.c4_scale, "generation_s": gen_s,
+                   "terms_stepped_exact": tm["terms_stepped"], "terms_stepped_residual": tm["terms_residual"]},
         "graph_ms": tm["total_ms"],
         "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
                        "min": float(np.min(lat)), "max": float(np.max(lat))},
-        "roofline": ({"bound": "valu", "achieved": FLOPS_PER_TERM * terms / amp_s / 1e12,
-                      "peak": FP32_PEAK / 1e12, "unit": "TFLOP/s",
-                      "frac": FLOPS_PER_TERM * terms / amp_s / FP32_PEAK, "traffic": None,
-                      "kernel": "amp_persistent_kernel",
-                      "note": "every trajectory's shells are linearly spaced: three-term recurrence across "
-                              "shells, 8 FP32 flops per term"}
-                     if stepped else
-                     {"bound": "valu", "achieved": 2.0 * terms / amp_s / 1e12, "peak": TRANS_PEAK / 1e12,
-                      "unit": "Tsincos-ops/s", "frac": 2.0 * terms / amp_s / TRANS_PEAK, "traffic": None,
-                      "kernel": "amp_persistent_kernel"}),
+        "roofline": {"bound": "valu", "achieved": legs["fp64"]["achieved"], "peak": legs["fp64"]["peak"],
+                     "unit": "TFLOP/s", "frac": legs["fp64"]["frac"], "traffic": None,
+                     "kernel": "amp_persistent_kernel", "legs": legs,
+                     "executed": {"flops_per_term": ex_flops / terms, "achieved": ex_flops / amp_s / 1e12,
+                                  "unit": "TFLOP/s", "frac_of_unpacked_fp32": ex_flops / amp_s / (FP32_PEAK / 2),
+                                  "note": "8 FP32 flops per term of exactly stepped shells, 12 with residual "
+                                          "corrections, 4 (plus sin/cos) on the direct path"}},
     }
     print(json.dumps(out), flush=True)
 

@@ -202,14 +202,19 @@ typedef struct sx_batch_timing {
     double amplitude_ms;  /* persistent amplitude kernel (last sx_batch_time_amplitude) */
     size_t terms;         /* sum over trajectories of shells*NM*NA*NF */
     size_t tiles;         /* amplitude work items in the queue */
-    int tile_shape;       /* oriented tile shape used (0..4) */
+    int tile_shape;       /* oriented tile shape used: 0..4 the fixed shapes of cfg.tile_shape 1..5,
+                             5 mixed (1x8, 1x16 and 1x20 tiles chosen per trajectory) */
+    size_t terms_stepped;  /* terms of trajectories whose shells step phasors (exact progression) */
+    size_t terms_residual; /* terms of trajectories stepped with residual corrections */
 } sx_batch_timing;
 
 int sx_batch_create(const sx_config *cfg, sx_batch **out);
 void sx_batch_destroy(sx_batch *batch);
 const char *sx_batch_last_error(const sx_batch *batch);
-/* Stages every trajectory, autotunes the tile shape (cfg.tile_shape == 0)
-   and captures the batch graph.  The trajectory array (and its output
+/* Stages every trajectory, plans the amplitude tiles and captures the batch
+   graph.  cfg.tile_shape 1..5 runs every trajectory with that fixed shape;
+   0 covers each trajectory's shells by the cheapest mix of 1x8, 1x16 and
+   1x20 tiles (one persistent kernel runs all three).  The trajectory array (and its output
    buffers) must stay valid until the last sx_batch_run* call. */
 int sx_batch_stage(sx_batch *batch, const sx_trajectory *trajs, size_t ntraj);
 /* One launch of the whole batch; outputs are copied to every trajectory. */

@@ -215,7 +215,7 @@ __global__ __launch_bounds__(256, 2) void amp_oriented_kernel(OrientKernelArgs a
     const int sp = rest % a.split;
     const int ft = rest / a.split;
 
-    oriented_tile<VM, VS, PER_ROW_B>(a, ft, sp, og, sg);
+    oriented_tile<VM, VS, PER_ROW_B>(a, ft, sp, og, sg * VS);
 }
 
 // Shells that are an arithmetic progression up to small residuals (mode 2 of
@@ -234,7 +234,7 @@ __global__ __launch_bounds__(256, 2) void amp_oriented_res_kernel(OrientKernelAr
     rest /= a.ogroups;
     const int sp = rest % a.split;
     const int ft = rest / a.split;
-    oriented_tile_body<VM, VS, false, 2, true>(a, ft, sp, og, sg, tot_lds);
+    oriented_tile_body<VM, VS, false, 2, true>(a, ft, sp, og, sg * VS, tot_lds);
 }
 
 // Oriented tile shapes (orientations x shells per wavefront), picked by the

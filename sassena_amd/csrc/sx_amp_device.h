@@ -35,7 +35,7 @@ OrientKernelArgs make_orient_args(const OrientArgs &a);
 
 // One tile of the oriented amplitude sum: frame tile ft (one frame per
 // lane), atom range sp, orientations 4*VM*og.. (VM per wavefront) and
-// shells VS*sg..; 256 threads.  MODE 1: the shells of the call are in
+// shells sbase..sbase+VS-1; 256 threads.  MODE 1: the shells of the call are in
 // arithmetic progression (step a.qstep), and the tile's shells follow from its
 // middle one by the three-term recurrence.  MODE 2: as 1 on the ideal grid
 // q0 + s qstep, with each shell's residual r_s = q_s - (q0 + s qstep) (the
@@ -52,8 +52,21 @@ OrientKernelArgs make_orient_args(const OrientArgs &a);
 // float data format, no index stride.
 constexpr int kBufferDword3 = 0x00020000;
 
+// Wave-uniform, read-only kernel data (orientations, shells, residuals,
+// scattering factors) seen through the constant address space, so that it is
+// read by scalar loads into SGPRs.  A kernel that takes its pointers as
+// arguments gets this by itself; the persistent kernel reads its arguments
+// from memory, and through such generic pointers the compiler issues vector
+// flat loads that it cannot separate from the LDS totals, waits for every
+// load in flight after each one, and re-reads the factors inside the atom
+// loop.  The data must not be written while the kernel runs.
+template <typename T>
+__device__ __forceinline__ const __attribute__((address_space(4))) T *uniform_ptr(const T *p) {
+    return (const __attribute__((address_space(4))) T *)p;
+}
+
 template <int VM, int VS, bool PER_ROW_B, int MODE, bool BUF>
-__device__ __forceinline__ void oriented_tile_body(const OrientKernelArgs &a, int ft, int sp, int og, int sg,
+__device__ __forceinline__ void oriented_tile_body(const OrientKernelArgs &a, int ft, int sp, int og, int sbase,
                                                    double *__restrict__ tot) {
     constexpr bool AP = MODE >= 1;
     constexpr bool RES = MODE == 2;
@@ -63,29 +76,32 @@ __device__ __forceinline__ void oriented_tile_body(const OrientKernelArgs &a, in
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     const int mbase = (og * 4 + wave) * VM;
-    const int sbase = sg * VS;
     const size_t t = (size_t)ft * kWave + lane;
 
+    const auto U = uniform_ptr(a.u);
+    const auto QL = uniform_ptr(a.qlen);
+    const auto QR = uniform_ptr(a.qres);
+    const auto B = uniform_ptr(a.b);
     double ux[VM], uy[VM], uz[VM], ql[AP ? 1 : VS];
 #pragma unroll
     for (int i = 0; i < VM; ++i) {
         const int m = min(mbase + i, a.NM - 1);
-        ux[i] = a.u[3 * m];
-        uy[i] = a.u[3 * m + 1];
-        uz[i] = a.u[3 * m + 2];
+        ux[i] = U[3 * m];
+        uy[i] = U[3 * m + 1];
+        uz[i] = U[3 * m + 2];
     }
     float qr[RES ? VS : 1];
     if constexpr (RES) {
         // the tile's middle shell on the ideal grid, and the residuals
         ql[0] = a.q0 + (double)(sbase + VS / 2) * a.qstep;
 #pragma unroll
-        for (int k = 0; k < VS; ++k) qr[k] = a.qres[min(sbase + k, a.ns - 1)];
+        for (int k = 0; k < VS; ++k) qr[k] = QR[min(sbase + k, a.ns - 1)];
     } else if constexpr (AP) {
         // the tile's middle shell, which need not exist in a partial last tile
-        ql[0] = a.qlen[min(sbase, a.ns - 1)] + (double)(VS / 2) * a.qstep;
+        ql[0] = QL[min(sbase, a.ns - 1)] + (double)(VS / 2) * a.qstep;
     } else {
 #pragma unroll
-        for (int k = 0; k < VS; ++k) ql[k] = a.qlen[min(sbase + k, a.ns - 1)];
+        for (int k = 0; k < VS; ++k) ql[k] = QL[min(sbase + k, a.ns - 1)];
     }
 
     // FP64 totals: registers, or this wavefront's slice of LDS ([VM*VS][2][64])
@@ -145,11 +161,15 @@ __device__ __forceinline__ void oriented_tile_body(const OrientKernelArgs &a, in
                     zs[c] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rz, voff, so, 0));
                 }
             } else {
+                // global (not generic) pointers: loads that only count on
+                // vmcnt, so the batch is consumed as it arrives
+                typedef const __attribute__((address_space(1))) float *gptr;
+                const gptr gx = (gptr)X, gy = (gptr)Y, gz = (gptr)Z;
 #pragma unroll
                 for (int c = 0; c < CH; ++c) {
-                    xs[c] = X[(j + c) * a.NFp + lane];
-                    ys[c] = Y[(j + c) * a.NFp + lane];
-                    zs[c] = Z[(j + c) * a.NFp + lane];
+                    xs[c] = gx[(j + c) * a.NFp + lane];
+                    ys[c] = gy[(j + c) * a.NFp + lane];
+                    zs[c] = gz[(j + c) * a.NFp + lane];
                 }
             }
 #pragma unroll
@@ -158,7 +178,7 @@ __device__ __forceinline__ void oriented_tile_body(const OrientKernelArgs &a, in
                 float bk[VS];
 #pragma unroll
                 for (int k = 0; k < VS; ++k)
-                    bk[k] = PER_ROW_B ? a.b[(size_t)min(sbase + k, a.ns - 1) * a.NAp + j + c] : a.b[j + c];
+                    bk[k] = PER_ROW_B ? B[(size_t)min(sbase + k, a.ns - 1) * a.NAp + j + c] : B[j + c];
 #pragma unroll
                 for (int i = 0; i < VM; ++i) {
                     double d = ux[i] * xd;
@@ -290,11 +310,11 @@ __device__ __forceinline__ void oriented_tile_body(const OrientKernelArgs &a, in
 // Modes 0 and 1 (the mode is uniform over a launch or a trajectory); mode 2
 // has kernels of its own, since it needs `tot` in LDS.
 template <int VM, int VS, bool PER_ROW_B, bool BUF = true>
-__device__ __forceinline__ void oriented_tile(const OrientKernelArgs &a, int ft, int sp, int og, int sg) {
+__device__ __forceinline__ void oriented_tile(const OrientKernelArgs &a, int ft, int sp, int og, int sbase) {
     if (a.ap)
-        oriented_tile_body<VM, VS, PER_ROW_B, 1, BUF>(a, ft, sp, og, sg, nullptr);
+        oriented_tile_body<VM, VS, PER_ROW_B, 1, BUF>(a, ft, sp, og, sbase, nullptr);
     else
-        oriented_tile_body<VM, VS, PER_ROW_B, 0, BUF>(a, ft, sp, og, sg, nullptr);
+        oriented_tile_body<VM, VS, PER_ROW_B, 0, BUF>(a, ft, sp, og, sbase, nullptr);
 }
 
 }  // namespace sx

@@ -24,31 +24,49 @@
 
 namespace sx {
 
-// Trajectories whose shells step with residual corrections (mode 2) keep
-// their FP64 totals in the dynamic LDS, which the host sizes to
-// kResidualLds<VM, VS> when the batch holds any such trajectory; each
-// wavefront touches only its own slice, so tiles need no barrier between them.
-template <int VM, int VS>
+// RES = false: tiles of trajectories whose shells take one phase each or an
+// exact progression (modes 0 and 1); RES = true: tiles of trajectories
+// stepped with residual corrections (mode 2), whose FP64 totals live in the
+// dynamic LDS (kResidualLds<VM, VS>); each wavefront touches only its own
+// slice, so tiles need no barrier between them.  The two kinds run as two
+// persistent launches with a work queue each, so neither kernel carries the
+// other's body.
+// The LDS word through which thread 0 hands the next tile index to its
+// workgroup.  The residual kernel's dynamic LDS fills the CU at two
+// workgroups (80 KiB for 1 x 20), and a static word on top would leave room
+// for one, so there the index passes through the first word of wave 0's
+// slice of the totals: thread 0 writes it only after its own wave has
+// finished the previous tile, and the other waves never touch that slice.
+template <bool RES>
+__device__ inline int *tile_slot(double *dyn) {
+    if constexpr (RES) {
+        return reinterpret_cast<int *>(dyn);
+    } else {
+        __shared__ int cur;
+        return &cur;
+    }
+}
+
+template <int VM, int VS, bool RES>
 __global__ __launch_bounds__(256, 2) void amp_persistent_kernel(const OrientKernelArgs *__restrict__ targs,
                                                              const int4 *__restrict__ items,
                                                              const int *__restrict__ qstart,
                                                              unsigned *__restrict__ counters) {
     extern __shared__ double tot_lds[];
-    __shared__ int cur;
+    int *cur = tile_slot<RES>(tot_lds);
     const int x = blockIdx.x & 7;
     const int begin = qstart[x], n = qstart[x + 1] - begin;
     for (;;) {
-        if (threadIdx.x == 0) cur = (int)atomicAdd(&counters[x], 1u);
+        if (threadIdx.x == 0) *cur = (int)atomicAdd(&counters[x], 1u);
         __syncthreads();
-        const int it = cur;
+        const int it = *cur;
         __syncthreads();  // every thread has read cur before it is overwritten
         if (it >= n) break;
         const int4 w = items[begin + it];
-        const OrientKernelArgs &ta = targs[w.x];
-        if (ta.ap == 2)
-            oriented_tile_body<VM, VS, false, 2, false>(ta, w.y, w.w, w.z & 0xffff, w.z >> 16, tot_lds);
+        if constexpr (RES)
+            oriented_tile_body<VM, VS, false, 2, true>(targs[w.x], w.y, 0, w.z & 0xffff, w.z >> 16, tot_lds);
         else
-            oriented_tile<VM, VS, false, false>(ta, w.y, w.w, w.z & 0xffff, w.z >> 16);
+            oriented_tile<VM, VS, false, false>(targs[w.x], w.y, 0, w.z & 0xffff, w.z >> 16);
     }
 }
 
@@ -75,6 +93,13 @@ __global__ __launch_bounds__(256) void final_batch_kernel(const FinalArgs *__res
 using PersistentFn = void (*)(const OrientKernelArgs *, const int4 *, const int *, unsigned *);
 using CorrBatchFn = void (*)(const CorrArgs *, const int4 *);
 
+// Plans with mixed tiles: each trajectory's shells are covered by the
+// cheapest mix of 1 x 8, 1 x 16 and 1 x 20 tiles (fixed shapes 2, 3 and 4,
+// by mixed code 0, 1, 2), whose queues run one after the other.
+constexpr int kMixedShape = kOrientShapes;
+constexpr int kMixedVS[3] = {8, 16, 20};
+constexpr int kMixedFixed[3] = {2, 3, 4};
+
 static size_t persistent_res_lds(int shape) {
     switch (shape) {
         case 0: return kResidualLds<2, 8>;
@@ -85,14 +110,19 @@ static size_t persistent_res_lds(int shape) {
     }
 }
 
-static PersistentFn persistent_fn(int shape) {
+template <bool RES>
+static PersistentFn persistent_fn_of(int shape) {
     switch (shape) {
-        case 0: return amp_persistent_kernel<2, 8>;
-        case 1: return amp_persistent_kernel<4, 4>;
-        case 2: return amp_persistent_kernel<1, 8>;
-        case 3: return amp_persistent_kernel<1, 16>;
-        default: return amp_persistent_kernel<1, 20>;
+        case 0: return amp_persistent_kernel<2, 8, RES>;
+        case 1: return amp_persistent_kernel<4, 4, RES>;
+        case 2: return amp_persistent_kernel<1, 8, RES>;
+        case 3: return amp_persistent_kernel<1, 16, RES>;
+        default: return amp_persistent_kernel<1, 20, RES>;
     }
+}
+
+static PersistentFn persistent_fn(int shape, bool res) {
+    return res ? persistent_fn_of<true>(shape) : persistent_fn_of<false>(shape);
 }
 
 static CorrBatchFn corr_batch_fn(int E) {
@@ -123,11 +153,22 @@ struct TrajWork {
     int chunks = 1;
 };
 
+// One persistent amplitude launch: its tiles in 8 per-XCD runs.
+struct Queue {
+    DevBuf items, qstart, counters;
+    int nitems = 0, persistent_blocks = 0;
+    bool res = false;  // residual-corrected tiles
+    int shape = 0;     // fixed tile shape of the kernel
+};
+
 // Device-side launch plan of a set of trajectories.
 struct Plan {
     std::vector<int> ks;
-    DevBuf oka, items, qstart, counters, cargs, citems, fargs, fitems;
-    int nitems = 0, nfitems = 0, persistent_blocks = 0;
+    DevBuf oka, cargs, citems, fargs, fitems;
+    // persistent launches in order: tiles of modes 0 / 1, then of mode 2,
+    // each by tile shape (largest first)
+    std::vector<Queue> queues;
+    int nitems = 0, nfitems = 0;
     size_t final_lds = 0;
     struct CorrGroup {
         int E, T, first, count;
@@ -137,7 +178,9 @@ struct Plan {
     hipGraphExec_t exec = nullptr;
     ~Plan() {
         if (exec) (void)hipGraphExecDestroy(exec);
-        for (DevBuf *d : {&oka, &items, &qstart, &counters, &cargs, &citems, &fargs, &fitems}) d->release();
+        for (DevBuf *d : {&oka, &cargs, &citems, &fargs, &fitems}) d->release();
+        for (Queue &q : queues)
+            for (DevBuf *d : {&q.items, &q.qstart, &q.counters}) d->release();
     }
 };
 
@@ -150,8 +193,9 @@ struct sx_batch {
     std::vector<sx_trajectory> trajs;
     std::vector<TrajWork> work;
     int shape = 0;
-    int max_resident = 0;  // persistent blocks that fit the GPU at once
-    bool residuals = false;  // some trajectory steps its shells with residual corrections
+    // persistent workgroups that fit the GPU at once, per kernel (plain,
+    // residual) and tile shape
+    int max_resident[2][sx::kOrientShapes] = {};
     std::unique_ptr<Plan> all;
     std::vector<std::unique_ptr<Plan>> one;
     sx_batch_timing timing{};
@@ -199,62 +243,124 @@ sx::OrientKernelArgs orient_args(const TrajWork &w, const sx_trajectory &t, int 
     return sx::make_orient_args(oa);
 }
 
-// Builds the device work lists of trajectories ks for tile shape `shape`;
-// stride > 1 keeps every stride-th amplitude tile (autotuning samples), but
-// never fewer than about kMinSampleTiles tiles.
-constexpr size_t kMinSampleTiles = 2048;
+// Cost of one shell slot (one shell of a tile, for one orientation, atom and
+// frame) of the mixed plans' 1 x 8, 1 x 16 and 1 x 20 tiles relative to
+// 1 x 20, for exactly stepped tiles (modes 0 and 1) and residual-corrected
+// ones (mode 2); measured on an MI355X through the persistent kernels, four
+// trajectories of 20,000 atoms x 2,000 frames and 200 orientations with 8,
+// 16 and 20 shells (residual: 2.137, 1.482 and 1.378 ms per shell; exact:
+// 1.731, 1.193 and 1.073 ms).
+constexpr double kSlotCost[2][3] = {{1.614, 1.112, 1.0}, {1.551, 1.076, 1.0}};
 
-int build_plan(sx_batch *b, const std::vector<int> &ks, int shape, int stride, Plan &P) {
+struct ShellGroup {
+    int first, code;  // first shell; tile code (kMixedVS[code] shells)
+};
+
+// The cheapest cover of ns shells by mixed tiles, by dynamic programming
+// over the shells left: best[n] = min over codes g of kSlotCost[g] * VS_g +
+// best[max(0, n - VS_g)].  Groups run from shell 0; only the last one can be
+// partial.
+std::vector<ShellGroup> shell_groups(int ns, bool res) {
+    const double *c = kSlotCost[res ? 1 : 0];
+    std::vector<double> best(ns + 1, 0.0);
+    std::vector<int> pick(ns + 1, 0);
+    for (int n = 1; n <= ns; ++n) {
+        best[n] = 1e300;
+        for (int g = 0; g < 3; ++g) {
+            const double v = c[g] * sx::kMixedVS[g] + best[std::max(0, n - sx::kMixedVS[g])];
+            if (v < best[n]) {
+                best[n] = v;
+                pick[n] = g;
+            }
+        }
+    }
+    std::vector<ShellGroup> out;
+    int first = 0;
+    for (int n = ns; n > 0; n -= sx::kMixedVS[pick[n]]) {
+        out.push_back({first, pick[n]});
+        first += sx::kMixedVS[pick[n]];
+    }
+    return out;
+}
+
+// Builds the device work lists of trajectories ks for tile shape `shape`
+// (a fixed shape, or sx::kMixedShape).
+int build_plan(sx_batch *b, const std::vector<int> &ks, int shape, Plan &P) {
     P.ks = ks;
-    const int VM = sx::orient_shape_vm(shape), VS = sx::orient_shape_vs(shape);
+    const bool mixed = shape == sx::kMixedShape;
+    // mixed tiles have one orientation per wavefront, as 1 x 20
+    const int args_shape = mixed ? 4 : shape;
+    const int VS = sx::orient_shape_vs(args_shape);
     std::vector<sx::OrientKernelArgs> oka(b->trajs.size());
     struct Item {
         int4 w;
         long cost;
     };
-    std::vector<Item> items;
+    // [residual][kernel shape]
+    std::vector<Item> all_items[2][sx::kOrientShapes];
     for (int k : ks) {
         const sx_trajectory &t = b->trajs[k];
-        oka[k] = orient_args(b->work[k], t, shape);
-        const int ftiles = oka[k].ftiles, og = oka[k].ogroups, sg = oka[k].sgroups;
-        const long cost = (long)oka[k].atoms_per_split;
-        for (int ft = 0; ft < ftiles; ++ft)
-            for (int o = 0; o < og; ++o)
-                for (int s = 0; s < sg; ++s) items.push_back({make_int4(k, ft, o | (s << 16), 0), cost});
+        const bool res = b->work[k].mode == 2;
+        oka[k] = orient_args(b->work[k], t, args_shape);
+        const int ftiles = oka[k].ftiles, og = oka[k].ogroups;
+        std::vector<ShellGroup> groups;
+        if (mixed) {
+            groups = shell_groups((int)t.nshells, res);
+        } else {
+            for (int s = 0; s < oka[k].sgroups; ++s) groups.push_back({s * VS, 0});
+        }
+        // tile cost in atom shell slots (x 1000 in kSlotCost units)
+        for (size_t g = 0; g < groups.size(); ++g) {
+            const int code = groups[g].code;
+            const long cost = mixed ? (long)(oka[k].atoms_per_split * sx::kMixedVS[code] *
+                                             kSlotCost[res ? 1 : 0][code] * 1000.0)
+                                    : (long)oka[k].atoms_per_split;
+            std::vector<Item> &items = all_items[res ? 1 : 0][mixed ? sx::kMixedFixed[code] : shape];
+            for (int ft = 0; ft < ftiles; ++ft)
+                for (int o = 0; o < og; ++o) items.push_back({make_int4(k, ft, o | (groups[g].first << 16), 0), cost});
+        }
     }
-    (void)VM;
-    (void)VS;
-    // largest tiles first; within a trajectory, tiles sharing a frame tile stay adjacent
-    std::stable_sort(items.begin(), items.end(), [](const Item &a, const Item &c) { return a.cost > c.cost; });
-    if (stride > 1) stride = std::max(1, std::min(stride, (int)(items.size() / kMinSampleTiles)));
-    if (stride > 1) {
-        std::vector<Item> keep;
-        for (size_t i = 0; i < items.size(); i += stride) keep.push_back(items[i]);
-        items.swap(keep);
+    P.nitems = 0;
+    P.queues.clear();
+    static const int kLaunchOrder[sx::kOrientShapes] = {4, 3, 0, 1, 2};  // 1x20, 1x16, 2x8, 4x4, 1x8
+    for (int qi = 0; qi < 2; ++qi)
+        for (int sh : kLaunchOrder) {
+            if (all_items[qi][sh].empty()) continue;
+            P.queues.emplace_back();
+            P.queues.back().res = qi == 1;
+            P.queues.back().shape = sh;
+        }
+    for (Queue &Q : P.queues) {
+        const int qi = Q.res ? 1 : 0;
+        std::vector<Item> &items = all_items[qi][Q.shape];
+        // largest tiles first; within a trajectory, tiles sharing a frame tile stay adjacent
+        std::stable_sort(items.begin(), items.end(), [](const Item &a, const Item &c) { return a.cost > c.cost; });
+        // 8 contiguous runs of equal cost, one per XCD
+        long total = 0;
+        for (const Item &it : items) total += it.cost;
+        std::vector<int> qstart(kXcds + 1, (int)items.size());
+        qstart[0] = 0;
+        long acc = 0;
+        int q = 1;
+        for (size_t i = 0; i < items.size() && q < kXcds; ++i) {
+            acc += items[i].cost;
+            while (q < kXcds && acc * kXcds >= total * q) qstart[q++] = (int)i + 1;
+        }
+        std::vector<int4> w(items.size());
+        for (size_t i = 0; i < items.size(); ++i) w[i] = items[i].w;
+        Q.nitems = (int)items.size();
+        P.nitems += Q.nitems;
+        SXB_HIP(b, Q.items.ensure(sizeof(int4) * std::max<size_t>(1, w.size())));
+        if (!w.empty()) SXB_HIP(b, hipMemcpy(Q.items.p, w.data(), sizeof(int4) * w.size(), hipMemcpyHostToDevice));
+        SXB_HIP(b, Q.qstart.ensure(sizeof(int) * qstart.size()));
+        SXB_HIP(b, hipMemcpy(Q.qstart.p, qstart.data(), sizeof(int) * qstart.size(), hipMemcpyHostToDevice));
+        SXB_HIP(b, Q.counters.ensure(sizeof(unsigned) * kXcds));
+        Q.persistent_blocks =
+            std::max(kXcds, std::min(b->max_resident[qi][Q.shape], (Q.nitems + kXcds - 1) / kXcds * kXcds));
+        Q.persistent_blocks = (Q.persistent_blocks + kXcds - 1) / kXcds * kXcds;
     }
-    // 8 contiguous runs of equal cost, one per XCD
-    long total = 0;
-    for (const Item &it : items) total += it.cost;
-    std::vector<int> qstart(kXcds + 1, (int)items.size());
-    qstart[0] = 0;
-    long acc = 0;
-    int q = 1;
-    for (size_t i = 0; i < items.size() && q < kXcds; ++i) {
-        acc += items[i].cost;
-        while (q < kXcds && acc * kXcds >= total * q) qstart[q++] = (int)i + 1;
-    }
-    std::vector<int4> w(items.size());
-    for (size_t i = 0; i < items.size(); ++i) w[i] = items[i].w;
-    P.nitems = (int)items.size();
     SXB_HIP(b, P.oka.ensure(sizeof(sx::OrientKernelArgs) * oka.size()));
     SXB_HIP(b, hipMemcpy(P.oka.p, oka.data(), sizeof(sx::OrientKernelArgs) * oka.size(), hipMemcpyHostToDevice));
-    SXB_HIP(b, P.items.ensure(sizeof(int4) * std::max<size_t>(1, w.size())));
-    if (!w.empty()) SXB_HIP(b, hipMemcpy(P.items.p, w.data(), sizeof(int4) * w.size(), hipMemcpyHostToDevice));
-    SXB_HIP(b, P.qstart.ensure(sizeof(int) * qstart.size()));
-    SXB_HIP(b, hipMemcpy(P.qstart.p, qstart.data(), sizeof(int) * qstart.size(), hipMemcpyHostToDevice));
-    SXB_HIP(b, P.counters.ensure(sizeof(unsigned) * kXcds));
-    P.persistent_blocks = std::max(kXcds, std::min(b->max_resident, (P.nitems + kXcds - 1) / kXcds * kXcds));
-    P.persistent_blocks = (P.persistent_blocks + kXcds - 1) / kXcds * kXcds;
 
     // correlation blocks grouped by FFT length; finalisation blocks
     std::vector<sx::CorrArgs> ca(b->trajs.size());
@@ -328,16 +434,20 @@ int build_plan(sx_batch *b, const std::vector<int> &ks, int shape, int stride, P
     return SX_OK;
 }
 
-int launch_amplitude(sx_batch *b, Plan &P, int shape) {
-    hipLaunchKernelGGL(sx::reset_counters_kernel, dim3(1), dim3(kXcds), 0, b->stream, P.counters.as<unsigned>(),
-                       kXcds);
-    SXB_HIP(b, hipGetLastError());
-    // (the dynamic-LDS attribute of every shape is set once, in sx_batch_stage)
-    const size_t lds = b->residuals ? sx::persistent_res_lds(shape) : 0;
-    hipLaunchKernelGGL(sx::persistent_fn(shape), dim3(P.persistent_blocks), dim3(256), lds, b->stream,
-                       P.oka.as<sx::OrientKernelArgs>(), P.items.as<int4>(), P.qstart.as<int>(),
-                       P.counters.as<unsigned>());
-    SXB_HIP(b, hipGetLastError());
+int launch_amplitude(sx_batch *b, Plan &P) {
+    for (Queue &Q : P.queues) {
+        if (Q.nitems == 0) continue;
+        const int shape = Q.shape;
+        hipLaunchKernelGGL(sx::reset_counters_kernel, dim3(1), dim3(kXcds), 0, b->stream, Q.counters.as<unsigned>(),
+                           kXcds);
+        SXB_HIP(b, hipGetLastError());
+        // (the dynamic-LDS attribute of the residual kernels is set once, in sx_batch_stage)
+        const size_t lds = Q.res ? sx::persistent_res_lds(shape) : 0;
+        hipLaunchKernelGGL(sx::persistent_fn(shape, Q.res), dim3(Q.persistent_blocks), dim3(256), lds, b->stream,
+                           P.oka.as<sx::OrientKernelArgs>(), Q.items.as<int4>(), Q.qstart.as<int>(),
+                           Q.counters.as<unsigned>());
+        SXB_HIP(b, hipGetLastError());
+    }
     return SX_OK;
 }
 
@@ -364,7 +474,7 @@ int capture(sx_batch *b, Plan &P) {
                                    (int)P.final_lds));
     hipGraph_t g = nullptr;
     SXB_HIP(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-    int rc = launch_amplitude(b, P, b->shape);
+    int rc = launch_amplitude(b, P);
     if (rc == SX_OK) rc = launch_rest(b, P);
     hipError_t e = hipStreamEndCapture(b->stream, &g);
     if (rc != SX_OK) {
@@ -475,7 +585,6 @@ int sx_batch_stage(sx_batch *b, const sx_trajectory *trajs, size_t ntraj) {
             for (size_t s = 0; s < t.nshells; ++s) rr[s] = (float)(kTwoPi * g.res[s]);  // radians per Angstrom
             SXB_HIP(b, w.qres.ensure(sizeof(float) * t.nshells));
             SXB_HIP(b, hipMemcpy(w.qres.p, rr.data(), sizeof(float) * t.nshells, hipMemcpyHostToDevice));
-            b->residuals = true;
         }
         w.chunks = (int)std::min<size_t>(t.NM, 8);
         SXB_HIP(b, w.qlen.ensure(sizeof(double) * t.nshells));
@@ -492,49 +601,33 @@ int sx_batch_stage(sx_batch *b, const sx_trajectory *trajs, size_t ntraj) {
     // persistent grid: the workgroups that are resident at once
     int dev = b->cfg.device, cus = 0, per_cu = 0;
     SXB_HIP(b, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-    int min_per_cu = 1 << 30;
-    for (int sh = 0; sh < sx::kOrientShapes; ++sh) {
-        const size_t lds = b->residuals ? sx::persistent_res_lds(sh) : 0;
-        if (lds > 0)
-            SXB_HIP(b, hipFuncSetAttribute((const void *)sx::persistent_fn(sh),
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-        SXB_HIP(b, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)sx::persistent_fn(sh), 256,
-                                                               lds));
-        min_per_cu = std::min(min_per_cu, std::max(per_cu, 1));
-    }
-    b->max_resident = cus * min_per_cu;
+    for (int qi = 0; qi < 2; ++qi)
+        for (int sh = 0; sh < sx::kOrientShapes; ++sh) {
+            const size_t lds = qi ? sx::persistent_res_lds(sh) : 0;
+            const void *fn = (const void *)sx::persistent_fn(sh, qi == 1);
+            if (lds > 0) SXB_HIP(b, hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+            SXB_HIP(b, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 256, lds));
+            b->max_resident[qi][sh] = cus * std::max(per_cu, 1);
+        }
 
     std::vector<int> all(ntraj);
     std::iota(all.begin(), all.end(), 0);
-    // autotune: each shape on every 64th amplitude tile of the batch (at least
-    // kMinSampleTiles tiles)
-    if (b->cfg.tile_shape > 0) {
-        b->shape = b->cfg.tile_shape - 1;
-    } else {
-        float best = 1e30f;
-        for (int sh = 0; sh < sx::kOrientShapes; ++sh) {
-            Plan P;
-            int rc = build_plan(b, all, sh, 64, P);
-            if (rc != SX_OK) return rc;
-            SXB_HIP(b, hipEventRecord(b->e0, b->stream));
-            rc = launch_amplitude(b, P, sh);
-            if (rc != SX_OK) return rc;
-            SXB_HIP(b, hipEventRecord(b->e1, b->stream));
-            SXB_HIP(b, hipEventSynchronize(b->e1));
-            const float ms = elapsed(b->e0, b->e1);
-            if (ms < best) {
-                best = ms;
-                b->shape = sh;
-            }
-        }
-    }
+    // a fixed tile shape when asked for, else the mixed kernel
+    b->shape = b->cfg.tile_shape > 0 ? b->cfg.tile_shape - 1 : sx::kMixedShape;
     b->all = std::make_unique<Plan>();
-    int rc = build_plan(b, all, b->shape, 1, *b->all);
+    int rc = build_plan(b, all, b->shape, *b->all);
     if (rc != SX_OK) return rc;
     b->timing.tiles = (size_t)b->all->nitems;
     b->timing.tile_shape = b->shape;
     size_t terms = 0;
-    for (const sx_trajectory &t : b->trajs) terms += t.nshells * t.NM * t.NA * t.NF;
+    b->timing.terms_stepped = b->timing.terms_residual = 0;
+    for (size_t k = 0; k < ntraj; ++k) {
+        const sx_trajectory &t = b->trajs[k];
+        const size_t n = t.nshells * t.NM * t.NA * t.NF;
+        terms += n;
+        if (b->work[k].mode == 1) b->timing.terms_stepped += n;
+        if (b->work[k].mode == 2) b->timing.terms_residual += n;
+    }
     b->timing.terms = terms;
     rc = capture(b, *b->all);
     if (rc != SX_OK) return rc;
@@ -560,7 +653,7 @@ int sx_batch_time_amplitude(sx_batch *b, double *ms) {
     if (!b->all) return bfail(b, SX_ERR_STATE, "sx_batch_time_amplitude: stage the batch first");
     SXB_HIP(b, hipSetDevice(b->cfg.device));
     SXB_HIP(b, hipEventRecord(b->e0, b->stream));
-    int rc = launch_amplitude(b, *b->all, b->shape);
+    int rc = launch_amplitude(b, *b->all);
     if (rc != SX_OK) return rc;
     SXB_HIP(b, hipEventRecord(b->e1, b->stream));
     SXB_HIP(b, hipEventSynchronize(b->e1));
@@ -576,7 +669,7 @@ int sx_batch_run_one(sx_batch *b, size_t k, double *ms) {
     SXB_HIP(b, hipSetDevice(b->cfg.device));
     if (!b->one[k]) {
         auto P = std::make_unique<Plan>();
-        int rc = build_plan(b, {(int)k}, b->shape, 1, *P);
+        int rc = build_plan(b, {(int)k}, b->shape, *P);
         if (rc == SX_OK) rc = capture(b, *P);
         if (rc != SX_OK) return rc;
         b->one[k] = std::move(P);

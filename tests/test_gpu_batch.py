@@ -42,7 +42,7 @@ LONG_FRAMES = (1100, 1800, 2100, 2600, 3500, 5001, 6000, 8000, 9000)
 @pytest.mark.parametrize("shape", [0, 1, 2, 3, 4, 5])
 def test_batch_against_oracle(oracle_mod, shape):
     """Six small trajectories and one per long FFT plan class against the
-    FP64 oracle, every tile shape (0: autotuned)."""
+    FP64 oracle, every tile shape (0: mixed tiles)."""
     import sassena_amd
     assert [sassena_amd.pad_length(nf) // 2 for nf in LONG_FRAMES] == [1536, 2048, 2560, 3072, 4096, 5120, 6144,
                                                                           8192, 10240]
@@ -56,6 +56,46 @@ def test_batch_against_oracle(oracle_mod, shape):
         for s in range(len(ql)):
             rfqt, rfq, rfq2 = oracle_mod.compute("all", frames, b, ql[s] * u, nthreads=8)
             assert shell_error(fqt[s], rfqt)[0] < 1e-5
+            assert abs(fq[s] - rfq) < 1e-5 * abs(rfqt[0])
+            assert abs(fq2[s] - rfq2) <= fq2_bound(rfq2, rfqt[0])
+
+
+@pytest.mark.parametrize("kind", ["random", "exact", "scan"])
+def test_mixed_tiles_against_oracle(oracle_mod, kind):
+    """Shell counts whose cheapest cover mixes 1x8, 1x16 and 1x20 tiles with a
+    partial last tile (7: 8; 10: 16; 21 and 23: 16 + 8; 35: 20 + 16; 44:
+    20 + 16 + 8), the shells at random |q| (one phase each), on an exact grid
+    (stepped phasors) or from a single-precision scan (stepped with residual
+    corrections; no count has n - 1 a power of two, where the scan is exact):
+    every shell against the FP64 oracle, and the whole result against fixed
+    1 x 20 tiles (to the oracle's bound: the two step phasors from different
+    middle shells, and their rounding differs at 1e-6)."""
+    import sassena_amd
+    from sassena_amd import synth
+    rng = np.random.default_rng(71)
+    trajs = []
+    for k, ns in enumerate((7, 10, 21, 23, 35, 44)):
+        frames, b = small_trajectory(int(rng.integers(60, 200)), int(rng.integers(100, 400)), seed=700 + k)
+        if kind == "random":
+            ql = np.sort(rng.uniform(0.1, 4.0, ns))
+        elif kind == "exact":
+            ql = synth.exact_linear_shells(0.1, 4.0, ns)
+        else:
+            ql = synth.linear_shells(0.1, 4.0, ns)
+        trajs.append((frames, b, ql, unit_vectors(int(rng.integers(2, 7)), 70 + k)))
+    staged = [(np.ascontiguousarray(f.transpose(2, 1, 0)), b, ql, u) for f, b, ql, u in trajs]
+    with sassena_amd.Batch() as bt:
+        bt.stage(staged)
+        res = bt.run()
+        assert bt.timing()["tile_shape"] == 5
+    with sassena_amd.Batch(tile_shape=5) as bt:
+        bt.stage(staged)
+        fixed = bt.run()
+    for (frames, b, ql, u), (fqt, fq, fq2), (xfqt, _, _) in zip(trajs, res, fixed):
+        assert np.max(shell_error(fqt, xfqt)) < 1e-5
+        for s in range(len(ql)):
+            rfqt, rfq, rfq2 = oracle_mod.compute("all", frames, b, ql[s] * u, nthreads=8)
+            assert shell_error(fqt[s], rfqt)[0] < 1e-5, (len(ql), s)
             assert abs(fq[s] - rfq) < 1e-5 * abs(rfqt[0])
             assert abs(fq2[s] - rfq2) <= fq2_bound(rfq2, rfqt[0])
 
