@@ -73,9 +73,10 @@ typedef struct sx_config {
                             different blocks, 2 both in one block's sweep, 1 one after the
                             other in one block; 0 picks 3 where the FFT plan and memory
                             allow, else 2 */
-    int multipole_terms; /* sphere multipole: the spherical coordinates of every (atom, frame)
-                            1 formed once per call in a pre-pass (36 B each), 2 formed by the
-                            amplitude kernel itself; 0 picks 1 up to 16 GB of terms, else 2 */
+    int multipole_terms; /* multipole: the spherical (sphere, 36 B) or cylinder-frame
+                            (cylinder, 12 B) coordinates of every (atom, frame) 1 formed once
+                            per call in a pre-pass, 2 formed by the amplitude kernel itself;
+                            0 picks 1 up to 16 GB of terms, else 2 */
     int reserved[6];
 } sx_config;
 

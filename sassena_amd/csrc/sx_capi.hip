@@ -463,13 +463,18 @@ int run_batch(sx_ctx *c, int ns, size_t NM, const double *qv, const double *qlen
             ma.per_shell_b = b_shell ? 1 : 0;
             ma.amp = c->amp.as<float2>();
             ma.split = split;
-            // Sphere: the spherical terms of every (atom, frame) once per call
-            // (36 B each) when they fit the budget; else the kernel forms them.
-            const size_t sph_bytes = sx::kSphereTermBytes * c->NAp * c->NFp;
+            // The spherical (36 B) or cylinder (12 B) terms of every (atom,
+            // frame) once per call when they fit the budget; else the kernel
+            // forms them.
+            const size_t sph_bytes =
+                (mp->kind == 0 ? sx::kSphereTermBytes : sx::kCylinderTermBytes) * c->NAp * c->NFp;
             const int terms = c->cfg.multipole_terms;
-            if (mp->kind == 0 && (terms == 1 || (terms == 0 && sph_bytes <= kSphereTermBudget))) {
+            if (terms == 1 || (terms == 0 && sph_bytes <= kSphereTermBudget)) {
                 SX_HIP(c, c->mpsph.ensure(sph_bytes));
-                SX_HIP(c, sx::launch_multipole_sphere_terms(ma, c->mpsph.p, c->stream));
+                if (mp->kind == 0)
+                    SX_HIP(c, sx::launch_multipole_sphere_terms(ma, c->mpsph.p, c->stream));
+                else
+                    SX_HIP(c, sx::launch_multipole_cylinder_terms(ma, c->mpsph.p, c->stream));
                 ma.sph = c->mpsph.p;
             }
             SX_HIP(c, sx::launch_multipole(ma, c->stream));

@@ -224,9 +224,10 @@ struct MultipoleArgs {
     int per_shell_b;
     float2 *amp;           // [split][ns * nmo][NFp]: one partial set per atom range
     int split = 1;
-    // sphere: per (atom, frame) spherical terms from launch_multipole_sphere_terms
-    // ([NAp][NFp] planes: r as float, then cos(theta), sin(theta), sin(phi),
-    // cos(phi) as double), or nullptr to form them in the kernel
+    // per (atom, frame) terms, or nullptr to form them in the kernel: sphere
+    // from launch_multipole_sphere_terms ([NAp][NFp] planes: r as float, then
+    // cos(theta), sin(theta), sin(phi), cos(phi) as double), cylinder from
+    // launch_multipole_cylinder_terms (r, phi, z in the axis frame as float)
     const void *sph = nullptr;
 };
 hipError_t launch_multipole(const MultipoleArgs &m, hipStream_t stream);
@@ -236,6 +237,8 @@ int multipole_base_blocks(size_t NFp, int ns, int nmo);
 // Bytes of the sphere-term planes for NAp x NFp (atom, frame) pairs.
 constexpr size_t kSphereTermBytes = sizeof(float) + 4 * sizeof(double);
 hipError_t launch_multipole_sphere_terms(const MultipoleArgs &m, void *sph, hipStream_t stream);
+constexpr size_t kCylinderTermBytes = 3 * sizeof(float);
+hipError_t launch_multipole_cylinder_terms(const MultipoleArgs &m, void *terms, hipStream_t stream);
 
 // "self" with dsp.type square (mode 1) / plain (mode 2), frames in tiles of
 // kMaxSeriesFrames; with more than one tile each tile's part of a per pair

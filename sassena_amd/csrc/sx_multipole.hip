@@ -39,6 +39,7 @@ struct MultipoleKernelArgs {
     size_t range;           // atoms per range (the last may be shorter)
     const float *srf;       // sphere terms per (atom, frame), or nullptr: r (float-rounded),
     const double *sct, *sst, *ssp, *scp;  // cos theta, sin theta, sin phi, cos phi
+    const float *cphi, *cz;  // cylinder terms (r in srf), float-rounded as the reference stages them
 };
 
 // j_l(x) for the moment degrees ls[0..kMoments) of a wavefront in one
@@ -269,12 +270,36 @@ __device__ void to_cylinder(double x, double y, double z, double &r, double &phi
     zz = z;
 }
 
+// The cylinder-frame coordinates of every (atom, frame) (the frame of
+// MultipoleKernelArgs::base), rounded to float as the reference stages them:
+// one thread per (atom, frame), once per call.
+__global__ __launch_bounds__(256) void cylinder_terms_kernel(const float *__restrict__ X, const float *__restrict__ Y,
+                                                             const float *__restrict__ Z, size_t n,
+                                                             const double *__restrict__ base, float *rr, float *pp,
+                                                             float *zz) {
+    const size_t e = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= n) return;
+    const double x = X[e], y = Y[e], z = Z[e];
+    const double px = x * base[0] + y * base[1] + z * base[2];
+    const double py = x * base[3] + y * base[4] + z * base[5];
+    const double pz = x * base[6] + y * base[7] + z * base[8];
+    double r, phi, zc;
+    to_cylinder(px, py, pz, r, phi, zc);
+    rr[e] = (float)r;
+    pp[e] = (float)phi;
+    zz[e] = (float)zc;
+}
+
 // KIND 0 sphere, 1 cylinder, each instantiated on its own.  Both are bound by
 // the latency of serial FP64 recurrences, so they are held to 3 (sphere) and
 // 2 (cylinder) wavefronts per SIMD, the bounds that measured fastest despite
 // some spilled registers (1 wavefront per SIMD took 1.6x (sphere) and 1.8x
 // (cylinder) as long).
-template <int KIND>
+// PRE: the spherical or cylinder terms of every (atom, frame) come from
+// sphere_terms_kernel / cylinder_terms_kernel (a.srf ...); else they are
+// formed here, whose acos, atan and sincos would otherwise add their
+// registers to the common path.
+template <int KIND, bool PRE>
 __global__ __launch_bounds__(256, KIND == 0 ? 3 : 2) void multipole_kernel(MultipoleKernelArgs a) {
     constexpr int kind = KIND;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -289,45 +314,54 @@ __global__ __launch_bounds__(256, KIND == 0 ? 3 : 2) void multipole_kernel(Multi
     const int k0 = mb * kMoments;
     const int nk = min(kMoments, a.nmo - k0);
     const size_t t = (size_t)ftile * 64 + lane;
-    int ls[kMoments], ms[kMoments];
-    double cms[kMoments];  // sphere: start of each moment's Legendre recurrence
-#pragma unroll
-    for (int k = 0; k < kMoments; ++k) {
-        const int kk = min(k0 + k, a.nmo - 1);
-        ls[k] = a.moments[2 * kk];
-        ms[k] = a.moments[2 * kk + 1];
-        cms[k] = kind == 0 ? a.ynorm[2 * kk] : 0.0;
-    }
     double ar[kMoments], ai[kMoments];
-    int lhi = 0;
-    bool sorted = true;  // degrees non-decreasing over the wavefront's moments
 #pragma unroll
-    for (int k = 0; k < kMoments; ++k) {
-        ar[k] = ai[k] = 0.0;
-        lhi = max(lhi, ls[k]);
-        if (k > 0 && ls[k] < ls[k - 1]) sorted = false;
-    }
+    for (int k = 0; k < kMoments; ++k) ar[k] = ai[k] = 0.0;
     const float *brow = a.b + (a.per_shell_b ? (size_t)s * a.NAp : 0);
-    double b9[9];
-    // cylinder: Bessel order per moment (J_0 for (0,0), J_2l for m = 0, 1,
-    // J_2l-1 for m = 2, 3) and the largest
-    int cn[kMoments];
-    int nhi = 0;
-    if (kind == 1) {
-        for (int k = 0; k < 9; ++k) b9[k] = a.base[k];
-#pragma unroll
-        for (int k = 0; k < kMoments; ++k) {
-            const int l = ls[k], m = ms[k];
-            cn[k] = (l == 0 || m > 3 || m < 0) ? 0 : (m <= 1 ? 2 * l : 2 * l - 1);
-            nhi = max(nhi, cn[k]);
-        }
-    }
     const size_t jend = min(a.NAp, (size_t)(sp + 1) * a.range);
     for (size_t j = (size_t)sp * a.range; j < jend; ++j) {
         const float bj = brow[j];
         if (bj == 0.f) continue;  // padding atoms and zero lengths add nothing
+        // The moments' parameters, read again in every atom iteration through
+        // an index the compiler cannot take for a constant: held across the
+        // atom loop they fill the SGPRs and spill into VGPR lanes, with a
+        // v_readlane at every use.  The reads are scalar loads of a few cached
+        // words.
+        int zo = 0;
+        asm volatile("" : "+s"(zo));
+        int ls[kMoments], ms[kMoments];
+        double cms[kMoments];  // sphere: start of each moment's Legendre recurrence
+#pragma unroll
+        for (int k = 0; k < kMoments; ++k) {
+            const int kk = min(k0 + k, a.nmo - 1) + zo;
+            ls[k] = a.moments[2 * kk];
+            ms[k] = a.moments[2 * kk + 1];
+            cms[k] = kind == 0 ? a.ynorm[2 * kk] : 0.0;
+        }
+        int lhi = 0;
+        bool sorted = true;  // degrees non-decreasing over the wavefront's moments
+#pragma unroll
+        for (int k = 0; k < kMoments; ++k) {
+            lhi = max(lhi, ls[k]);
+            if (k > 0 && ls[k] < ls[k - 1]) sorted = false;
+        }
+        double b9[9];
+        // cylinder: Bessel order per moment (J_0 for (0,0), J_2l for m = 0, 1,
+        // J_2l-1 for m = 2, 3) and the largest
+        int cn[kMoments];
+        int nhi = 0;
+        if (kind == 1) {
+            if (!PRE)
+                for (int k = 0; k < 9; ++k) b9[k] = a.base[k + zo];
+#pragma unroll
+            for (int k = 0; k < kMoments; ++k) {
+                const int l = ls[k], m = ms[k];
+                cn[k] = (l == 0 || m > 3 || m < 0) ? 0 : (m <= 1 ? 2 * l : 2 * l - 1);
+                nhi = max(nhi, cn[k]);
+            }
+        }
         double x = 0.0, y = 0.0, z = 0.0;
-        if (kind != 0 || !a.srf) {
+        if constexpr (!PRE) {
             x = a.X[j * a.NFp + t];
             y = a.Y[j * a.NFp + t];
             z = a.Z[j * a.NFp + t];
@@ -335,7 +369,7 @@ __global__ __launch_bounds__(256, KIND == 0 ? 3 : 2) void multipole_kernel(Multi
         if (kind == 0) {
             const double ql = a.qinfo[s];
             double rf, ct, somx2, sp, cp;
-            if (a.srf) {
+            if constexpr (PRE) {
                 const size_t e = j * a.NFp + t;
                 rf = a.srf[e];
                 ct = a.sct[e];
@@ -407,12 +441,22 @@ __global__ __launch_bounds__(256, KIND == 0 ? 3 : 2) void multipole_kernel(Multi
             }
         } else {
             const double qr = a.qinfo[3 * s], qphi = a.qinfo[3 * s + 1], qz = a.qinfo[3 * s + 2];
-            const double px = x * b9[0] + y * b9[1] + z * b9[2];
-            const double py = x * b9[3] + y * b9[4] + z * b9[5];
-            const double pz = x * b9[6] + y * b9[7] + z * b9[8];
-            double r, phi, zz;
-            to_cylinder(px, py, pz, r, phi, zz);
-            const double rf = (float)r, phif = (float)phi, zf = (float)zz;
+            double rf, phif, zf;
+            if constexpr (PRE) {
+                const size_t e = j * a.NFp + t;
+                rf = a.srf[e];
+                phif = a.cphi[e];
+                zf = a.cz[e];
+            } else {
+                const double px = x * b9[0] + y * b9[1] + z * b9[2];
+                const double py = x * b9[3] + y * b9[4] + z * b9[5];
+                const double pz = x * b9[6] + y * b9[7] + z * b9[8];
+                double r, phi, zz;
+                to_cylinder(px, py, pz, r, phi, zz);
+                rf = (float)r;
+                phif = (float)phi;
+                zf = (float)zz;
+            }
             double sgn = 1.0;
             if ((zf != 0) && (qz != 0)) sgn = (zf * qz) / (fabs(zf) * fabs(qz));
             double exs, exc;
@@ -476,6 +520,9 @@ hipError_t launch_multipole(const MultipoleArgs &m, hipStream_t stream) {
     k.zgroups = (k.mblocks + 3) / 4;
     k.range = (m.NAp + k.split - 1) / k.split;
     const size_t n = m.NAp * m.NFp;
+    k.srf = nullptr;
+    k.sct = k.sst = k.ssp = k.scp = nullptr;
+    k.cphi = k.cz = nullptr;
     if (m.sph && m.kind == 0) {
         k.srf = static_cast<const float *>(m.sph);
         const double *d = reinterpret_cast<const double *>(static_cast<const char *>(m.sph) + sizeof(float) * n);
@@ -483,22 +530,27 @@ hipError_t launch_multipole(const MultipoleArgs &m, hipStream_t stream) {
         k.sst = d + n;
         k.ssp = d + 2 * n;
         k.scp = d + 3 * n;
-    } else {
-        k.srf = nullptr;
-        k.sct = k.sst = k.ssp = k.scp = nullptr;
+    } else if (m.sph) {
+        k.srf = static_cast<const float *>(m.sph);
+        k.cphi = k.srf + n;
+        k.cz = k.srf + 2 * n;
     }
     if ((size_t)k.zgroups * k.split > 65535) return hipErrorInvalidValue;
     dim3 grid((unsigned)(m.NFp / 64), (unsigned)m.ns, (unsigned)(k.zgroups * k.split));
-    if (m.kind == 0)
-        hipLaunchKernelGGL(multipole_kernel<0>, grid, dim3(256), 0, stream, k);
+    if (m.kind == 0 && k.srf)
+        hipLaunchKernelGGL((multipole_kernel<0, true>), grid, dim3(256), 0, stream, k);
+    else if (m.kind == 0)
+        hipLaunchKernelGGL((multipole_kernel<0, false>), grid, dim3(256), 0, stream, k);
+    else if (k.srf)
+        hipLaunchKernelGGL((multipole_kernel<1, true>), grid, dim3(256), 0, stream, k);
     else
-        hipLaunchKernelGGL(multipole_kernel<1>, grid, dim3(256), 0, stream, k);
+        hipLaunchKernelGGL((multipole_kernel<1, false>), grid, dim3(256), 0, stream, k);
     return hipGetLastError();
 }
 
 hipError_t multipole_blocks_per_cu(int kind, int *blocks) {
-    return kind == 0 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, multipole_kernel<0>, 256, 0)
-                     : hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, multipole_kernel<1>, 256, 0);
+    return kind == 0 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, multipole_kernel<0, true>, 256, 0)
+                     : hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, multipole_kernel<1, true>, 256, 0);
 }
 
 int multipole_base_blocks(size_t NFp, int ns, int nmo) {
@@ -513,6 +565,15 @@ hipError_t launch_multipole_sphere_terms(const MultipoleArgs &m, void *sph, hipS
     const float *X = m.coords, *Y = m.coords + n, *Z = m.coords + 2 * n;
     hipLaunchKernelGGL(sphere_terms_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, X, Y, Z, n, srf,
                        d, d + n, d + 2 * n, d + 3 * n);
+    return hipGetLastError();
+}
+
+hipError_t launch_multipole_cylinder_terms(const MultipoleArgs &m, void *terms, hipStream_t stream) {
+    const size_t n = m.NAp * m.NFp;
+    float *rr = static_cast<float *>(terms);
+    const float *X = m.coords, *Y = m.coords + n, *Z = m.coords + 2 * n;
+    hipLaunchKernelGGL(cylinder_terms_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, X, Y, Z, n,
+                       m.base, rr, rr + n, rr + 2 * n);
     return hipGetLastError();
 }
 

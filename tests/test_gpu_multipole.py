@@ -59,12 +59,15 @@ def test_sphere_against_oracle(oracle_mod, dsp, method, terms):
                                                                    method=method))
 
 
+@pytest.mark.parametrize("terms", [1, 2])
 @pytest.mark.parametrize("axis", [(0.0, 0.0, 1.0), (0.3, -0.2, 0.9), (1.0, 0.0, 0.0)])
-def test_cylinder_against_oracle(oracle_mod, axis):
+def test_cylinder_against_oracle(oracle_mod, axis, terms):
+    """Both sources of the cylinder-frame coordinates: the per-(atom, frame)
+    pre-pass (1) and the kernel forming them itself (2)."""
     frames, b = centred_walk(100, 300, 12)
     qv = np.array([[0.4, 0.1, 0.2], [-0.3, 0.9, -0.5], [0.0, 0.0, 1.1], [1.2, -0.6, 0.0]])
     mom = oracle_mod.cylinder_moments(6)
-    fqt, fq, fq2 = run("cylinder", frames, b, qv, mom, axis=axis)
+    fqt, fq, fq2 = run("cylinder", frames, b, qv, mom, axis=axis, terms=terms)
     for s in range(len(qv)):
         check(fqt[s], fq[s], fq2[s], oracle_mod.compute_multipole("cylinder", frames, b, qv[s], mom, axis=axis))
 
