@@ -506,7 +506,8 @@ def main():
         "roofline": dict(roof, traffic=AMP_TRAFFIC_BYTES if not (args.generic or args.vector_block or
                                                                  args.atom_split) else None,
                          traffic_unit="bytes/launch (rocprofv3 FETCH_SIZE+WRITE_SIZE, copy-calibrated)",
-                         kernel="amp_all_kernel" if args.generic else "amp_oriented_kernel"),
+                         kernel="amp_all_kernel" if args.generic else
+                         ("amp_oriented_res_kernel" if residual else "amp_oriented_kernel")),
     }
     if not args.no_cpu:
         threads = host_threads()
