@@ -189,9 +189,11 @@ def bench_c3(args):
     if not args.no_cpu:
         import oracle
         threads = host_threads()
-        idx = np.flatnonzero(T.b)[:threads * 4]
+        # about 10 s of host work: 300 atoms per thread x 16 vectors of the
+        # last shell over all frames
+        idx = np.flatnonzero(T.b)[:threads * 300]
         atoms = np.ascontiguousarray(np.transpose(T.xyz[:, idx, :], (1, 2, 0)))
-        qv = T.qvectors(T.nshells - 1)[:8]
+        qv = T.qvectors(T.nshells - 1)[:16]
         t1 = time.perf_counter()
         oracle.compute("self", atoms, T.b[idx], qv, "autocorrelate", "fftw", nthreads=threads)
         dt = time.perf_counter() - t1

@@ -48,9 +48,6 @@ OrientKernelArgs make_orient_args(const OrientArgs &a);
 //
 // Atom sums run in FP32 over sub-chunks of CH atoms (coordinates loaded
 // together) and are folded into FP64 totals every kFold atoms.
-// Word 3 of a raw (unswizzled, stride 0) buffer resource on CDNA: 32-bit
-// float data format, no index stride.
-constexpr int kBufferDword3 = 0x00020000;
 
 // Wave-uniform, read-only kernel data (orientations, shells, residuals,
 // scattering factors) seen through the constant address space, so that it is

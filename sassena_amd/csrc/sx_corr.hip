@@ -134,9 +134,7 @@ __global__ __launch_bounds__(512) void corr_self_kernel(CorrArgs a) {
         const int j = a.aidx ? a.aidx[ja] : ja;
         const double *q = a.qrev + ((size_t)s * a.NM + m) * 3;
         SelfInput in;
-        in.X = a.coords + (size_t)j * a.NFp;
-        in.Y = in.X + plane;
-        in.Z = in.X + 2 * plane;
+        in.set(a.coords + (size_t)j * a.NFp, plane, a.NF);
         in.qx = q[0];
         in.qy = q[1];
         in.qz = q[2];
@@ -192,9 +190,7 @@ __device__ __forceinline__ void self_half_body(const CorrArgs &a, double2 *__res
         const int j = a.aidx ? a.aidx[ja] : ja;
         const double *q = a.qrev + ((size_t)s * a.NM + m) * 3;
         In in;
-        in.X = a.coords + (size_t)j * a.NFp;
-        in.Y = in.X + plane;
-        in.Z = in.X + 2 * plane;
+        in.set(a.coords + (size_t)j * a.NFp, plane, a.NF);
         in.qx = q[0];
         in.qy = q[1];
         in.qz = q[2];
@@ -223,7 +219,9 @@ __device__ __forceinline__ void self_half_body(const CorrArgs &a, double2 *__res
 }
 
 template <typename C, int E>
-__global__ __launch_bounds__(256) void corr_self_split_kernel(CorrArgs a, double2 *apair) {
+// E = 20 (halves of 5,120 points): four workgroups per CU, as many as the
+// LDS allows, so at most 128 VGPRs.
+__global__ __launch_bounds__(256, E == 20 ? 4 : 1) void corr_self_split_kernel(CorrArgs a, double2 *apair) {
     extern __shared__ double2 lds_d[];
     C *buf = reinterpret_cast<C *>(lds_d);
     const int sc = blockIdx.x >> 1;
@@ -252,9 +250,7 @@ __global__ __launch_bounds__(256) void self_tail_kernel(CorrArgs a, double2 *__r
         const int j = a.aidx ? a.aidx[ja] : ja;
         const double *q = a.qrev + ((size_t)s * a.NM + m) * 3;
         SelfInput in;
-        in.X = a.coords + (size_t)j * a.NFp;
-        in.Y = in.X + plane;
-        in.Z = in.X + 2 * plane;
+        in.set(a.coords + (size_t)j * a.NFp, plane, a.NF);
         in.qx = q[0];
         in.qy = q[1];
         in.qz = q[2];
@@ -714,9 +710,7 @@ __global__ __launch_bounds__(256) void self_series_kernel(int mode, CorrArgs a, 
         const int j = (int)(pr / a.NM), m = (int)(pr % a.NM);
         const double *q = a.qrev + ((size_t)s * a.NM + m) * 3;
         SelfInput in;
-        in.X = a.coords + (size_t)j * a.NFp;
-        in.Y = in.X + plane;
-        in.Z = in.X + 2 * plane;
+        in.set(a.coords + (size_t)j * a.NFp, plane, a.NF);
         in.qx = q[0];
         in.qy = q[1];
         in.qz = q[2];

@@ -130,6 +130,26 @@ struct Dft<5, C> {
 // series share every barrier; NH = 1: one half per sweep, half the LDS).
 // Half k of the sweep lives at buf[k*Lh, (k+1)*Lh) and is global half h0+k.
 
+// tw[base + k] through a buffer resource, k per lane and base uniform.
+template <typename C>
+__device__ __forceinline__ C twiddle_load(const C *tw, unsigned k, unsigned base) {
+    const __amdgpu_buffer_rsrc_t r = __builtin_amdgcn_make_buffer_rsrc((void *)tw, (short)0, 0x7fffffff, kBufferDword3);
+    if constexpr (sizeof(C) == 8) {
+        const auto v = __builtin_amdgcn_raw_buffer_load_b64(r, k * 8u, base * 8u, 0);
+        return Cx<C>::make(__uint_as_float(v[0]), __uint_as_float(v[1]));
+    } else {
+        const auto v = __builtin_amdgcn_raw_buffer_load_b128(r, k * 16u, base * 16u, 0);
+        return Cx<C>::make(__hiloint2double((int)v[1], (int)v[0]), __hiloint2double((int)v[3], (int)v[2]));
+    }
+}
+
+// w[base + j] through a buffer resource, j per lane and base uniform.
+__device__ __forceinline__ double2 weight_load(const double2 *w, unsigned j, unsigned base) {
+    const __amdgpu_buffer_rsrc_t r = __builtin_amdgcn_make_buffer_rsrc((void *)w, (short)0, 0x7fffffff, kBufferDword3);
+    const auto v = __builtin_amdgcn_raw_buffer_load_b128(r, j * 16u, base * 16u, 0);
+    return make_double2(__hiloint2double((int)v[1], (int)v[0]), __hiloint2double((int)v[3], (int)v[2]));
+}
+
 // Middle pass p: read radix-R groups from LDS, twiddle, butterfly, write back.
 template <typename C, int NH, int E, int R>
 __device__ __forceinline__ void pass_mid(const FftPlan &P, int p, C (&v)[NH][E], C *buf,
@@ -144,9 +164,9 @@ __device__ __forceinline__ void pass_mid(const FftPlan &P, int p, C (&v)[NH][E],
         const bool shared = Ns <= T;
         C ws[R];
         if (shared) {
-            const C *tk = twp + P.toff[p] + (ut & (Ns - 1)) * R;
+            const unsigned kk = (ut & (Ns - 1)) * R;
 #pragma unroll
-            for (int q = 1; q < R; ++q) ws[q] = tk[q];
+            for (int q = 1; q < R; ++q) ws[q] = twiddle_load(twp, kk, P.toff[p] + q);
         }
 #pragma unroll
         for (int i = 0; i < E / R; ++i) {
@@ -167,10 +187,9 @@ __device__ __forceinline__ void pass_mid(const FftPlan &P, int p, C (&v)[NH][E],
 #pragma unroll
                     for (int k = 0; k < NH; ++k) v[k][i * R + q] = cmul(v[k][i * R + q], ws[q]);
             } else {
-                const C *tk = twp + P.toff[p] + kk * R;
 #pragma unroll
                 for (int q = 1; q < R; ++q) {
-                    const C w = tk[q];
+                    const C w = twiddle_load(twp, kk * R, P.toff[p] + q);
 #pragma unroll
                     for (int k = 0; k < NH; ++k) v[k][i * R + q] = cmul(v[k][i * R + q], w);
                 }
@@ -210,10 +229,9 @@ __device__ __forceinline__ void pass_last(const FftPlan &P, int h0, C (&v)[NH][E
 #pragma unroll
         for (int i = 0; i < E / R; ++i) {
             const unsigned j = ut + i * T;  // k == j in the last pass
-            const C *tk = twp + P.toff[P.npass - 1] + j * R;
 #pragma unroll
             for (int q = 1; q < R; ++q) {
-                const C t = tk[q];
+                const C t = twiddle_load(twp, j * R, P.toff[P.npass - 1] + q);
 #pragma unroll
                 for (int k = 0; k < NH; ++k) v[k][i * R + q] = cmul(v[k][i * R + q], t);
             }
@@ -225,8 +243,10 @@ __device__ __forceinline__ void pass_last(const FftPlan &P, int h0, C (&v)[NH][E
                     const C x = v[k][i * R + q];
                     const typename Cx<C>::R pw = x.x * x.x + x.y * x.y;
                     acc[h0 + k][i * R + q] += (Acc)pw;
-                    // bin f = 2 (j + q stride) + h0 + k, weights in visiting order
-                    const double2 wf = w[((unsigned)(h0 + k) * R + q) * stride + j];
+                    // bin f = 2 (j + q stride) + h0 + k, weights in visiting order:
+                    // the thread's j as the vector offset, the rest as the
+                    // scalar one (no 64-bit address per load)
+                    const double2 wf = weight_load(w, j, ((unsigned)(h0 + k) * R + q) * stride);
                     ar = fma(wf.x, (double)pw, ar);
                     ai = fma(wf.y, (double)pw, ai);
                 }
@@ -252,7 +272,10 @@ __device__ __forceinline__ void pass_first(const FftPlan &P, int h0, C (&v)[NH][
             for (int q = 0; q < R; ++q) {
                 const unsigned n = j + q * stride;
                 C x = Cx<C>::make(0, 0);
-                if (n < unf) x = in(n);
+                if constexpr (Input::kBounded)
+                    x = in(n);  // 0 past NF by itself
+                else if (n < unf)
+                    x = in(n);
 #pragma unroll
                 for (int k = 0; k < NH; ++k)
                     v[k][i * R + q] = (Input::kPretwiddled || h0 + k == 0 || n >= unf) ? x : cmul(x, pre[n]);
@@ -378,6 +401,7 @@ constexpr int kFlushEvery = 16;
 template <typename C>
 struct AmpInput {
     static constexpr bool kPretwiddled = false;
+    static constexpr bool kBounded = false;
     const float2 *p;
     size_t split_stride;
     int split;
@@ -473,34 +497,60 @@ __device__ __forceinline__ void corr_all_block(const CorrArgs &a, int s, int c, 
 
 // "self": the series of pair (atom j, vector m) is b_j exp(i q_m . r_j(t)),
 // evaluated on the fly in the first FFT pass from the SoA coordinates
-// (self_vectors_scatter_device.cpp:306-319).
-struct SelfInput {
+// (self_vectors_scatter_device.cpp:306-319).  The rows are read through
+// buffer resources built once per pair and bounded at NF frames, so a read
+// past the series returns 0 and the padding needs no branch (b is zeroed by
+// a select): the first pass can issue the loads of all its points together,
+// where a branch per point waited for each point's loads in turn.
+struct SelfRows {
+    __amdgpu_buffer_rsrc_t rx, ry, rz;
+    unsigned nf;
+    __device__ __forceinline__ void set(const float *X, size_t plane, int NF) {
+        nf = (unsigned)NF;
+        rx = __builtin_amdgcn_make_buffer_rsrc((void *)X, (short)0, nf * 4u, kBufferDword3);
+        ry = __builtin_amdgcn_make_buffer_rsrc((void *)(X + plane), (short)0, nf * 4u, kBufferDword3);
+        rz = __builtin_amdgcn_make_buffer_rsrc((void *)(X + 2 * plane), (short)0, nf * 4u, kBufferDword3);
+    }
+    __device__ __forceinline__ void load(unsigned n, float &x, float &y, float &z) const {
+        x = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rx, n * 4u, 0, 0));
+        y = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(ry, n * 4u, 0, 0));
+        z = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rz, n * 4u, 0, 0));
+    }
+};
+
+struct SelfInput : SelfRows {
     static constexpr bool kPretwiddled = false;
-    const float *X, *Y, *Z;
+    static constexpr bool kBounded = true;  // 0 past NF without a branch
     double qx, qy, qz;
     float b;
     __device__ __forceinline__ float2 operator()(unsigned n) const {
-        double p = qx * (double)X[n];
-        p = fma(qy, (double)Y[n], p);
-        p = fma(qz, (double)Z[n], p);
+        float xs, ys, zs;
+        load(n, xs, ys, zs);
+        double p = qx * (double)xs;
+        p = fma(qy, (double)ys, p);
+        p = fma(qz, (double)zs, p);
         const float f = (float)__builtin_amdgcn_fract(p);
-        return make_float2(b * __builtin_amdgcn_cosf(f), b * __builtin_amdgcn_sinf(f));
+        const float bn = n < nf ? b : 0.f;
+        return make_float2(bn * __builtin_amdgcn_cosf(f), bn * __builtin_amdgcn_sinf(f));
     }
 };
 
 // The odd-bin input x_n W_L^n of a self series with the pre-twiddle folded
 // into the phase: b exp(2 pi i (q.r_n - n/L)), reduced in FP64.
-struct SelfInputShifted {
+struct SelfInputShifted : SelfRows {
     static constexpr bool kPretwiddled = true;
-    const float *X, *Y, *Z;
+    static constexpr bool kBounded = true;
     double qx, qy, qz, invL;
     float b;
     __device__ __forceinline__ float2 operator()(unsigned n) const {
-        double p = fma(-(double)n, invL, qx * (double)X[n]);
-        p = fma(qy, (double)Y[n], p);
-        p = fma(qz, (double)Z[n], p);
+        float xs, ys, zs;
+        load(n, xs, ys, zs);
+        double p = fma(-(double)n, invL, qx * (double)xs);
+        p = fma(qy, (double)ys, p);
+        p = fma(qz, (double)zs, p);
         const float f = (float)__builtin_amdgcn_fract(p);
-        return make_float2(b * __builtin_amdgcn_cosf(f), b * __builtin_amdgcn_sinf(f));
+        const float bn = n < nf ? b : 0.f;
+        return make_float2(bn * __builtin_amdgcn_cosf(f), bn * __builtin_amdgcn_sinf(f));
     }
 };
 
@@ -605,6 +655,7 @@ __device__ __forceinline__ void final_block(const FinalArgs &a, int s, int tile,
 template <typename C, typename Src>
 struct RowInput {
     static constexpr bool kPretwiddled = true;
+    static constexpr bool kBounded = false;
     Src src;
     const C *tw;
     unsigned L, NF, Lh;

@@ -20,6 +20,10 @@
 
 namespace sx {
 
+// Word 3 of a raw (unswizzled, stride 0) buffer resource on CDNA: 32-bit
+// float data format, no index stride.
+constexpr int kBufferDword3 = 0x00020000;
+
 constexpr int kWave = 64;         // CDNA wavefront width
 constexpr int kAtomChunk = 8;     // atoms per coordinate batch of amp_all_kernel
 constexpr int kMaxPasses = 16;
