@@ -476,6 +476,14 @@ int run_batch(sx_ctx *c, int ns, size_t NM, const double *qv, const double *qlen
                 else
                     SX_HIP(c, sx::launch_multipole_cylinder_terms(ma, c->mpsph.p, c->stream));
                 ma.sph = c->mpsph.p;
+                // the two lowest Bessel functions of every (shell, atom, frame)
+                // once per call (16 B each) when they fit the same budget
+                const size_t j01_bytes = sizeof(double2) * (size_t)ns * c->NAp * c->NFp;
+                if (j01_bytes <= kSphereTermBudget) {
+                    SX_HIP(c, c->mpj.ensure(j01_bytes));
+                    SX_HIP(c, sx::launch_multipole_bessel01(ma, c->mpj.p, c->stream));
+                    ma.cj01 = c->mpj.p;
+                }
             }
             SX_HIP(c, sx::launch_multipole(ma, c->stream));
         } else if (oriented) {
@@ -771,7 +779,7 @@ void sx_destroy(sx_ctx *c) {
     if (!c) return;
     (void)hipSetDevice(c->cfg.device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
-    for (DevBuf *d : {&c->coords, &c->b, &c->twL, &c->twL64, &c->twp, &c->twp64, &c->tw64, &c->w, &c->aidx, &c->apair, &c->tail, &c->qrev, &c->qlen, &c->uvec, &c->brow, &c->bshell, &c->mpmom, &c->mpq, &c->mpsph, &c->qres, &c->redbuf, &c->amp,
+    for (DevBuf *d : {&c->coords, &c->b, &c->twL, &c->twL64, &c->twp, &c->twp64, &c->tw64, &c->w, &c->aidx, &c->apair, &c->tail, &c->qrev, &c->qlen, &c->uvec, &c->brow, &c->bshell, &c->mpmom, &c->mpq, &c->mpsph, &c->mpj, &c->qres, &c->redbuf, &c->amp,
                       &c->ppart, &c->apart, &c->fqt, &c->fq, &c->fq2, &c->stage, &c->stage2})
         d->release();
     for (auto &e : c->ev)

@@ -82,7 +82,7 @@ struct sx_ctx {
     sxh::DevBuf coords, b, twL, twL64, twp, twp64, tw64, w, aidx, apair, tail;
     std::vector<int> aidx_h;  // "self": atoms with a nonzero length in the current call
     std::vector<float> b_h;   // host copy of the float factors of sx_set_factors
-    sxh::DevBuf qrev, qlen, uvec, brow, bshell, amp, ppart, apart, fqt, fq, fq2, stage, stage2, mpmom, mpq, mpsph,
+    sxh::DevBuf qrev, qlen, uvec, brow, bshell, amp, ppart, apart, fqt, fq, fq2, stage, stage2, mpmom, mpq, mpsph, mpj,
         qres, redbuf;
     sx_timing timing{};
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};

@@ -233,6 +233,10 @@ struct MultipoleArgs {
     // cos(theta), sin(theta), sin(phi), cos(phi) as double), cylinder from
     // launch_multipole_cylinder_terms (r, phi, z in the axis frame as float)
     const void *sph = nullptr;
+    // with the terms: j_0, j_1 of |q| r (sphere) or J_0, J_1 of q_r r
+    // (cylinder) per (shell, atom, frame) from launch_multipole_bessel01, or
+    // nullptr to form them in the kernel
+    const void *cj01 = nullptr;
 };
 hipError_t launch_multipole(const MultipoleArgs &m, hipStream_t stream);
 // Workgroups of the multipole kernel that fit one CU, and its workgroups per atom range.
@@ -243,6 +247,8 @@ constexpr size_t kSphereTermBytes = sizeof(float) + 4 * sizeof(double);
 hipError_t launch_multipole_sphere_terms(const MultipoleArgs &m, void *sph, hipStream_t stream);
 constexpr size_t kCylinderTermBytes = 3 * sizeof(float);
 hipError_t launch_multipole_cylinder_terms(const MultipoleArgs &m, void *terms, hipStream_t stream);
+// Needs m.sph set to the terms (r first); out: [ns][NAp][NFp] double2.
+hipError_t launch_multipole_bessel01(const MultipoleArgs &m, void *out, hipStream_t stream);
 
 // "self" with dsp.type square (mode 1) / plain (mode 2), frames in tiles of
 // kMaxSeriesFrames; with more than one tile each tile's part of a per pair

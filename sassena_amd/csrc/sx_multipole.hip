@@ -40,6 +40,7 @@ struct MultipoleKernelArgs {
     const float *srf;       // sphere terms per (atom, frame), or nullptr: r (float-rounded),
     const double *sct, *sst, *ssp, *scp;  // cos theta, sin theta, sin phi, cos phi
     const float *cphi, *cz;  // cylinder terms (r in srf), float-rounded as the reference stages them
+    const double2 *cj01;     // j_0, j_1 (sphere) or J_0, J_1 (cylinder) per (shell, atom, frame), or nullptr
 };
 
 // j_l(x) for the moment degrees ls[0..kMoments) of a wavefront in one
@@ -290,18 +291,67 @@ __global__ __launch_bounds__(256) void cylinder_terms_kernel(const float *__rest
     zz[e] = (float)zc;
 }
 
+// j_0(x), j_1(x) and 1/x (0, 1, 0 at x = 0) from the argument's sin and cos.
+__device__ __forceinline__ void sph_bessel01(double x, double &j0, double &j1, double &ix) {
+    double sx = 0.0, cx = 1.0;
+    ix = 0.0;
+    j0 = 1.0;
+    j1 = 0.0;
+    if (x != 0.0) {
+        sincos(x, &sx, &cx);
+        ix = 1.0 / x;
+        j0 = sx * ix;
+        j1 = (j0 - cx) * ix;
+    }
+}
+
+// j_0 and j_1 of |q| r for every (shell, atom, frame) of a sphere call, from
+// the sphere terms (r rounded to float, as the amplitude kernel forms its
+// argument): formed once per call instead of once per moment block.
+__global__ __launch_bounds__(256) void sphere_bessel01_kernel(const float *__restrict__ rr, size_t n,
+                                                              const double *__restrict__ qinfo, int ns,
+                                                              double2 *__restrict__ out) {
+    const size_t e = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= n) return;
+    const double rf = rr[e];
+    for (int s = 0; s < ns; ++s) {
+        double j0, j1, ix;
+        sph_bessel01(qinfo[s] * rf, j0, j1, ix);
+        out[(size_t)s * n + e] = make_double2(j0, j1);
+    }
+}
+
+// J_0 and J_1 of q_r r for every (shell, atom, frame) of a cylinder call, from
+// the cylinder terms (r rounded to float, as the amplitude kernel forms p):
+// formed once per call instead of once per moment block (the library
+// Bessel functions took a third of the amplitude kernel's time).
+__global__ __launch_bounds__(256) void cylinder_bessel01_kernel(const float *__restrict__ rr, size_t n,
+                                                                const double *__restrict__ qinfo, int ns,
+                                                                double2 *__restrict__ out) {
+    const size_t e = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= n) return;
+    const double rf = rr[e];
+    for (int s = 0; s < ns; ++s) {
+        const double p = rf * qinfo[3 * s];
+        out[(size_t)s * n + e] = make_double2(j0(p), j1(p));
+    }
+}
+
 // KIND 0 sphere, 1 cylinder, each instantiated on its own.  Both are bound by
 // the latency of serial FP64 recurrences, so they are held to 3 (sphere) and
 // 2 (cylinder) wavefronts per SIMD, the bounds that measured fastest despite
 // some spilled registers (1 wavefront per SIMD took 1.6x (sphere) and 1.8x
 // (cylinder) as long).
-// PRE: the spherical or cylinder terms of every (atom, frame) come from
-// sphere_terms_kernel / cylinder_terms_kernel (a.srf ...); else they are
-// formed here, whose acos, atan and sincos would otherwise add their
-// registers to the common path.
-template <int KIND, bool PRE>
+// SRC 0: the spherical or cylinder coordinates of every (atom, frame) are
+// formed here; 1: they come from sphere_terms_kernel / cylinder_terms_kernel
+// (a.srf ...); 2 (cylinder): as 1, and J_0, J_1 of every (shell, atom, frame)
+// come from cylinder_bessel01_kernel (a.cj01).  Each is its own
+// instantiation, so the acos, atan, sincos and Bessel code of the paths a
+// kernel does not take adds nothing to its registers or code.
+template <int KIND, int SRC>
 __global__ __launch_bounds__(256, KIND == 0 ? 3 : 2) void multipole_kernel(MultipoleKernelArgs a) {
     constexpr int kind = KIND;
+    constexpr bool PRE = SRC >= 1;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     // block = (frame tile, shell, group of 4 moment blocks + zgroups x atom range)
@@ -379,14 +429,16 @@ __global__ __launch_bounds__(256, KIND == 0 ? 3 : 2) void multipole_kernel(Multi
             } else {
                 sphere_terms(x, y, z, rf, ct, somx2, sp, cp);
             }
-            // the argument's sin/cos, j_0, j_1 and 1/x once for all moments
+            // j_0, j_1 and 1/x of the argument once for all moments
             const double xq = ql * rf;
-            double sx = 0.0, cx = 1.0, ix = 0.0, j0 = 1.0, j1 = 0.0;
-            if (xq != 0.0) {
-                sincos(xq, &sx, &cx);
-                ix = 1.0 / xq;
-                j0 = sx * ix;
-                j1 = (j0 - cx) * ix;
+            double ix = 0.0, j0 = 1.0, j1 = 0.0;
+            if constexpr (SRC == 2) {
+                const double2 jj = a.cj01[((size_t)s * a.NAp + j) * a.NFp + t];
+                j0 = jj.x;
+                j1 = jj.y;
+                if (xq != 0.0) ix = 1.0 / xq;
+            } else {
+                sph_bessel01(xq, j0, j1, ix);
             }
             double jv[kMoments];
             if (sorted)
@@ -462,8 +514,17 @@ __global__ __launch_bounds__(256, KIND == 0 ? 3 : 2) void multipole_kernel(Multi
             double exs, exc;
             sincos(sgn * zf * qz, &exs, &exc);
             const double p = rf * qr, psi = phif - qphi;
+            double J0, J1;
+            if constexpr (SRC == 2) {
+                const double2 jj = a.cj01[((size_t)s * a.NAp + j) * a.NFp + t];
+                J0 = jj.x;
+                J1 = jj.y;
+            } else {
+                J0 = j0(p);
+                J1 = j1(p);
+            }
             double jn[kMoments];
-            cyl_bessel_orders(cn, nhi, p, j0(p), j1(p), jn);
+            cyl_bessel_orders(cn, nhi, p, J0, J1, jn);
             double sn = 0.0, cs = 1.0;
 #pragma unroll
             for (int k = 0; k < kMoments; ++k) {
@@ -523,6 +584,7 @@ hipError_t launch_multipole(const MultipoleArgs &m, hipStream_t stream) {
     k.srf = nullptr;
     k.sct = k.sst = k.ssp = k.scp = nullptr;
     k.cphi = k.cz = nullptr;
+    k.cj01 = static_cast<const double2 *>(m.cj01);
     if (m.sph && m.kind == 0) {
         k.srf = static_cast<const float *>(m.sph);
         const double *d = reinterpret_cast<const double *>(static_cast<const char *>(m.sph) + sizeof(float) * n);
@@ -537,20 +599,24 @@ hipError_t launch_multipole(const MultipoleArgs &m, hipStream_t stream) {
     }
     if ((size_t)k.zgroups * k.split > 65535) return hipErrorInvalidValue;
     dim3 grid((unsigned)(m.NFp / 64), (unsigned)m.ns, (unsigned)(k.zgroups * k.split));
-    if (m.kind == 0 && k.srf)
-        hipLaunchKernelGGL((multipole_kernel<0, true>), grid, dim3(256), 0, stream, k);
+    if (m.kind == 0 && k.srf && k.cj01)
+        hipLaunchKernelGGL((multipole_kernel<0, 2>), grid, dim3(256), 0, stream, k);
+    else if (m.kind == 0 && k.srf)
+        hipLaunchKernelGGL((multipole_kernel<0, 1>), grid, dim3(256), 0, stream, k);
     else if (m.kind == 0)
-        hipLaunchKernelGGL((multipole_kernel<0, false>), grid, dim3(256), 0, stream, k);
+        hipLaunchKernelGGL((multipole_kernel<0, 0>), grid, dim3(256), 0, stream, k);
+    else if (k.srf && k.cj01)
+        hipLaunchKernelGGL((multipole_kernel<1, 2>), grid, dim3(256), 0, stream, k);
     else if (k.srf)
-        hipLaunchKernelGGL((multipole_kernel<1, true>), grid, dim3(256), 0, stream, k);
+        hipLaunchKernelGGL((multipole_kernel<1, 1>), grid, dim3(256), 0, stream, k);
     else
-        hipLaunchKernelGGL((multipole_kernel<1, false>), grid, dim3(256), 0, stream, k);
+        hipLaunchKernelGGL((multipole_kernel<1, 0>), grid, dim3(256), 0, stream, k);
     return hipGetLastError();
 }
 
 hipError_t multipole_blocks_per_cu(int kind, int *blocks) {
-    return kind == 0 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, multipole_kernel<0, true>, 256, 0)
-                     : hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, multipole_kernel<1, true>, 256, 0);
+    return kind == 0 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, multipole_kernel<0, 2>, 256, 0)
+                     : hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, multipole_kernel<1, 2>, 256, 0);
 }
 
 int multipole_base_blocks(size_t NFp, int ns, int nmo) {
@@ -574,6 +640,17 @@ hipError_t launch_multipole_cylinder_terms(const MultipoleArgs &m, void *terms, 
     const float *X = m.coords, *Y = m.coords + n, *Z = m.coords + 2 * n;
     hipLaunchKernelGGL(cylinder_terms_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, X, Y, Z, n,
                        m.base, rr, rr + n, rr + 2 * n);
+    return hipGetLastError();
+}
+
+hipError_t launch_multipole_bessel01(const MultipoleArgs &m, void *out, hipStream_t stream) {
+    const size_t n = m.NAp * m.NFp;
+    if (m.kind == 0)
+        hipLaunchKernelGGL(sphere_bessel01_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream,
+                           static_cast<const float *>(m.sph), n, m.qinfo, m.ns, static_cast<double2 *>(out));
+    else
+        hipLaunchKernelGGL(cylinder_bessel01_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream,
+                           static_cast<const float *>(m.sph), n, m.qinfo, m.ns, static_cast<double2 *>(out));
     return hipGetLastError();
 }
 
