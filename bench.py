@@ -42,14 +42,15 @@ ALG_TRANS = 2.0
 # Executed per term on linearly spaced shells (the stepped-phasor path): two
 # FMAs step cos and sin to the next shell and two add b cos, b sin to the
 # sums; shells of a reference q scan (residual-corrected steps) add two FMAs
-# of the correction sums.  The FP64 phase and four hardware sin/cos per
-# (orientation, atom, frame) are shared by a tile's shells.
+# of the correction sums, except in tiles whose shells lie in one exact run
+# of the scan (sx_timing.shell_exact_runs).  The FP64 phase and four hardware
+# sin/cos per (orientation, atom, frame) are shared by a tile's shells.
 EXEC_FLOPS_STEPPED = 8.0
 EXEC_FLOPS_RESIDUAL = 12.0
 # HBM bytes per amplitude launch on this workload (tile 1 x 20, 4 atom
 # ranges), from rocprofv3 --pmc FETCH_SIZE (x2.0) and WRITE_SIZE (x1.0), the
 # factors a 1 GiB device copy reads for the same counters on MI355X.
-AMP_TRAFFIC_BYTES = 1.97e9 + 0.524e9
+AMP_TRAFFIC_BYTES = 2.467e9 + 0.524e9
 
 
 # FP64 flops per (atom, frame, moment, shell) term of the multipole amplitude
@@ -311,7 +312,8 @@ def bench_c4(args):
     terms = tm["terms"]
     amp_s = amp_ms * 1e-3
     legs = roofline_legs(terms, amp_s)
-    ex_flops = (EXEC_FLOPS_STEPPED * tm["terms_stepped"] + EXEC_FLOPS_RESIDUAL * tm["terms_residual"]
+    ex_flops = (EXEC_FLOPS_STEPPED * (tm["terms_stepped"] + tm["terms_exact_runs"])
+                + EXEC_FLOPS_RESIDUAL * (tm["terms_residual"] - tm["terms_exact_runs"])
                 + ALG_FP32_FLOPS * (terms - tm["terms_stepped"] - tm["terms_residual"]))
     out = {
         "metric": "atom*frame*q-vector terms/s (64 uneven coherent trajectories, one graph launch)",
@@ -324,7 +326,8 @@ def bench_c4(args):
         "config": {"workload": "64 uneven coherent trajectories (batched graph, persistent amplitude kernel)",
                    "trajectories": len(trajs), "terms_per_step": terms, "tiles": tm["tiles"],
                    "tile_shape": tm["tile_shape"], "scale": args.c4_scale, "generation_s": gen_s,
-                   "terms_stepped_exact": tm["terms_stepped"], "terms_stepped_residual": tm["terms_residual"]},
+                   "terms_stepped_exact": tm["terms_stepped"], "terms_stepped_residual": tm["terms_residual"],
+                   "terms_residual_in_exact_runs": tm["terms_exact_runs"]},
         "graph_ms": tm["total_ms"],
         "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
                        "min": float(np.min(lat)), "max": float(np.max(lat))},
@@ -472,11 +475,13 @@ def main():
                             "transcendentals, 4 FP32 flops; the FP64 leg binds",
             "legs": legs}
     if stepped:
-        ex = EXEC_FLOPS_RESIDUAL if residual else EXEC_FLOPS_STEPPED
+        exact_shells = tm.get("shell_exact_runs", 0) if residual else T.nshells
+        ex = (EXEC_FLOPS_STEPPED * exact_shells + EXEC_FLOPS_RESIDUAL * (T.nshells - exact_shells)) / T.nshells
         roof["executed"] = {
             "algorithm": "stepped phasors across linearly spaced shells" +
-                         (" with first-order corrections of the scan's single-precision residuals" if residual
-                          else ""),
+                         (" with first-order corrections of the scan's single-precision residuals, except "
+                          "in tiles whose shells form one exact run" if residual else ""),
+            "shells_exact_runs": exact_shells if residual else None,
             "flops_per_term": ex, "achieved": ex * terms_per_launch / amp_avg_s / 1e12, "unit": "TFLOP/s",
             "frac_of_unpacked_fp32": ex * terms_per_launch / amp_avg_s / (FP32_PEAK / 2),
             "tile_shape": tm.get("tile_shape"),

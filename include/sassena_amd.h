@@ -96,6 +96,9 @@ typedef struct sx_timing {
     int shell_residuals;   /* oriented shells: 1 if that progression held only up to small residuals
                               (a reference q scan rounded by single-precision powf,
                               parameters.cpp:1151), corrected to first order per shell, else 0 */
+    int shell_exact_runs;  /* oriented shells with residuals: shells of the last launch whose tiles lie
+                              in one exact progression run of the scan and were stepped without
+                              corrections */
 } sx_timing;
 
 const char *sx_version(void);
@@ -207,6 +210,8 @@ typedef struct sx_batch_timing {
                              5 mixed (1x8, 1x16 and 1x20 tiles chosen per trajectory) */
     size_t terms_stepped;  /* terms of trajectories whose shells step phasors (exact progression) */
     size_t terms_residual; /* terms of trajectories stepped with residual corrections */
+    size_t terms_exact_runs; /* of those, terms of tiles whose shells lie in one exact run of the scan,
+                                stepped without corrections */
 } sx_batch_timing;
 
 int sx_batch_create(const sx_config *cfg, sx_batch **out);

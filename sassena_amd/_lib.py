@@ -51,7 +51,8 @@ class sx_timing(ctypes.Structure):
                 ("finalize_ms", ctypes.c_double), ("total_ms", ctypes.c_double),
                 ("amplitude_launches", ctypes.c_size_t), ("correlate_launches", ctypes.c_size_t),
                 ("terms", ctypes.c_size_t), ("pad_length", ctypes.c_size_t), ("tile_shape", ctypes.c_int),
-                ("shell_progression", ctypes.c_int), ("shell_residuals", ctypes.c_int)]
+                ("shell_progression", ctypes.c_int), ("shell_residuals", ctypes.c_int),
+                ("shell_exact_runs", ctypes.c_int)]
 
 
 _lib = None

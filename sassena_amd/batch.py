@@ -24,7 +24,7 @@ class sx_trajectory(ctypes.Structure):
 class sx_batch_timing(ctypes.Structure):
     _fields_ = [("total_ms", ctypes.c_double), ("amplitude_ms", ctypes.c_double), ("terms", ctypes.c_size_t),
                 ("tiles", ctypes.c_size_t), ("tile_shape", ctypes.c_int), ("terms_stepped", ctypes.c_size_t),
-                ("terms_residual", ctypes.c_size_t)]
+                ("terms_residual", ctypes.c_size_t), ("terms_exact_runs", ctypes.c_size_t)]
 
 
 def _bind(L):

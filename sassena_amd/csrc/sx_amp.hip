@@ -202,20 +202,40 @@ __global__ __launch_bounds__(256) void amp_all_kernel(AmpKernelArgs a) {
 }
 
 
+// Work item of an oriented launch: XCD x = blockIdx & 7 takes an equal
+// share of the (frame tile, atom range, orientation group) items of every
+// shell tile, the shell tiles one after another.  At any time the CUs then
+// run tiles of one shell group: with the shell tile varying fastest, the
+// dispatch order tied each CU to the items of one shell tile, so tiles of
+// unequal cost (exactly stepped and corrected ones, a partial last tile) left
+// some CUs idle while others finished theirs.
+struct OrientItem {
+    int ft, sp, og, sg;
+};
+
+__device__ __forceinline__ bool orient_item(const OrientKernelArgs &a, OrientItem &w) {
+    const int nsub = a.ftiles * a.split * a.ogroups;
+    const int per = (nsub + 7) >> 3;
+    const int local = (int)(blockIdx.x >> 3);
+    w.sg = local / per;
+    const int r = (int)(blockIdx.x & 7) * per + (local - w.sg * per);
+    if (w.sg >= a.sgroups || r >= nsub) return false;
+    w.og = r % a.ogroups;
+    const int rest = r / a.ogroups;
+    w.sp = rest % a.split;
+    w.ft = rest / a.split;
+    return true;
+}
+
+static int oriented_grid(int ftiles, int split, int ogroups, int sgroups) {
+    return ((ftiles * split * ogroups + 7) / 8) * 8 * sgroups;
+}
+
 template <int VM, int VS, bool PER_ROW_B>
 __global__ __launch_bounds__(256, 2) void amp_oriented_kernel(OrientKernelArgs a) {
-    const int nitems = a.ftiles * a.split * a.ogroups * a.sgroups;
-    const int per = (nitems + 7) >> 3;
-    const int item = (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
-    if (item >= nitems) return;
-    const int sg = item % a.sgroups;
-    int rest = item / a.sgroups;
-    const int og = rest % a.ogroups;
-    rest /= a.ogroups;
-    const int sp = rest % a.split;
-    const int ft = rest / a.split;
-
-    oriented_tile<VM, VS, PER_ROW_B>(a, ft, sp, og, sg * VS);
+    OrientItem w;
+    if (!orient_item(a, w)) return;
+    oriented_tile<VM, VS, PER_ROW_B>(a, w.ft, w.sp, w.og, w.sg * VS);
 }
 
 // Shells that are an arithmetic progression up to small residuals (mode 2 of
@@ -224,17 +244,12 @@ __global__ __launch_bounds__(256, 2) void amp_oriented_kernel(OrientKernelArgs a
 template <int VM, int VS>
 __global__ __launch_bounds__(256, 2) void amp_oriented_res_kernel(OrientKernelArgs a) {
     extern __shared__ double tot_lds[];
-    const int nitems = a.ftiles * a.split * a.ogroups * a.sgroups;
-    const int per = (nitems + 7) >> 3;
-    const int item = (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
-    if (item >= nitems) return;
-    const int sg = item % a.sgroups;
-    int rest = item / a.sgroups;
-    const int og = rest % a.ogroups;
-    rest /= a.ogroups;
-    const int sp = rest % a.split;
-    const int ft = rest / a.split;
-    oriented_tile_body<VM, VS, false, 2, true>(a, ft, sp, og, sg * VS, tot_lds);
+    OrientItem w;
+    if (!orient_item(a, w)) return;
+    if (exact_run<VS>(a, w.sg * VS))
+        oriented_tile_body<VM, VS, false, 3, true>(a, w.ft, w.sp, w.og, w.sg * VS, tot_lds);
+    else
+        oriented_tile_body<VM, VS, false, 2, true>(a, w.ft, w.sp, w.og, w.sg * VS, tot_lds);
 }
 
 // Oriented tile shapes (orientations x shells per wavefront), picked by the
@@ -290,6 +305,7 @@ OrientKernelArgs make_orient_args(const OrientArgs &a) {
     k.qstep = a.qstep;
     k.q0 = a.q0;
     k.qres = a.qres;
+    k.runs = a.ap == 2 ? a.runs : nullptr;
     return k;
 }
 
@@ -307,10 +323,28 @@ int shell_progression(const double *qlen, int ns, double rmax, ShellGrid *g) {
     }
     g->q0 = qlen[0];
     g->step = d;
-    if (emax <= 1e-12 * qmax) {
+    const double tol = 1e-12 * qmax;
+    if (emax <= tol) {
         g->mode = 1;
     } else if (emax <= kMaxShellResidual * qmax && emax * rmax <= kMaxResidualPhase / kTwoPiTurns) {
         g->mode = 2;
+        g->runs.resize(ns);
+        for (int s = 0; s < ns; ++s) {
+            // extend the run from s while every shell inside stays on the line
+            // through its end points
+            int last = s;
+            double step = 0.0;
+            for (int e = s + 1; e < std::min(ns, s + kMaxRunShells); ++e) {
+                const double st = (qlen[e] - qlen[s]) / (double)(e - s);
+                bool on = true;
+                for (int k = s + 1; k < e && on; ++k)
+                    on = std::fabs(qlen[k] - (qlen[s] + (double)(k - s) * st)) <= tol;
+                if (!on) break;
+                last = e;
+                step = st;
+            }
+            g->runs[s] = ShellRun{step, last, 0};
+        }
     } else {
         g->res.clear();
         g->step = 0.0;
@@ -321,8 +355,7 @@ int shell_progression(const double *qlen, int ns, double rmax, ShellGrid *g) {
 hipError_t launch_amp_oriented(const OrientArgs &a, hipStream_t stream) {
     if (a.shape < 0 || a.shape >= kOrientShapes) return hipErrorInvalidValue;
     const OrientKernelArgs k = make_orient_args(a);
-    const int nitems = k.ftiles * k.split * k.ogroups * k.sgroups;
-    const int blocks = ((nitems + 7) / 8) * 8;
+    const int blocks = oriented_grid(k.ftiles, k.split, k.ogroups, k.sgroups);
     const bool per_row = a.per_shell_b != 0;
     switch (a.shape) {
         case 0: return launch_oriented_shape<2, 8>(k, per_row, blocks, stream);

@@ -24,6 +24,7 @@ struct OrientKernelArgs {
     double qstep;
     double q0;             // |q_0| / 2 pi of the ideal grid (ap == 2)
     const float *qres;     // [ns] residuals of the shells against the ideal grid (ap == 2)
+    const ShellRun *runs;  // [ns] exact runs of shells (ap == 2), or null
 };
 
 // Workgroup LDS of the residual-corrected tile: the FP64 totals of the 4
@@ -44,7 +45,10 @@ OrientKernelArgs make_orient_args(const OrientArgs &a);
 // neglected term (r_s d)^2 / 2 being bounded by the host.  The correction
 // sums g_s = sum_j b_j d_j e^{i q~_s d_j} run beside the amplitude sums and
 // enter them, scaled by r_s, at every fold; the FP64 totals then live in LDS
-// (`tot`, kResidualLds bytes) to leave the registers to g_s.
+// (`tot`, kResidualLds bytes) to leave the registers to g_s.  MODE 3: a tile
+// of a mode-2 call whose shells form one exact run (a.runs[sbase]): stepped
+// as MODE 1 with the run's step from the tile's first shell, totals in LDS as
+// MODE 2, so that both share one kernel's registers.
 //
 // Atom sums run in FP32 over sub-chunks of CH atoms (coordinates loaded
 // together) and are folded into FP64 totals every kFold atoms.
@@ -62,12 +66,20 @@ __device__ __forceinline__ const __attribute__((address_space(4))) T *uniform_pt
     return (const __attribute__((address_space(4))) T *)p;
 }
 
+// Whether the shells sbase..sbase+VS-1 (those below ns) of a mode-2 tile lie
+// in one exact run, so that the tile is stepped without corrections (MODE 3).
+template <int VS>
+__device__ __forceinline__ bool exact_run(const OrientKernelArgs &a, int sbase) {
+    return a.runs != nullptr && uniform_ptr(a.runs)[sbase].last >= min(sbase + VS, a.ns) - 1;
+}
+
 template <int VM, int VS, bool PER_ROW_B, int MODE, bool BUF>
 __device__ __forceinline__ void oriented_tile_body(const OrientKernelArgs &a, int ft, int sp, int og, int sbase,
                                                    double *__restrict__ tot) {
     constexpr bool AP = MODE >= 1;
     constexpr bool RES = MODE == 2;
-    static_assert(!(RES && PER_ROW_B), "residual-corrected tiles take one factor per atom");
+    constexpr bool LT = MODE >= 2;  // FP64 totals in LDS
+    static_assert(!(LT && PER_ROW_B), "residual-corrected tiles take one factor per atom");
     constexpr int CH = 16;     // atoms per coordinate batch
     constexpr int kFold = 64;  // atoms per FP32 partial sum
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -79,6 +91,7 @@ __device__ __forceinline__ void oriented_tile_body(const OrientKernelArgs &a, in
     const auto QL = uniform_ptr(a.qlen);
     const auto QR = uniform_ptr(a.qres);
     const auto B = uniform_ptr(a.b);
+    const double qstep = MODE == 3 ? uniform_ptr(a.runs)[sbase].step : a.qstep;
     double ux[VM], uy[VM], uz[VM], ql[AP ? 1 : VS];
 #pragma unroll
     for (int i = 0; i < VM; ++i) {
@@ -95,20 +108,20 @@ __device__ __forceinline__ void oriented_tile_body(const OrientKernelArgs &a, in
         for (int k = 0; k < VS; ++k) qr[k] = QR[min(sbase + k, a.ns - 1)];
     } else if constexpr (AP) {
         // the tile's middle shell, which need not exist in a partial last tile
-        ql[0] = QL[min(sbase, a.ns - 1)] + (double)(VS / 2) * a.qstep;
+        ql[0] = QL[min(sbase, a.ns - 1)] + (double)(VS / 2) * qstep;
     } else {
 #pragma unroll
         for (int k = 0; k < VS; ++k) ql[k] = QL[min(sbase + k, a.ns - 1)];
     }
 
     // FP64 totals: registers, or this wavefront's slice of LDS ([VM*VS][2][64])
-    double tr[RES ? 1 : VM][RES ? 1 : VS], ti[RES ? 1 : VM][RES ? 1 : VS];
-    double *wt = RES ? tot + (size_t)wave * (VM * VS * 2 * kWave) + lane : nullptr;
+    double tr[LT ? 1 : VM][LT ? 1 : VS], ti[LT ? 1 : VM][LT ? 1 : VS];
+    double *wt = LT ? tot + (size_t)wave * (VM * VS * 2 * kWave) + lane : nullptr;
 #pragma unroll
     for (int i = 0; i < VM; ++i)
 #pragma unroll
         for (int k = 0; k < VS; ++k) {
-            if constexpr (RES) {
+            if constexpr (LT) {
                 wt[((i * VS + k) * 2) * kWave] = 0.0;
                 wt[((i * VS + k) * 2 + 1) * kWave] = 0.0;
             } else {
@@ -200,7 +213,7 @@ __device__ __forceinline__ void oriented_tile_body(const OrientKernelArgs &a, in
                         // order, n being within 1e-6 of 1).
                         constexpr int C = VS / 2;
                         const float fc = (float)__builtin_amdgcn_fract(ql[0] * d);
-                        const float fh = 0.5f * (float)__builtin_amdgcn_fract(a.qstep * d);
+                        const float fh = 0.5f * (float)__builtin_amdgcn_fract(qstep * d);
                         const float sh = __builtin_amdgcn_sinf(fh), ch = __builtin_amdgcn_cosf(fh);
                         const float rn = 2.f - fmaf(sh, sh, ch * ch);
                         const bool near0 = sh < 0.70710678f;
@@ -277,6 +290,9 @@ __device__ __forceinline__ void oriented_tile_body(const OrientKernelArgs &a, in
                     // sum b e^{i q~ d} (1 + i r d) = (cr - r gr) + i (ci + r gi)
                     wt[((i * VS + k) * 2) * kWave] += (double)fmaf(-qr[k], gr[i][k], cr[i][k]);
                     wt[((i * VS + k) * 2 + 1) * kWave] += (double)fmaf(qr[k], gi[i][k], ci[i][k]);
+                } else if constexpr (LT) {
+                    wt[((i * VS + k) * 2) * kWave] += (double)cr[i][k];
+                    wt[((i * VS + k) * 2 + 1) * kWave] += (double)ci[i][k];
                 } else {
                     tr[i][k] += (double)cr[i][k];
                     ti[i][k] += (double)ci[i][k];
@@ -292,7 +308,7 @@ __device__ __forceinline__ void oriented_tile_body(const OrientKernelArgs &a, in
             if (m < a.NM && sh < a.ns) {
                 const size_t v = (size_t)sh * a.NM + m;
                 double xr, xi;
-                if constexpr (RES) {
+                if constexpr (LT) {
                     xr = wt[((i * VS + k) * 2) * kWave];
                     xi = wt[((i * VS + k) * 2 + 1) * kWave];
                 } else {

@@ -63,9 +63,12 @@ __global__ __launch_bounds__(256, 2) void amp_persistent_kernel(const OrientKern
         __syncthreads();  // every thread has read cur before it is overwritten
         if (it >= n) break;
         const int4 w = items[begin + it];
-        if constexpr (RES)
-            oriented_tile_body<VM, VS, false, 2, true>(targs[w.x], w.y, 0, w.z & 0xffff, w.z >> 16, tot_lds);
-        else
+        if constexpr (RES) {
+            if (exact_run<VS>(targs[w.x], w.z >> 16))
+                oriented_tile_body<VM, VS, false, 3, true>(targs[w.x], w.y, 0, w.z & 0xffff, w.z >> 16, tot_lds);
+            else
+                oriented_tile_body<VM, VS, false, 2, true>(targs[w.x], w.y, 0, w.z & 0xffff, w.z >> 16, tot_lds);
+        } else
             oriented_tile<VM, VS, false, false>(targs[w.x], w.y, 0, w.z & 0xffff, w.z >> 16);
     }
 }
@@ -148,6 +151,7 @@ struct TrajWork {
     sx_ctx *ctx = nullptr;  // staged coordinates, factors and FFT tables
     DevBuf qlen, u, qres, amp, ppart, apart, fqt, fq, fq2;
     int mode = 0;          // shell stepping mode (sx::shell_progression)
+    std::vector<sx::ShellRun> runs;  // exact runs of the shells (mode 2)
     double qstep = 0.0;    // step and origin of the shells in |q| / 2 pi
     double q0 = 0.0;
     int chunks = 1;
@@ -169,6 +173,7 @@ struct Plan {
     // each by tile shape (largest first)
     std::vector<Queue> queues;
     int nitems = 0, nfitems = 0;
+    size_t terms_exact_runs = 0;  // terms of mode-2 tiles that lie in one exact run
     size_t final_lds = 0;
     struct CorrGroup {
         int E, T, first, count;
@@ -239,7 +244,8 @@ sx::OrientKernelArgs orient_args(const TrajWork &w, const sx_trajectory &t, int 
     oa.ap = w.mode;
     oa.qstep = w.qstep;
     oa.q0 = w.q0;
-    oa.qres = w.qres.as<float>();
+    oa.runs = w.mode == 2 ? w.qres.as<sx::ShellRun>() : nullptr;
+    oa.qres = w.mode == 2 ? reinterpret_cast<const float *>(oa.runs + t.nshells) : nullptr;
     return sx::make_orient_args(oa);
 }
 
@@ -292,6 +298,7 @@ int build_plan(sx_batch *b, const std::vector<int> &ks, int shape, Plan &P) {
     const int args_shape = mixed ? 4 : shape;
     const int VS = sx::orient_shape_vs(args_shape);
     std::vector<sx::OrientKernelArgs> oka(b->trajs.size());
+    P.terms_exact_runs = 0;
     struct Item {
         int4 w;
         long cost;
@@ -312,6 +319,12 @@ int build_plan(sx_batch *b, const std::vector<int> &ks, int shape, Plan &P) {
         // tile cost in atom shell slots (x 1000 in kSlotCost units)
         for (size_t g = 0; g < groups.size(); ++g) {
             const int code = groups[g].code;
+            if (res) {
+                const int first = groups[g].first, n = std::min(mixed ? sx::kMixedVS[code] : VS,
+                                                                (int)t.nshells - first);
+                if (b->work[k].runs[first].last >= first + n - 1)
+                    P.terms_exact_runs += (size_t)n * t.NM * t.NA * t.NF;
+            }
             const long cost = mixed ? (long)(oka[k].atoms_per_split * sx::kMixedVS[code] *
                                              kSlotCost[res ? 1 : 0][code] * 1000.0)
                                     : (long)oka[k].atoms_per_split;
@@ -578,13 +591,18 @@ int sx_batch_stage(sx_batch *b, const sx_trajectory *trajs, size_t ntraj) {
         for (size_t s = 0; s < t.nshells; ++s) ql[s] = t.qlen[s] / kTwoPi;
         sx::ShellGrid g;
         w.mode = sx::shell_progression(ql.data(), (int)t.nshells, w.ctx->rmax, &g);
+        w.runs = g.runs;
         w.qstep = g.step;
         w.q0 = g.q0;
         if (w.mode == 2) {
+            // [ns] exact runs, then [ns] residuals in radians per Angstrom
             std::vector<float> rr(t.nshells);
-            for (size_t s = 0; s < t.nshells; ++s) rr[s] = (float)(kTwoPi * g.res[s]);  // radians per Angstrom
-            SXB_HIP(b, w.qres.ensure(sizeof(float) * t.nshells));
-            SXB_HIP(b, hipMemcpy(w.qres.p, rr.data(), sizeof(float) * t.nshells, hipMemcpyHostToDevice));
+            for (size_t s = 0; s < t.nshells; ++s) rr[s] = (float)(kTwoPi * g.res[s]);
+            const size_t rb = sizeof(sx::ShellRun) * t.nshells;
+            SXB_HIP(b, w.qres.ensure(sx::shell_table_bytes((int)t.nshells)));
+            SXB_HIP(b, hipMemcpy(w.qres.p, g.runs.data(), rb, hipMemcpyHostToDevice));
+            SXB_HIP(b, hipMemcpy(static_cast<char *>(w.qres.p) + rb, rr.data(), sizeof(float) * t.nshells,
+                                 hipMemcpyHostToDevice));
         }
         w.chunks = (int)std::min<size_t>(t.NM, 8);
         SXB_HIP(b, w.qlen.ensure(sizeof(double) * t.nshells));
@@ -628,6 +646,7 @@ int sx_batch_stage(sx_batch *b, const sx_trajectory *trajs, size_t ntraj) {
         if (b->work[k].mode == 1) b->timing.terms_stepped += n;
         if (b->work[k].mode == 2) b->timing.terms_residual += n;
     }
+    b->timing.terms_exact_runs = b->all->terms_exact_runs;
     b->timing.terms = terms;
     rc = capture(b, *b->all);
     if (rc != SX_OK) return rc;

@@ -246,6 +246,15 @@ int default_shape(const sx_ctx *c, int ns, bool per_shell) {
     return ns > 16 ? 4 : ns > 8 ? 3 : 0;
 }
 
+// Shells of a residual-corrected launch with VS shells per tile whose tiles
+// lie in one exact run (the tiles the kernel steps without corrections).
+int exact_run_shells(const std::vector<sx::ShellRun> &runs, int ns, int VS) {
+    int n = 0;
+    for (int s0 = 0; s0 < ns && (size_t)ns <= runs.size(); s0 += VS)
+        if (runs[s0].last >= std::min(s0 + VS, ns) - 1) n += std::min(VS, ns - s0);
+    return n;
+}
+
 // OrientArgs of the current call (stepping mode and tables from run_batch).
 sx::OrientArgs oriented_args(const sx_ctx *c, int ns, int NM, const float *bptr, bool per_shell, int shape,
                              int split) {
@@ -265,7 +274,9 @@ sx::OrientArgs oriented_args(const sx_ctx *c, int ns, int NM, const float *bptr,
     oa.ap = c->q_mode;
     oa.qstep = c->q_step;
     oa.q0 = c->q_origin;
-    oa.qres = c->qres.as<float>();
+    // mode 2: [ns] exact runs, then [ns] residuals
+    oa.runs = c->q_mode == 2 ? c->qres.as<sx::ShellRun>() : nullptr;
+    oa.qres = c->q_mode == 2 ? reinterpret_cast<const float *>(oa.runs + ns) : nullptr;
     return oa;
 }
 
@@ -361,11 +372,16 @@ int run_batch(sx_ctx *c, int ns, size_t NM, const double *qv, const double *qlen
         if (c->q_mode == 2 && b_shell) c->q_mode = 0;
         c->q_step = g.step;
         c->q_origin = g.q0;
+        c->q_runs = g.runs;
         if (c->q_mode == 2) {
+            // [ns] exact runs, then [ns] residuals in radians per Angstrom
             std::vector<float> rr(ns);
-            for (int s = 0; s < ns; ++s) rr[s] = (float)(kTwoPi * g.res[s]);  // radians per Angstrom
-            SX_HIP(c, c->qres.ensure(sizeof(float) * ns));
-            SX_HIP(c, hipMemcpyAsync(c->qres.p, rr.data(), sizeof(float) * ns, hipMemcpyHostToDevice, c->stream));
+            for (int s = 0; s < ns; ++s) rr[s] = (float)(kTwoPi * g.res[s]);
+            const size_t rb = sizeof(sx::ShellRun) * ns;
+            SX_HIP(c, c->qres.ensure(sx::shell_table_bytes(ns)));
+            SX_HIP(c, hipMemcpyAsync(c->qres.p, g.runs.data(), rb, hipMemcpyHostToDevice, c->stream));
+            SX_HIP(c, hipMemcpyAsync(static_cast<char *>(c->qres.p) + rb, rr.data(), sizeof(float) * ns,
+                                     hipMemcpyHostToDevice, c->stream));
         }
         SX_HIP(c, c->qlen.ensure(sizeof(double) * ns));
         SX_HIP(c, hipMemcpyAsync(c->qlen.p, ql.data(), sizeof(double) * ns, hipMemcpyHostToDevice, c->stream));
@@ -492,6 +508,7 @@ int run_batch(sx_ctx *c, int ns, size_t NM, const double *qv, const double *qlen
             c->timing.tile_shape = shape + 1;
             c->timing.shell_progression = oa.ap ? 1 : 0;
             c->timing.shell_residuals = oa.ap == 2 ? 1 : 0;
+            c->timing.shell_exact_runs = oa.ap == 2 ? exact_run_shells(c->q_runs, ns, sx::orient_shape_vs(shape)) : 0;
         } else {
             sx::AmpArgs aa;
             aa.coords = c->coords.as<float>();
@@ -1059,7 +1076,7 @@ size_t noalloc_batch_bytes(sx_ctx *c, int ns, const double *qlen, size_t NM, boo
         sizeof(double) * 3 * V,                                   // qrev
         sizeof(double) * ns,                                      // qlen
         sizeof(double) * 3 * NM,                                  // uvec
-        mode == 2 ? sizeof(float) * ns : 0,                       // qres
+        mode == 2 ? sx::shell_table_bytes(ns) : 0,                // qres (runs and residuals)
         per_shell ? sizeof(float) * ns * c->NAp : 0,              // bshell
         per_shell ? sizeof(int) * V : 0,                          // brow
         sizeof(float2) * V * c->NFp * split,                      // amp

@@ -85,6 +85,19 @@ hipError_t launch_amp_all(const AmpArgs &a, hipStream_t stream);
 // Workgroups of amp_all_kernel with vb vectors per wavefront that fit one CU.
 hipError_t all_blocks_per_cu(int vb, bool per_row, int *blocks);
 
+// An exact run of shells from one shell s: shells s..last (at most
+// kMaxRunShells) form an arithmetic progression to 1e-12 relative with this
+// step.  A tile of a residual-corrected call whose shells lie in one run is
+// stepped exactly, without the first-order corrections.  A reference scan's
+// powf points form such runs wherever the rounding of j / (points - 1) keeps
+// one offset (for 40 points from 0.1 to 4.0: shells 10-19 and 20-39).
+struct ShellRun {
+    double step;   // turns per Angstrom
+    int last;      // last shell of the run from this one
+    int pad;
+};
+constexpr int kMaxRunShells = 32;
+
 // Oriented shells: v = s * NM + m has q_v = |q_s| u_m.  NAp must be a multiple
 // of 16 (it is a multiple of 64).
 struct OrientArgs {
@@ -105,6 +118,7 @@ struct OrientArgs {
     double qstep = 0.0;
     double q0 = 0.0;
     const float *qres = nullptr;
+    const ShellRun *runs = nullptr;  // [ns] exact runs within the grid (ap == 2), or null
     // Atoms summed (the first natoms of the NAp rows); 0: all NAp.  The
     // autotuner times shapes on a prefix of the atoms.
     size_t natoms = 0;
@@ -122,7 +136,10 @@ struct ShellGrid {
     int mode = 0;
     double q0 = 0.0, step = 0.0;    // turns per Angstrom
     std::vector<double> res;        // [ns], turns per Angstrom (mode 2)
+    std::vector<ShellRun> runs;     // [ns] exact runs from each shell (mode 2)
 };
+// Device bytes of the mode-2 shell tables: [ns] ShellRun, then [ns] float residuals.
+inline size_t shell_table_bytes(int ns) { return (sizeof(ShellRun) + sizeof(float)) * (size_t)ns; }
 constexpr double kMaxShellResidual = 1e-6;
 constexpr double kMaxResidualPhase = 1e-3;
 constexpr double kTwoPiTurns = 6.283185307179586476925286766559;

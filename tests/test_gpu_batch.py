@@ -64,9 +64,11 @@ def test_batch_against_oracle(oracle_mod, shape):
 def test_mixed_tiles_against_oracle(oracle_mod, kind):
     """Shell counts whose cheapest cover mixes 1x8, 1x16 and 1x20 tiles with a
     partial last tile (7: 8; 10: 16; 21 and 23: 16 + 8; 35: 20 + 16; 44:
-    20 + 16 + 8), the shells at random |q| (one phase each), on an exact grid
-    (stepped phasors) or from a single-precision scan (stepped with residual
-    corrections; no count has n - 1 a power of two, where the scan is exact):
+    20 + 16 + 8; 40: 20 + 20), the shells at random |q| (one phase each), on
+    an exact grid (stepped phasors) or from a single-precision scan (stepped
+    with residual corrections; no count has n - 1 a power of two, where the
+    scan is exact; the 40-point scan's shells 20-39 form one exact run, whose
+    tile is stepped without corrections):
     every shell against the FP64 oracle, and the whole result against fixed
     1 x 20 tiles (to the oracle's bound: the two step phasors from different
     middle shells, and their rounding differs at 1e-6)."""
@@ -74,7 +76,7 @@ def test_mixed_tiles_against_oracle(oracle_mod, kind):
     from sassena_amd import synth
     rng = np.random.default_rng(71)
     trajs = []
-    for k, ns in enumerate((7, 10, 21, 23, 35, 44)):
+    for k, ns in enumerate((7, 10, 21, 23, 35, 40, 44)):
         frames, b = small_trajectory(int(rng.integers(60, 200)), int(rng.integers(100, 400)), seed=700 + k)
         if kind == "random":
             ql = np.sort(rng.uniform(0.1, 4.0, ns))
@@ -87,7 +89,9 @@ def test_mixed_tiles_against_oracle(oracle_mod, kind):
     with sassena_amd.Batch() as bt:
         bt.stage(staged)
         res = bt.run()
-        assert bt.timing()["tile_shape"] == 5
+        tmb = bt.timing()
+        assert tmb["tile_shape"] == 5
+        assert (tmb["terms_exact_runs"] > 0) == (kind == "scan")
     with sassena_amd.Batch(tile_shape=5) as bt:
         bt.stage(staged)
         fixed = bt.run()

@@ -335,7 +335,10 @@ def test_scan_shells_residual_steps(sx, oracle_mod, shape):
     grid): the stepped-phasor path with first-order residual corrections,
     on every tile shape, full-size headline trajectory, two orientations;
     and a list perturbed by 4e-7 relative (residual phases 20x larger) on
-    the same path."""
+    the same path.  The scan's powf points form exact runs (shells 10-19 and
+    20-39 each lie on one arithmetic progression), and tiles inside a run are
+    stepped without corrections: 16, 28, 16, 8 and 20 shells for the shapes
+    2x8, 4x4, 1x8, 1x16 and 1x20; none on the perturbed list."""
     from sassena_amd import synth
     T = synth.case("c2")
     frames = synth.to_frames_aos(T.xyz)
@@ -343,6 +346,7 @@ def test_scan_shells_residual_steps(sx, oracle_mod, shape):
     scan = synth.linear_shells(0.1, 4.0, 40)
     assert np.max(np.abs(scan - synth.exact_linear_shells(0.1, 4.0, 40))) > 1e-8
     bent = synth.exact_linear_shells(0.1, 4.0, 40) * (1 + 4e-7 * np.cos(np.arange(40)))
+    exact_shells = {1: 16, 2: 28, 3: 16, 4: 8, 5: 20}[shape]
     refs = {}
     for qlen in (scan, bent):
         with sx.ScatterDevice("all", tile_shape=shape) as dev:
@@ -351,7 +355,8 @@ def test_scan_shells_residual_steps(sx, oracle_mod, shape):
             fqt, fq, fq2 = dev.compute_oriented(qlen, u)
             tm = dev.timing()
             assert tm["shell_progression"] == 1 and tm["shell_residuals"] == 1
-        for s in (1, 5, 9, 13, 20, 27, 33, 38):
+            assert tm["shell_exact_runs"] == (exact_shells if qlen is scan else 0)
+        for s in (1, 5, 9, 13, 17, 20, 27, 33, 38, 39):
             key = float(qlen[s])
             if key not in refs:
                 refs[key] = oracle_mod.compute("all", frames, T.b, qlen[s] * u, nthreads=8)
