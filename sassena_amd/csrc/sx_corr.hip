@@ -167,7 +167,7 @@ __global__ __launch_bounds__(512) void corr_self_kernel(CorrArgs a) {
 
 // One half (H) of every pair of a chunk: the pair loop of corr_self_kernel
 // with a single sweep per pair.
-template <typename C, int E, int H>
+template <typename C, int E, int H, typename PT>
 __device__ __forceinline__ void self_half_body(const CorrArgs &a, double2 *__restrict__ apair, int s, int c,
                                                C *buf) {
     using Acc = typename Cx<C>::R;
@@ -199,7 +199,10 @@ __device__ __forceinline__ void self_half_body(const CorrArgs &a, double2 *__res
         double ar = 0.0, ai = 0.0;
         int t = tid;
         asm volatile("" : "+v"(t));
-        fft_sweep<C, 1, E>(P, H, v, acc, buf, a.twp, a.twL, a.w, a.NF, in, ar, ai, t);
+        if constexpr (std::is_void<PT>::value)
+            fft_sweep<C, 1, E>(P, H, v, acc, buf, a.twp, a.twL, a.w, a.NF, in, ar, ai, t);
+        else
+            fft_sweep_fixed<C, E, PT, H>(v[0], acc, buf, a.twp, a.w, in, ar, ai, (unsigned)t);
         // per-wavefront part of a; the parts are summed per pair in fixed order
         // by self_a_reduce (no block barrier here)
 #pragma unroll
@@ -218,7 +221,8 @@ __device__ __forceinline__ void self_half_body(const CorrArgs &a, double2 *__res
     if (since > 0 || first) flush_power_half<E, H>(P, acc, row, first, tid);
 }
 
-template <typename C, int E>
+// PT: void for the runtime plan, else a StaticPlan the launch's plan matches.
+template <typename C, int E, typename PT = void>
 // E = 20 (halves of 5,120 points): four workgroups per CU, as many as the
 // LDS allows, so at most 128 VGPRs.
 __global__ __launch_bounds__(256, E == 20 ? 4 : 1) void corr_self_split_kernel(CorrArgs a, double2 *apair) {
@@ -227,9 +231,9 @@ __global__ __launch_bounds__(256, E == 20 ? 4 : 1) void corr_self_split_kernel(C
     const int sc = blockIdx.x >> 1;
     const int s = sc / a.chunks, c = sc % a.chunks;
     if (blockIdx.x & 1)
-        self_half_body<C, E, 1>(a, apair, s, c, buf);
+        self_half_body<C, E, 1, PT>(a, apair, s, c, buf);
     else
-        self_half_body<C, E, 0>(a, apair, s, c, buf);
+        self_half_body<C, E, 0, PT>(a, apair, s, c, buf);
 }
 
 // Tail lags of the self path: one wavefront per pair; lane l holds x_l and
@@ -476,8 +480,9 @@ size_t corr_all_lds_bytes(const FftPlan &P) { return corr_lds_bytes<double2, 1>(
 
 using SelfSplitFn = void (*)(CorrArgs, double2 *);
 
-static SelfSplitFn self_split_fn(int E) {
-    switch (E) {
+static SelfSplitFn self_split_fn(const FftPlan &P) {
+    if (Plan5120::matches(P)) return corr_self_split_kernel<float2, 20, Plan5120>;
+    switch (P.E) {
         case 4: return corr_self_split_kernel<float2, 4>;
         case 8: return corr_self_split_kernel<float2, 8>;
         case 16: return corr_self_split_kernel<float2, 16>;
@@ -489,7 +494,7 @@ static SelfSplitFn self_split_fn(int E) {
 
 hipError_t self_split_blocks_per_cu(const FftPlan &P, int *blocks) {
     *blocks = 0;
-    const SelfSplitFn k = self_split_fn(P.E);
+    const SelfSplitFn k = self_split_fn(P);
     if (!k || P.T > 256) return hipErrorInvalidValue;
     const size_t lds = corr_lds_bytes<float2, 1>(P);
     hipError_t e = hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
@@ -498,7 +503,7 @@ hipError_t self_split_blocks_per_cu(const FftPlan &P, int *blocks) {
 }
 
 hipError_t launch_corr_self_split(const CorrArgs &a, double2 *apair, hipStream_t stream) {
-    const SelfSplitFn k = self_split_fn(a.plan.E);
+    const SelfSplitFn k = self_split_fn(a.plan);
     if (!k) return hipErrorInvalidValue;
     if (a.plan.T > 256) return hipErrorInvalidValue;  // the split kernel is built for 256-thread blocks
     const size_t lds = corr_lds_bytes<float2, 1>(a.plan);

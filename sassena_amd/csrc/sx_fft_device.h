@@ -321,6 +321,142 @@ __device__ __forceinline__ void fft_sweep(const FftPlan &P, int h0, C (&v)[NH][E
     }
 }
 
+// ------------------------------------------- passes of a fixed plan ------
+// The same Stockham sweep (one half, NH = 1) for a plan known at compile
+// time: strides, Ns and twiddle offsets become constants, so every LDS
+// access is the lane's base address plus an immediate offset and the integer
+// index arithmetic of the runtime plan disappears.  The twiddle table is the
+// runtime plan's (StaticPlan::matches checks that the layouts agree).
+template <unsigned LH, unsigned TT, int... RS>
+struct StaticPlan {
+    static constexpr unsigned Lh = LH, T = TT;
+    static constexpr int npass = sizeof...(RS);
+    static constexpr int kRadix[npass] = {RS...};
+    static constexpr int radix(int p) { return kRadix[p]; }
+    static constexpr unsigned ns(int p) {
+        unsigned n = 1;
+        for (int i = 0; i < p; ++i) n *= (unsigned)kRadix[i];
+        return n;
+    }
+    static constexpr unsigned toff(int p) {
+        unsigned o = 0;
+        for (int i = 1; i < p; ++i) o += ns(i) * (unsigned)kRadix[i];
+        return o;
+    }
+    static bool matches(const FftPlan &P) {
+        if (P.Lh != (int)Lh || P.T != (int)T || P.npass != npass || P.N1 != 1) return false;
+        for (int i = 0; i < npass; ++i)
+            if (P.radix[i] != kRadix[i] || P.ns[i] != (int)ns(i) || (i > 0 && P.toff[i] != (int)toff(i)))
+                return false;
+        return true;
+    }
+};
+
+template <typename C, int E, typename PT, typename Input>
+__device__ __forceinline__ void spass_first(C (&v)[E], C *buf, const Input &in, unsigned ut) {
+    constexpr int R = PT::radix(0);
+    constexpr unsigned T = PT::T, stride = PT::Lh / R;
+    static_assert(E % R == 0 && Input::kBounded, "first pass of a fixed plan");
+    // the coordinate loads of half the pass's points in flight before the
+    // first is used, those of the other half issued once the first half is
+    // being consumed (all at once: 128 VGPRs and a spill)
+    float xs[E], ys[E], zs[E];
+#pragma unroll
+    for (int e = 0; e < E / 2; ++e) in.load(ut + (e / R) * T + (e % R) * stride, xs[e], ys[e], zs[e]);
+#pragma unroll
+    for (int i = 0; i < E / R; ++i) {
+        if (i == (E / R) / 2) {
+#pragma unroll
+            for (int e = E / 2; e < E; ++e) in.load(ut + (e / R) * T + (e % R) * stride, xs[e], ys[e], zs[e]);
+        }
+#pragma unroll
+        for (int q = 0; q < R; ++q)
+            v[i * R + q] = in.at(ut + i * T + q * stride, xs[i * R + q], ys[i * R + q], zs[i * R + q]);
+        Dft<R, C>::run(&v[i * R]);
+#pragma unroll
+        for (int q = 0; q < R; ++q) buf[ut * R + i * T * R + q] = v[i * R + q];
+    }
+    __syncthreads();
+}
+
+template <typename C, int E, typename PT, int p>
+__device__ __forceinline__ void spass_mid(C (&v)[E], C *buf, const C *__restrict__ twp, unsigned ut) {
+    constexpr int R = PT::radix(p);
+    constexpr unsigned T = PT::T, Ns = PT::ns(p), stride = PT::Lh / R, toff = PT::toff(p);
+    static_assert(E % R == 0 && Ns <= T, "middle pass of a fixed plan");
+    // every group of the thread has the twiddle index kk = ut mod Ns
+    const unsigned kk = ut & (Ns - 1);
+    C ws[R];
+#pragma unroll
+    for (int q = 1; q < R; ++q) ws[q] = twiddle_load(twp, kk * R, toff + q);
+#pragma unroll
+    for (int i = 0; i < E / R; ++i)
+#pragma unroll
+        for (int q = 0; q < R; ++q) v[i * R + q] = buf[ut + i * T + q * stride];
+    __syncthreads();
+    const unsigned ob = (ut - kk) * R + kk;
+#pragma unroll
+    for (int i = 0; i < E / R; ++i) {
+#pragma unroll
+        for (int q = 1; q < R; ++q) v[i * R + q] = cmul(v[i * R + q], ws[q]);
+        Dft<R, C>::run(&v[i * R]);
+#pragma unroll
+        for (int q = 0; q < R; ++q) buf[ob + i * T * R + q * Ns] = v[i * R + q];
+    }
+    __syncthreads();
+}
+
+template <typename C, int E, typename PT, int H, typename Acc>
+__device__ __forceinline__ void spass_last(C (&v)[E], Acc (&acc)[2][E], const C *buf, const C *__restrict__ twp,
+                                           const double2 *__restrict__ w, double &ar, double &ai, unsigned ut) {
+    constexpr int p = PT::npass - 1, R = PT::radix(p);
+    constexpr unsigned T = PT::T, stride = PT::Lh / R, toff = PT::toff(p);
+    static_assert(E % R == 0 && PT::ns(p) == stride, "last pass of a fixed plan");
+#pragma unroll
+    for (int i = 0; i < E / R; ++i)
+#pragma unroll
+        for (int q = 0; q < R; ++q) v[i * R + q] = buf[ut + i * T + q * stride];
+    __syncthreads();  // the buffer may now be overwritten by the next series
+#pragma unroll
+    for (int i = 0; i < E / R; ++i) {
+#pragma unroll
+        for (int q = 1; q < R; ++q)
+            v[i * R + q] = cmul(v[i * R + q], twiddle_load(twp, ut * R, toff + q + i * T * R));
+        Dft<R, C>::run(&v[i * R]);
+#pragma unroll
+        for (int q = 0; q < R; ++q) {
+            const C x = v[i * R + q];
+            const typename Cx<C>::R pw = x.x * x.x + x.y * x.y;
+            acc[H][i * R + q] += (Acc)pw;
+            const double2 wf = weight_load(w, ut, ((unsigned)H * R + q) * stride + i * T);
+            ar = fma(wf.x, (double)pw, ar);
+            ai = fma(wf.y, (double)pw, ai);
+        }
+    }
+}
+
+template <typename C, int E, typename PT, int p>
+__device__ __forceinline__ void spass_mids(C (&v)[E], C *buf, const C *__restrict__ twp, unsigned ut) {
+    if constexpr (p < PT::npass - 1) {
+        spass_mid<C, E, PT, p>(v, buf, twp, ut);
+        spass_mids<C, E, PT, p + 1>(v, buf, twp, ut);
+    }
+}
+
+// One half H of a padded series through the fixed plan PT.
+template <typename C, int E, typename PT, int H, typename Acc, typename Input>
+__device__ __forceinline__ void fft_sweep_fixed(C (&v)[E], Acc (&acc)[2][E], C *buf, const C *__restrict__ tw,
+                                                const double2 *__restrict__ w, const Input &in, double &ar,
+                                                double &ai, unsigned ut) {
+    static_assert(H == 0 || Input::kPretwiddled, "the odd-bin half needs its input pre-twiddled");
+    spass_first<C, E, PT>(v, buf, in, ut);
+    spass_mids<C, E, PT, 1>(v, buf, tw, ut);
+    spass_last<C, E, PT, H>(v, acc, buf, tw, w, ar, ai, ut);
+}
+
+// The self path's 5,120-point halves (NF up to 5,120 frames): 4^5 x 5 on 256 threads.
+using Plan5120 = StaticPlan<5120, 256, 4, 4, 4, 4, 4, 5>;
+
 // |X_f|^2 of one padded series accumulated into acc; a += sum_f w_f |X_f|^2.
 // tw: packed pass twiddles; pre: exp(-2 pi i n / L) for the odd bins.
 template <typename C, int NH, int E, typename Acc, typename Input>
@@ -523,15 +659,19 @@ struct SelfInput : SelfRows {
     static constexpr bool kBounded = true;  // 0 past NF without a branch
     double qx, qy, qz;
     float b;
-    __device__ __forceinline__ float2 operator()(unsigned n) const {
-        float xs, ys, zs;
-        load(n, xs, ys, zs);
+    // the sample at frame n from its loaded coordinates
+    __device__ __forceinline__ float2 at(unsigned n, float xs, float ys, float zs) const {
         double p = qx * (double)xs;
         p = fma(qy, (double)ys, p);
         p = fma(qz, (double)zs, p);
         const float f = (float)__builtin_amdgcn_fract(p);
         const float bn = n < nf ? b : 0.f;
         return make_float2(bn * __builtin_amdgcn_cosf(f), bn * __builtin_amdgcn_sinf(f));
+    }
+    __device__ __forceinline__ float2 operator()(unsigned n) const {
+        float xs, ys, zs;
+        load(n, xs, ys, zs);
+        return at(n, xs, ys, zs);
     }
 };
 
@@ -542,15 +682,18 @@ struct SelfInputShifted : SelfRows {
     static constexpr bool kBounded = true;
     double qx, qy, qz, invL;
     float b;
-    __device__ __forceinline__ float2 operator()(unsigned n) const {
-        float xs, ys, zs;
-        load(n, xs, ys, zs);
+    __device__ __forceinline__ float2 at(unsigned n, float xs, float ys, float zs) const {
         double p = fma(-(double)n, invL, qx * (double)xs);
         p = fma(qy, (double)ys, p);
         p = fma(qz, (double)zs, p);
         const float f = (float)__builtin_amdgcn_fract(p);
         const float bn = n < nf ? b : 0.f;
         return make_float2(bn * __builtin_amdgcn_cosf(f), bn * __builtin_amdgcn_sinf(f));
+    }
+    __device__ __forceinline__ float2 operator()(unsigned n) const {
+        float xs, ys, zs;
+        load(n, xs, ys, zs);
+        return at(n, xs, ys, zs);
     }
 };
 
