@@ -185,6 +185,9 @@ __device__ __forceinline__ void self_half_body(const CorrArgs &a, double2 *__res
     bool first = true;
     int since = 0;
     using In = typename std::conditional<H == 0, SelfInput, SelfInputShifted>::type;
+    using MT = typename std::conditional<std::is_void<PT>::value, int, MidTwiddles<C, PT>>::type;
+    MT mt{};
+    if constexpr (!std::is_void<PT>::value) mt.load(a.twp, (unsigned)tid);
     for (long pr = p0; pr < p1; ++pr) {
         const int ja = (int)(pr / a.NM), m = (int)(pr % a.NM);
         const int j = a.aidx ? a.aidx[ja] : ja;
@@ -202,7 +205,7 @@ __device__ __forceinline__ void self_half_body(const CorrArgs &a, double2 *__res
         if constexpr (std::is_void<PT>::value)
             fft_sweep<C, 1, E>(P, H, v, acc, buf, a.twp, a.twL, a.w, a.NF, in, ar, ai, t);
         else
-            fft_sweep_fixed<C, E, PT, H>(v[0], acc, buf, a.twp, a.w, in, ar, ai, (unsigned)t);
+            fft_sweep_fixed<C, E, PT, H>(v[0], acc, buf, a.twp, mt, a.w, in, ar, ai, (unsigned)t);
         // per-wavefront part of a; the parts are summed per pair in fixed order
         // by self_a_reduce (no block barrier here)
 #pragma unroll

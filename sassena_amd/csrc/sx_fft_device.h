@@ -379,16 +379,33 @@ __device__ __forceinline__ void spass_first(C (&v)[E], C *buf, const Input &in, 
     __syncthreads();
 }
 
-template <typename C, int E, typename PT, int p>
-__device__ __forceinline__ void spass_mid(C (&v)[E], C *buf, const C *__restrict__ twp, unsigned ut) {
-    constexpr int R = PT::radix(p);
-    constexpr unsigned T = PT::T, Ns = PT::ns(p), stride = PT::Lh / R, toff = PT::toff(p);
-    static_assert(E % R == 0 && Ns <= T, "middle pass of a fixed plan");
-    // every group of the thread has the twiddle index kk = ut mod Ns
-    const unsigned kk = ut & (Ns - 1);
-    C ws[R];
+// The twiddles of the middle passes of a fixed plan for one thread: every
+// group of the thread has the index kk = ut mod Ns in pass p, so each pass
+// needs R - 1 of them, the same for every series; loaded once per kernel and
+// kept in registers.
+template <typename C, typename PT>
+struct MidTwiddles {
+    C w[PT::npass][8];
+    template <int p>
+    __device__ __forceinline__ void load_from(const C *__restrict__ twp, unsigned ut) {
+        if constexpr (p < PT::npass - 1) {
+            constexpr int R = PT::radix(p);
+            const unsigned kk = ut & (PT::ns(p) - 1);
 #pragma unroll
-    for (int q = 1; q < R; ++q) ws[q] = twiddle_load(twp, kk * R, toff + q);
+            for (int q = 1; q < R; ++q) w[p][q] = twiddle_load(twp, kk * R, PT::toff(p) + q);
+            load_from<p + 1>(twp, ut);
+        }
+    }
+    __device__ __forceinline__ void load(const C *__restrict__ twp, unsigned ut) { load_from<1>(twp, ut); }
+};
+
+template <typename C, int E, typename PT, int p>
+__device__ __forceinline__ void spass_mid(C (&v)[E], C *buf, const MidTwiddles<C, PT> &mt, unsigned ut) {
+    constexpr int R = PT::radix(p);
+    constexpr unsigned T = PT::T, Ns = PT::ns(p), stride = PT::Lh / R;
+    static_assert(E % R == 0 && Ns <= T, "middle pass of a fixed plan");
+    const unsigned kk = ut & (Ns - 1);
+    const C(&ws)[8] = mt.w[p];
 #pragma unroll
     for (int i = 0; i < E / R; ++i)
 #pragma unroll
@@ -436,21 +453,21 @@ __device__ __forceinline__ void spass_last(C (&v)[E], Acc (&acc)[2][E], const C 
 }
 
 template <typename C, int E, typename PT, int p>
-__device__ __forceinline__ void spass_mids(C (&v)[E], C *buf, const C *__restrict__ twp, unsigned ut) {
+__device__ __forceinline__ void spass_mids(C (&v)[E], C *buf, const MidTwiddles<C, PT> &mt, unsigned ut) {
     if constexpr (p < PT::npass - 1) {
-        spass_mid<C, E, PT, p>(v, buf, twp, ut);
-        spass_mids<C, E, PT, p + 1>(v, buf, twp, ut);
+        spass_mid<C, E, PT, p>(v, buf, mt, ut);
+        spass_mids<C, E, PT, p + 1>(v, buf, mt, ut);
     }
 }
 
 // One half H of a padded series through the fixed plan PT.
 template <typename C, int E, typename PT, int H, typename Acc, typename Input>
 __device__ __forceinline__ void fft_sweep_fixed(C (&v)[E], Acc (&acc)[2][E], C *buf, const C *__restrict__ tw,
-                                                const double2 *__restrict__ w, const Input &in, double &ar,
-                                                double &ai, unsigned ut) {
+                                                const MidTwiddles<C, PT> &mt, const double2 *__restrict__ w,
+                                                const Input &in, double &ar, double &ai, unsigned ut) {
     static_assert(H == 0 || Input::kPretwiddled, "the odd-bin half needs its input pre-twiddled");
     spass_first<C, E, PT>(v, buf, in, ut);
-    spass_mids<C, E, PT, 1>(v, buf, tw, ut);
+    spass_mids<C, E, PT, 1>(v, buf, mt, ut);
     spass_last<C, E, PT, H>(v, acc, buf, tw, w, ar, ai, ut);
 }
 
