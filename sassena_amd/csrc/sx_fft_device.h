@@ -423,12 +423,19 @@ __device__ __forceinline__ void spass_mid(C (&v)[E], C *buf, const MidTwiddles<C
     __syncthreads();
 }
 
+// The last pass's twiddles exp(-2 pi i q (ut + i T) / Lh) are the thread's
+// exp(-2 pi i q ut / Lh) (R - 1 loads, issued before the pass's barrier)
+// times constants of the plan.
 template <typename C, int E, typename PT, int H, typename Acc>
 __device__ __forceinline__ void spass_last(C (&v)[E], Acc (&acc)[2][E], const C *buf, const C *__restrict__ twp,
                                            const double2 *__restrict__ w, double &ar, double &ai, unsigned ut) {
     constexpr int p = PT::npass - 1, R = PT::radix(p);
-    constexpr unsigned T = PT::T, stride = PT::Lh / R, toff = PT::toff(p);
+    constexpr unsigned T = PT::T, stride = PT::Lh / R;
     static_assert(E % R == 0 && PT::ns(p) == stride, "last pass of a fixed plan");
+    using Rl = typename Cx<C>::R;
+    C base[R];
+#pragma unroll
+    for (int q = 1; q < R; ++q) base[q] = twiddle_load(twp, ut * R, PT::toff(p) + q);
 #pragma unroll
     for (int i = 0; i < E / R; ++i)
 #pragma unroll
@@ -437,8 +444,17 @@ __device__ __forceinline__ void spass_last(C (&v)[E], Acc (&acc)[2][E], const C 
 #pragma unroll
     for (int i = 0; i < E / R; ++i) {
 #pragma unroll
-        for (int q = 1; q < R; ++q)
-            v[i * R + q] = cmul(v[i * R + q], twiddle_load(twp, ut * R, toff + q + i * T * R));
+        for (int q = 1; q < R; ++q) {
+            // exp(-2 pi i q i T / Lh), a constant of the plan
+            constexpr double kTurn = -6.283185307179586476925286766559 / (double)PT::Lh;
+            C t = base[q];
+            if (i > 0) {
+                const Rl cr = (Rl)__builtin_cos(kTurn * (double)(q * i * T));
+                const Rl ci = (Rl)__builtin_sin(kTurn * (double)(q * i * T));
+                t = cmul(t, Cx<C>::make(cr, ci));
+            }
+            v[i * R + q] = cmul(v[i * R + q], t);
+        }
         Dft<R, C>::run(&v[i * R]);
 #pragma unroll
         for (int q = 0; q < R; ++q) {
