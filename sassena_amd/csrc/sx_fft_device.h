@@ -468,11 +468,92 @@ __device__ __forceinline__ void spass_last(C (&v)[E], Acc (&acc)[2][E], const C 
     }
 }
 
-template <typename C, int E, typename PT, int p>
+template <typename C, int E, typename PT, int p, int END>
 __device__ __forceinline__ void spass_mids(C (&v)[E], C *buf, const MidTwiddles<C, PT> &mt, unsigned ut) {
-    if constexpr (p < PT::npass - 1) {
+    if constexpr (p < END) {
         spass_mid<C, E, PT, p>(v, buf, mt, ut);
-        spass_mids<C, E, PT, p + 1>(v, buf, mt, ut);
+        spass_mids<C, E, PT, p + 1, END>(v, buf, mt, ut);
+    }
+}
+
+constexpr int inverse_mod(int a, int m) {
+    for (int x = 1; x < m; ++x)
+        if ((a * x) % m == 1) return x;
+    return 0;
+}
+constexpr int gcd_int(int a, int b) { return b == 0 ? a : gcd_int(b, a % b); }
+
+// Whether the last two passes of PT (radices RA and RB coprime, the first of
+// them on Ns = T, so that each thread holds one group of RA RB = E points)
+// fuse into one radix-RA RB pass in registers.
+template <typename PT, int E>
+constexpr bool fused_last() {
+    if (PT::npass < 3) return false;
+    constexpr int pa = PT::npass - 2, pb = PT::npass - 1;
+    return PT::ns(pa) == PT::T && PT::radix(pa) * PT::radix(pb) == E &&
+           gcd_int(PT::radix(pa), PT::radix(pb)) == 1;
+}
+
+// The last two passes fused: the thread's group of R = RA RB points
+// x[ut + q T] times exp(-2 pi i q ut / Lh), then the R-point DFT by the
+// Good-Thomas mapping (input n = (RB n1 + RA n2) mod R, output
+// k = (RB (RB^-1 mod RA) k1 + RA (RA^-1 mod RB) k2) mod R: RB DFTs of RA
+// points, then RA DFTs of RB points, no twiddles between them).  Output k is
+// bin ut + k T of the half; its power is accumulated in the slot the runtime
+// plan's last pass would give it (radix RB, groups i = k mod RA, q = k / RA).
+// Twiddle q = RB a + b is exp(-2 pi i RB a ut / Lh) exp(-2 pi i b ut / Lh):
+// the first factor is pass RA's own twiddle a (kept in registers), the
+// second pass RB's twiddle b at kk = ut (R - 1 ... RB - 1 loads).
+template <typename C, int E, typename PT, int H, typename Acc>
+__device__ __forceinline__ void spass_last_fused(C (&v)[E], Acc (&acc)[2][E], const C *buf,
+                                                 const C *__restrict__ twp, const MidTwiddles<C, PT> &mt,
+                                                 const double2 *__restrict__ w, double &ar, double &ai,
+                                                 unsigned ut) {
+    constexpr int pa = PT::npass - 2, pb = PT::npass - 1;
+    constexpr int RA = PT::radix(pa), RB = PT::radix(pb), R = RA * RB;
+    constexpr unsigned T = PT::T;
+    static_assert(R == E && PT::Lh == T * R, "fused last passes of a fixed plan");
+    C base[RB];
+#pragma unroll
+    for (int b = 1; b < RB; ++b) base[b] = twiddle_load(twp, ut * RB, PT::toff(pb) + b);
+#pragma unroll
+    for (int q = 0; q < R; ++q) v[q] = buf[ut + q * T];
+    __syncthreads();  // the buffer may now be overwritten by the next series
+#pragma unroll
+    for (int a = 0; a < RA; ++a)
+#pragma unroll
+        for (int b = 0; b < RB; ++b) {
+            if (a == 0 && b == 0) continue;
+            const C t = a == 0 ? base[b] : b == 0 ? mt.w[pa][a] : cmul(mt.w[pa][a], base[b]);
+            v[RB * a + b] = cmul(v[RB * a + b], t);
+        }
+    C y[RA][RB];
+#pragma unroll
+    for (int n2 = 0; n2 < RB; ++n2) {
+        C t[RA];
+#pragma unroll
+        for (int n1 = 0; n1 < RA; ++n1) t[n1] = v[(RB * n1 + RA * n2) % R];
+        Dft<RA, C>::run(t);
+#pragma unroll
+        for (int k1 = 0; k1 < RA; ++k1) y[k1][n2] = t[k1];
+    }
+    constexpr int MA = RB * inverse_mod(RB % RA, RA), MB = RA * inverse_mod(RA % RB, RB);
+#pragma unroll
+    for (int k1 = 0; k1 < RA; ++k1) {
+        C z[RB];
+#pragma unroll
+        for (int n2 = 0; n2 < RB; ++n2) z[n2] = y[k1][n2];
+        Dft<RB, C>::run(z);
+#pragma unroll
+        for (int k2 = 0; k2 < RB; ++k2) {
+            const int k = (MA * k1 + MB * k2) % R;
+            const C x = z[k2];
+            const typename Cx<C>::R pw = x.x * x.x + x.y * x.y;
+            acc[H][(k % RA) * RB + k / RA] += (Acc)pw;
+            const double2 wf = weight_load(w, ut, (unsigned)H * PT::Lh + (unsigned)k * T);
+            ar = fma(wf.x, (double)pw, ar);
+            ai = fma(wf.y, (double)pw, ai);
+        }
     }
 }
 
@@ -483,8 +564,13 @@ __device__ __forceinline__ void fft_sweep_fixed(C (&v)[E], Acc (&acc)[2][E], C *
                                                 const Input &in, double &ar, double &ai, unsigned ut) {
     static_assert(H == 0 || Input::kPretwiddled, "the odd-bin half needs its input pre-twiddled");
     spass_first<C, E, PT>(v, buf, in, ut);
-    spass_mids<C, E, PT, 1>(v, buf, mt, ut);
-    spass_last<C, E, PT, H>(v, acc, buf, tw, w, ar, ai, ut);
+    if constexpr (fused_last<PT, E>()) {
+        spass_mids<C, E, PT, 1, PT::npass - 2>(v, buf, mt, ut);
+        spass_last_fused<C, E, PT, H>(v, acc, buf, tw, mt, w, ar, ai, ut);
+    } else {
+        spass_mids<C, E, PT, 1, PT::npass - 1>(v, buf, mt, ut);
+        spass_last<C, E, PT, H>(v, acc, buf, tw, w, ar, ai, ut);
+    }
 }
 
 // The self path's 5,120-point halves (NF up to 5,120 frames): 4^5 x 5 on 256 threads.
