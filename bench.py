@@ -42,8 +42,9 @@ ALG_TRANS = 2.0
 # Executed per term on linearly spaced shells (the stepped-phasor path): two
 # FMAs step cos and sin to the next shell and two add b cos, b sin to the
 # sums; shells of a reference q scan (residual-corrected steps) add two FMAs
-# of the correction sums, except in tiles whose shells lie in one exact run
-# of the scan (sx_timing.shell_exact_runs).  The FP64 phase and four hardware
+# of the correction sums, except on the shells that lie on their tile's line
+# (sx_timing.shell_exact_runs: the middle shells and the walks from them that
+# stay in an exact run of the scan).  The FP64 phase and four hardware
 # sin/cos per (orientation, atom, frame) are shared by a tile's shells.
 EXEC_FLOPS_STEPPED = 8.0
 EXEC_FLOPS_RESIDUAL = 12.0
@@ -480,7 +481,7 @@ def main():
         roof["executed"] = {
             "algorithm": "stepped phasors across linearly spaced shells" +
                          (" with first-order corrections of the scan's single-precision residuals, except "
-                          "in tiles whose shells form one exact run" if residual else ""),
+                          "on the shells that lie on their tile's line" if residual else ""),
             "shells_exact_runs": exact_shells if residual else None,
             "flops_per_term": ex, "achieved": ex * terms_per_launch / amp_avg_s / 1e12, "unit": "TFLOP/s",
             "frac_of_unpacked_fp32": ex * terms_per_launch / amp_avg_s / (FP32_PEAK / 2),

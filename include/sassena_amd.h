@@ -96,9 +96,9 @@ typedef struct sx_timing {
     int shell_residuals;   /* oriented shells: 1 if that progression held only up to small residuals
                               (a reference q scan rounded by single-precision powf,
                               parameters.cpp:1151), corrected to first order per shell, else 0 */
-    int shell_exact_runs;  /* oriented shells with residuals: shells of the last launch whose tiles lie
-                              in one exact progression run of the scan and were stepped without
-                              corrections */
+    int shell_exact_runs;  /* oriented shells with residuals: shells of the last launch stepped without
+                              first-order corrections, because they lie on their tile's line (an exact
+                              progression run of the scan through the tile's middle shell) */
 } sx_timing;
 
 const char *sx_version(void);
@@ -210,8 +210,8 @@ typedef struct sx_batch_timing {
                              5 mixed (1x8, 1x16 and 1x20 tiles chosen per trajectory) */
     size_t terms_stepped;  /* terms of trajectories whose shells step phasors (exact progression) */
     size_t terms_residual; /* terms of trajectories stepped with residual corrections */
-    size_t terms_exact_runs; /* of those, terms of tiles whose shells lie in one exact run of the scan,
-                                stepped without corrections */
+    size_t terms_exact_runs; /* of those, terms of shells stepped without corrections (on their
+                                tile's line, an exact run of the scan through its middle shell) */
 } sx_batch_timing;
 
 int sx_batch_create(const sx_config *cfg, sx_batch **out);

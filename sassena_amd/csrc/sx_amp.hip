@@ -238,18 +238,15 @@ __global__ __launch_bounds__(256, 2) void amp_oriented_kernel(OrientKernelArgs a
     oriented_tile<VM, VS, PER_ROW_B>(a, w.ft, w.sp, w.og, w.sg * VS);
 }
 
-// Shells that are an arithmetic progression up to small residuals (mode 2 of
-// oriented_tile_body); one factor per atom.  The FP64 totals take the
-// dynamic LDS (kResidualLds<VM, VS> bytes).
+// Shells that are an arithmetic progression up to small residuals (modes 2-5
+// of oriented_tile_body, by the tile line's flags); one factor per atom.
+// The FP64 totals take the dynamic LDS (kResidualLds<VM, VS> bytes).
 template <int VM, int VS>
 __global__ __launch_bounds__(256, 2) void amp_oriented_res_kernel(OrientKernelArgs a) {
     extern __shared__ double tot_lds[];
     OrientItem w;
     if (!orient_item(a, w)) return;
-    if (exact_run<VS>(a, w.sg * VS))
-        oriented_tile_body<VM, VS, false, 3, true>(a, w.ft, w.sp, w.og, w.sg * VS, tot_lds);
-    else
-        oriented_tile_body<VM, VS, false, 2, true>(a, w.ft, w.sp, w.og, w.sg * VS, tot_lds);
+    residual_tile<VM, VS>(a, w.ft, w.sp, w.og, w.sg * VS, tot_lds);
 }
 
 // Oriented tile shapes (orientations x shells per wavefront), picked by the
@@ -305,8 +302,63 @@ OrientKernelArgs make_orient_args(const OrientArgs &a) {
     k.qstep = a.qstep;
     k.q0 = a.q0;
     k.qres = a.qres;
-    k.runs = a.ap == 2 ? a.runs : nullptr;
+    k.lines = a.ap == 2 ? a.lines : nullptr;
     return k;
+}
+
+int tile_lines(const double *ql, int ns, double rmax, const ShellGrid &g, const int *firsts, const int *vs,
+               int ntiles, TileLine *lines, float *res) {
+    double qmax = 0.0;
+    for (int s = 0; s < ns; ++s) qmax = std::max(qmax, std::fabs(ql[s]));
+    const double tol = 1e-12 * qmax;
+    double bound = kMaxShellResidual * qmax;
+    if (rmax > 0.0) bound = std::min(bound, kMaxResidualPhase / kTwoPiTurns / rmax);
+    int plain = 0;
+    for (int t = 0; t < ntiles; ++t) {
+        const int first = firsts[t], VS = vs[t], C = VS / 2, n = std::min(VS, ns - first);
+        const int mid = first + C;
+        // candidate steps: the call's, and that of the exact run through the middle
+        double steps[2] = {g.step, 0.0};
+        int nsteps = 1;
+        if (mid < ns && g.runs[mid].last > mid) steps[nsteps++] = g.runs[mid].step;
+        else if (mid - 1 >= first && mid - 1 < ns && g.runs[mid - 1].last >= mid) steps[nsteps++] = g.runs[mid - 1].step;
+        else if (mid >= ns && g.runs[first].last > first) steps[nsteps++] = g.runs[first].step;
+        int best_flags = 4;
+        double best_max = 0.0;
+        TileLine best{};
+        for (int c = 0; c < nsteps; ++c) {
+            const double st = steps[c];
+            const double qm = mid < ns ? ql[mid] : ql[first] + (double)C * st;
+            int flags = 0;
+            double mx = 0.0;
+            for (int k = 0; k < n; ++k) {
+                const double r = std::fabs(ql[first + k] - (qm + (double)(k - C) * st));
+                mx = std::max(mx, r);
+                if (r > tol) flags |= k > C ? kTileCorrectUp : k < C ? kTileCorrectDown : 0;
+            }
+            if (mx > bound) continue;
+            const int pop = (flags & 1) + (flags >> 1);
+            const int bpop = best_flags > 3 ? 3 : (best_flags & 1) + (best_flags >> 1);
+            if (best_flags > 3 || pop < bpop || (pop == bpop && mx < best_max)) {
+                best_flags = flags;
+                best_max = mx;
+                best = TileLine{qm, st, flags, 0};
+            }
+        }
+        if (best_flags > 3) {
+            // the call's ideal grid and its residuals (within the bounds)
+            best = TileLine{g.q0 + (double)mid * g.step, g.step, kTileCorrectUp | kTileCorrectDown, 0};
+        }
+        lines[first] = best;
+        for (int k = 0; k < n; ++k) {
+            const double r = ql[first + k] - (best.qmid + (double)(k - C) * best.step);
+            res[first + k] = std::fabs(r) <= tol ? 0.f : (float)(kTwoPiTurns * r);
+        }
+        if (!(best.flags & kTileCorrectUp)) plain += std::max(0, n - C - 1);
+        if (!(best.flags & kTileCorrectDown)) plain += std::min(n, C);
+        if (n > C) plain += 1;  // the middle shell lies on the line
+    }
+    return plain;
 }
 
 int shell_progression(const double *qlen, int ns, double rmax, ShellGrid *g) {

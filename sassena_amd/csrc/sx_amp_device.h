@@ -24,7 +24,7 @@ struct OrientKernelArgs {
     double qstep;
     double q0;             // |q_0| / 2 pi of the ideal grid (ap == 2)
     const float *qres;     // [ns] residuals of the shells against the ideal grid (ap == 2)
-    const ShellRun *runs;  // [ns] exact runs of shells (ap == 2), or null
+    const TileLine *lines; // [ns] lines of the tiles at their first shells (ap == 2)
 };
 
 // Workgroup LDS of the residual-corrected tile: the FP64 totals of the 4
@@ -45,10 +45,12 @@ OrientKernelArgs make_orient_args(const OrientArgs &a);
 // neglected term (r_s d)^2 / 2 being bounded by the host.  The correction
 // sums g_s = sum_j b_j d_j e^{i q~_s d_j} run beside the amplitude sums and
 // enter them, scaled by r_s, at every fold; the FP64 totals then live in LDS
-// (`tot`, kResidualLds bytes) to leave the registers to g_s.  MODE 3: a tile
-// of a mode-2 call whose shells form one exact run (a.runs[sbase]): stepped
-// as MODE 1 with the run's step from the tile's first shell, totals in LDS as
-// MODE 2, so that both share one kernel's registers.
+// (`tot`, kResidualLds bytes) to leave the registers to g_s.  Modes 2-5 step
+// on the tile's own line (a.lines[sbase]: the middle shell's |q| and a step),
+// the residuals being those against it; the middle shell lies on it, and the
+// walks up and down from it carry corrections or not by the mode: 2 both,
+// 3 neither, 4 the up walk only, 5 the down walk only.  All four keep their
+// totals in LDS, so that they share one kernel's registers.
 //
 // Atom sums run in FP32 over sub-chunks of CH atoms (coordinates loaded
 // together) and are folded into FP64 totals every kFold atoms.
@@ -66,19 +68,18 @@ __device__ __forceinline__ const __attribute__((address_space(4))) T *uniform_pt
     return (const __attribute__((address_space(4))) T *)p;
 }
 
-// Whether the shells sbase..sbase+VS-1 (those below ns) of a mode-2 tile lie
-// in one exact run, so that the tile is stepped without corrections (MODE 3).
-template <int VS>
-__device__ __forceinline__ bool exact_run(const OrientKernelArgs &a, int sbase) {
-    return a.runs != nullptr && uniform_ptr(a.runs)[sbase].last >= min(sbase + VS, a.ns) - 1;
+// The walks of a mode-2 tile that carry corrections (TileLine flags).
+__device__ __forceinline__ int tile_flags(const OrientKernelArgs &a, int sbase) {
+    return uniform_ptr(a.lines)[sbase].flags;
 }
 
 template <int VM, int VS, bool PER_ROW_B, int MODE, bool BUF>
 __device__ __forceinline__ void oriented_tile_body(const OrientKernelArgs &a, int ft, int sp, int og, int sbase,
                                                    double *__restrict__ tot) {
     constexpr bool AP = MODE >= 1;
-    constexpr bool RES = MODE == 2;
-    constexpr bool LT = MODE >= 2;  // FP64 totals in LDS
+    constexpr bool LT = MODE >= 2;  // FP64 totals in LDS, the tile's own line
+    constexpr bool UPC = MODE == 2 || MODE == 4, DNC = MODE == 2 || MODE == 5;
+    constexpr bool RES = UPC || DNC;
     static_assert(!(LT && PER_ROW_B), "residual-corrected tiles take one factor per atom");
     constexpr int CH = 16;     // atoms per coordinate batch
     constexpr int kFold = 64;  // atoms per FP32 partial sum
@@ -91,7 +92,7 @@ __device__ __forceinline__ void oriented_tile_body(const OrientKernelArgs &a, in
     const auto QL = uniform_ptr(a.qlen);
     const auto QR = uniform_ptr(a.qres);
     const auto B = uniform_ptr(a.b);
-    const double qstep = MODE == 3 ? uniform_ptr(a.runs)[sbase].step : a.qstep;
+    const double qstep = LT ? uniform_ptr(a.lines)[sbase].step : a.qstep;
     double ux[VM], uy[VM], uz[VM], ql[AP ? 1 : VS];
 #pragma unroll
     for (int i = 0; i < VM; ++i) {
@@ -101,11 +102,13 @@ __device__ __forceinline__ void oriented_tile_body(const OrientKernelArgs &a, in
         uz[i] = U[3 * m + 2];
     }
     float qr[RES ? VS : 1];
-    if constexpr (RES) {
-        // the tile's middle shell on the ideal grid, and the residuals
-        ql[0] = a.q0 + (double)(sbase + VS / 2) * a.qstep;
+    if constexpr (LT) {
+        // the tile's middle shell on its line, and the residuals against it
+        ql[0] = uniform_ptr(a.lines)[sbase].qmid;
+        if constexpr (RES) {
 #pragma unroll
-        for (int k = 0; k < VS; ++k) qr[k] = QR[min(sbase + k, a.ns - 1)];
+            for (int k = 0; k < VS; ++k) qr[k] = QR[min(sbase + k, a.ns - 1)];
+        }
     } else if constexpr (AP) {
         // the tile's middle shell, which need not exist in a partial last tile
         ql[0] = QL[min(sbase, a.ns - 1)] + (double)(VS / 2) * qstep;
@@ -227,10 +230,6 @@ __device__ __forceinline__ void oriented_tile_body(const OrientKernelArgs &a, in
                         const float bd = RES ? bk[0] * (float)d : 0.f;
                         cr[i][C] = fmaf(bk[C], xc, cr[i][C]);
                         ci[i][C] = fmaf(bk[C], yc, ci[i][C]);
-                        if constexpr (RES) {
-                            gr[i][C] = fmaf(bd, yc, gr[i][C]);
-                            gi[i][C] = fmaf(bd, xc, gi[i][C]);
-                        }
                         {  // shells above the middle
                             float xp = xc, yp = yc;
                             float xk = fmaf(xc, cw, -yc * sw), yk = fmaf(yc, cw, xc * sw);
@@ -238,7 +237,7 @@ __device__ __forceinline__ void oriented_tile_body(const OrientKernelArgs &a, in
                             for (int k = C + 1; k < VS; ++k) {
                                 cr[i][k] = fmaf(bk[k], xk, cr[i][k]);
                                 ci[i][k] = fmaf(bk[k], yk, ci[i][k]);
-                                if constexpr (RES) {
+                                if constexpr (UPC) {
                                     gr[i][k] = fmaf(bd, yk, gr[i][k]);
                                     gi[i][k] = fmaf(bd, xk, gi[i][k]);
                                 }
@@ -258,7 +257,7 @@ __device__ __forceinline__ void oriented_tile_body(const OrientKernelArgs &a, in
                             for (int k = C - 1; k >= 0; --k) {
                                 cr[i][k] = fmaf(bk[k], xk, cr[i][k]);
                                 ci[i][k] = fmaf(bk[k], yk, ci[i][k]);
-                                if constexpr (RES) {
+                                if constexpr (DNC) {
                                     gr[i][k] = fmaf(bd, yk, gr[i][k]);
                                     gi[i][k] = fmaf(bd, xk, gi[i][k]);
                                 }
@@ -286,10 +285,16 @@ __device__ __forceinline__ void oriented_tile_body(const OrientKernelArgs &a, in
         for (int i = 0; i < VM; ++i)
 #pragma unroll
             for (int k = 0; k < VS; ++k) {
+                constexpr int C = VS / 2;
                 if constexpr (RES) {
-                    // sum b e^{i q~ d} (1 + i r d) = (cr - r gr) + i (ci + r gi)
-                    wt[((i * VS + k) * 2) * kWave] += (double)fmaf(-qr[k], gr[i][k], cr[i][k]);
-                    wt[((i * VS + k) * 2 + 1) * kWave] += (double)fmaf(qr[k], gi[i][k], ci[i][k]);
+                    if ((k > C && UPC) || (k < C && DNC)) {
+                        // sum b e^{i q~ d} (1 + i r d) = (cr - r gr) + i (ci + r gi)
+                        wt[((i * VS + k) * 2) * kWave] += (double)fmaf(-qr[k], gr[i][k], cr[i][k]);
+                        wt[((i * VS + k) * 2 + 1) * kWave] += (double)fmaf(qr[k], gi[i][k], ci[i][k]);
+                    } else {
+                        wt[((i * VS + k) * 2) * kWave] += (double)cr[i][k];
+                        wt[((i * VS + k) * 2 + 1) * kWave] += (double)ci[i][k];
+                    }
                 } else if constexpr (LT) {
                     wt[((i * VS + k) * 2) * kWave] += (double)cr[i][k];
                     wt[((i * VS + k) * 2 + 1) * kWave] += (double)ci[i][k];
@@ -320,8 +325,20 @@ __device__ __forceinline__ void oriented_tile_body(const OrientKernelArgs &a, in
         }
 }
 
-// Modes 0 and 1 (the mode is uniform over a launch or a trajectory); mode 2
-// has kernels of its own, since it needs `tot` in LDS.
+// A tile of a residual-corrected call, in the mode its line asks for.
+template <int VM, int VS>
+__device__ __forceinline__ void residual_tile(const OrientKernelArgs &a, int ft, int sp, int og, int sbase,
+                                              double *__restrict__ tot) {
+    switch (tile_flags(a, sbase)) {
+        case 0: oriented_tile_body<VM, VS, false, 3, true>(a, ft, sp, og, sbase, tot); break;
+        case kTileCorrectUp: oriented_tile_body<VM, VS, false, 4, true>(a, ft, sp, og, sbase, tot); break;
+        case kTileCorrectDown: oriented_tile_body<VM, VS, false, 5, true>(a, ft, sp, og, sbase, tot); break;
+        default: oriented_tile_body<VM, VS, false, 2, true>(a, ft, sp, og, sbase, tot); break;
+    }
+}
+
+// Modes 0 and 1 (the mode is uniform over a launch or a trajectory); modes
+// 2-5 have kernels of their own, since they need `tot` in LDS.
 template <int VM, int VS, bool PER_ROW_B, bool BUF = true>
 __device__ __forceinline__ void oriented_tile(const OrientKernelArgs &a, int ft, int sp, int og, int sbase) {
     if (a.ap)

@@ -63,12 +63,9 @@ __global__ __launch_bounds__(256, 2) void amp_persistent_kernel(const OrientKern
         __syncthreads();  // every thread has read cur before it is overwritten
         if (it >= n) break;
         const int4 w = items[begin + it];
-        if constexpr (RES) {
-            if (exact_run<VS>(targs[w.x], w.z >> 16))
-                oriented_tile_body<VM, VS, false, 3, true>(targs[w.x], w.y, 0, w.z & 0xffff, w.z >> 16, tot_lds);
-            else
-                oriented_tile_body<VM, VS, false, 2, true>(targs[w.x], w.y, 0, w.z & 0xffff, w.z >> 16, tot_lds);
-        } else
+        if constexpr (RES)
+            residual_tile<VM, VS>(targs[w.x], w.y, 0, w.z & 0xffff, w.z >> 16, tot_lds);
+        else
             oriented_tile<VM, VS, false, false>(targs[w.x], w.y, 0, w.z & 0xffff, w.z >> 16);
     }
 }
@@ -151,7 +148,8 @@ struct TrajWork {
     sx_ctx *ctx = nullptr;  // staged coordinates, factors and FFT tables
     DevBuf qlen, u, qres, amp, ppart, apart, fqt, fq, fq2;
     int mode = 0;          // shell stepping mode (sx::shell_progression)
-    std::vector<sx::ShellRun> runs;  // exact runs of the shells (mode 2)
+    sx::ShellGrid grid;              // the shells' grid and exact runs (mode 2)
+    std::vector<double> qturns;      // and their |q| / 2 pi
     double qstep = 0.0;    // step and origin of the shells in |q| / 2 pi
     double q0 = 0.0;
     int chunks = 1;
@@ -173,7 +171,7 @@ struct Plan {
     // each by tile shape (largest first)
     std::vector<Queue> queues;
     int nitems = 0, nfitems = 0;
-    size_t terms_exact_runs = 0;  // terms of mode-2 tiles that lie in one exact run
+    size_t terms_exact_runs = 0;  // terms of mode-2 shells stepped without corrections
     size_t final_lds = 0;
     struct CorrGroup {
         int E, T, first, count;
@@ -244,8 +242,9 @@ sx::OrientKernelArgs orient_args(const TrajWork &w, const sx_trajectory &t, int 
     oa.ap = w.mode;
     oa.qstep = w.qstep;
     oa.q0 = w.q0;
-    oa.runs = w.mode == 2 ? w.qres.as<sx::ShellRun>() : nullptr;
-    oa.qres = w.mode == 2 ? reinterpret_cast<const float *>(oa.runs + t.nshells) : nullptr;
+    // mode 2: [ns] tile lines, then [ns] residuals (build_plan)
+    oa.lines = w.mode == 2 ? w.qres.as<sx::TileLine>() : nullptr;
+    oa.qres = w.mode == 2 ? reinterpret_cast<const float *>(oa.lines + t.nshells) : nullptr;
     return sx::make_orient_args(oa);
 }
 
@@ -316,15 +315,28 @@ int build_plan(sx_batch *b, const std::vector<int> &ks, int shape, Plan &P) {
         } else {
             for (int s = 0; s < oka[k].sgroups; ++s) groups.push_back({s * VS, 0});
         }
+        if (res) {
+            // the lines of this trajectory's tiles and the residuals against them
+            TrajWork &w = b->work[k];
+            const int ns = (int)t.nshells;
+            std::vector<int> firsts, vs;
+            for (const ShellGroup &sgp : groups) {
+                firsts.push_back(sgp.first);
+                vs.push_back(mixed ? sx::kMixedVS[sgp.code] : VS);
+            }
+            std::vector<sx::TileLine> lines(ns);
+            std::vector<float> rr(ns, 0.f);
+            const int plain = sx::tile_lines(w.qturns.data(), ns, w.ctx->rmax, w.grid, firsts.data(), vs.data(),
+                                             (int)firsts.size(), lines.data(), rr.data());
+            P.terms_exact_runs += (size_t)plain * t.NM * t.NA * t.NF;
+            const size_t lb = sizeof(sx::TileLine) * ns;
+            SXB_HIP(b, hipMemcpy(w.qres.p, lines.data(), lb, hipMemcpyHostToDevice));
+            SXB_HIP(b, hipMemcpy(static_cast<char *>(w.qres.p) + lb, rr.data(), sizeof(float) * ns,
+                                 hipMemcpyHostToDevice));
+        }
         // tile cost in atom shell slots (x 1000 in kSlotCost units)
         for (size_t g = 0; g < groups.size(); ++g) {
             const int code = groups[g].code;
-            if (res) {
-                const int first = groups[g].first, n = std::min(mixed ? sx::kMixedVS[code] : VS,
-                                                                (int)t.nshells - first);
-                if (b->work[k].runs[first].last >= first + n - 1)
-                    P.terms_exact_runs += (size_t)n * t.NM * t.NA * t.NF;
-            }
             const long cost = mixed ? (long)(oka[k].atoms_per_split * sx::kMixedVS[code] *
                                              kSlotCost[res ? 1 : 0][code] * 1000.0)
                                     : (long)oka[k].atoms_per_split;
@@ -591,18 +603,13 @@ int sx_batch_stage(sx_batch *b, const sx_trajectory *trajs, size_t ntraj) {
         for (size_t s = 0; s < t.nshells; ++s) ql[s] = t.qlen[s] / kTwoPi;
         sx::ShellGrid g;
         w.mode = sx::shell_progression(ql.data(), (int)t.nshells, w.ctx->rmax, &g);
-        w.runs = g.runs;
         w.qstep = g.step;
         w.q0 = g.q0;
         if (w.mode == 2) {
-            // [ns] exact runs, then [ns] residuals in radians per Angstrom
-            std::vector<float> rr(t.nshells);
-            for (size_t s = 0; s < t.nshells; ++s) rr[s] = (float)(kTwoPi * g.res[s]);
-            const size_t rb = sizeof(sx::ShellRun) * t.nshells;
+            // tile lines and residuals, per tile plan (build_plan)
+            w.grid = g;
+            w.qturns = ql;
             SXB_HIP(b, w.qres.ensure(sx::shell_table_bytes((int)t.nshells)));
-            SXB_HIP(b, hipMemcpy(w.qres.p, g.runs.data(), rb, hipMemcpyHostToDevice));
-            SXB_HIP(b, hipMemcpy(static_cast<char *>(w.qres.p) + rb, rr.data(), sizeof(float) * t.nshells,
-                                 hipMemcpyHostToDevice));
         }
         w.chunks = (int)std::min<size_t>(t.NM, 8);
         SXB_HIP(b, w.qlen.ensure(sizeof(double) * t.nshells));

@@ -246,13 +246,27 @@ int default_shape(const sx_ctx *c, int ns, bool per_shell) {
     return ns > 16 ? 4 : ns > 8 ? 3 : 0;
 }
 
-// Shells of a residual-corrected launch with VS shells per tile whose tiles
-// lie in one exact run (the tiles the kernel steps without corrections).
-int exact_run_shells(const std::vector<sx::ShellRun> &runs, int ns, int VS) {
-    int n = 0;
-    for (int s0 = 0; s0 < ns && (size_t)ns <= runs.size(); s0 += VS)
-        if (runs[s0].last >= std::min(s0 + VS, ns) - 1) n += std::min(VS, ns - s0);
-    return n;
+// The tile lines and residuals of a residual-corrected launch of this tile
+// shape, onto the stream ahead of it (the tiles start at multiples of VS);
+// *plain: the shells it steps without corrections.
+int upload_tile_lines(sx_ctx *c, int ns, int shape, int *plain) {
+    *plain = 0;
+    if (c->q_mode != 2) return SX_OK;
+    const int VS = sx::orient_shape_vs(shape);
+    std::vector<int> firsts, vs;
+    for (int s0 = 0; s0 < ns; s0 += VS) {
+        firsts.push_back(s0);
+        vs.push_back(VS);
+    }
+    std::vector<sx::TileLine> lines(ns);
+    std::vector<float> res(ns, 0.f);
+    *plain = sx::tile_lines(c->q_turns.data(), ns, c->rmax, c->q_grid, firsts.data(), vs.data(), (int)firsts.size(),
+                            lines.data(), res.data());
+    const size_t lb = sizeof(sx::TileLine) * ns;
+    SX_HIP(c, hipMemcpyAsync(c->qres.p, lines.data(), lb, hipMemcpyHostToDevice, c->stream));
+    SX_HIP(c, hipMemcpyAsync(static_cast<char *>(c->qres.p) + lb, res.data(), sizeof(float) * ns,
+                             hipMemcpyHostToDevice, c->stream));
+    return SX_OK;
 }
 
 // OrientArgs of the current call (stepping mode and tables from run_batch).
@@ -274,9 +288,9 @@ sx::OrientArgs oriented_args(const sx_ctx *c, int ns, int NM, const float *bptr,
     oa.ap = c->q_mode;
     oa.qstep = c->q_step;
     oa.q0 = c->q_origin;
-    // mode 2: [ns] exact runs, then [ns] residuals
-    oa.runs = c->q_mode == 2 ? c->qres.as<sx::ShellRun>() : nullptr;
-    oa.qres = c->q_mode == 2 ? reinterpret_cast<const float *>(oa.runs + ns) : nullptr;
+    // mode 2: [ns] tile lines, then [ns] residuals (upload_tile_lines)
+    oa.lines = c->q_mode == 2 ? c->qres.as<sx::TileLine>() : nullptr;
+    oa.qres = c->q_mode == 2 ? reinterpret_cast<const float *>(oa.lines + ns) : nullptr;
     return oa;
 }
 
@@ -313,6 +327,9 @@ int pick_shape(sx_ctx *c, int ns, int NM, const float *bptr, bool per_shell, int
         size_t na = (size_t)std::ceil(frac * (double)c->NAp / 64.0) * 64;
         na = std::max(na, (size_t)64 * split);
         oa.natoms = std::min(na, c->NAp);
+        int plain = 0;
+        const int urc = upload_tile_lines(c, ns, sh, &plain);
+        if (urc != SX_OK) return urc;
         SX_HIP(c, hipEventRecord(e0, c->stream));
         SX_HIP(c, sx::launch_amp_oriented(oa, c->stream));
         SX_HIP(c, hipEventRecord(e1, c->stream));
@@ -372,16 +389,11 @@ int run_batch(sx_ctx *c, int ns, size_t NM, const double *qv, const double *qlen
         if (c->q_mode == 2 && b_shell) c->q_mode = 0;
         c->q_step = g.step;
         c->q_origin = g.q0;
-        c->q_runs = g.runs;
         if (c->q_mode == 2) {
-            // [ns] exact runs, then [ns] residuals in radians per Angstrom
-            std::vector<float> rr(ns);
-            for (int s = 0; s < ns; ++s) rr[s] = (float)(kTwoPi * g.res[s]);
-            const size_t rb = sizeof(sx::ShellRun) * ns;
+            // tile lines and residuals, per tile shape at each launch
+            c->q_grid = g;
+            c->q_turns = ql;
             SX_HIP(c, c->qres.ensure(sx::shell_table_bytes(ns)));
-            SX_HIP(c, hipMemcpyAsync(c->qres.p, g.runs.data(), rb, hipMemcpyHostToDevice, c->stream));
-            SX_HIP(c, hipMemcpyAsync(static_cast<char *>(c->qres.p) + rb, rr.data(), sizeof(float) * ns,
-                                     hipMemcpyHostToDevice, c->stream));
         }
         SX_HIP(c, c->qlen.ensure(sizeof(double) * ns));
         SX_HIP(c, hipMemcpyAsync(c->qlen.p, ql.data(), sizeof(double) * ns, hipMemcpyHostToDevice, c->stream));
@@ -504,11 +516,14 @@ int run_batch(sx_ctx *c, int ns, size_t NM, const double *qv, const double *qlen
             SX_HIP(c, sx::launch_multipole(ma, c->stream));
         } else if (oriented) {
             const sx::OrientArgs oa = oriented_args(c, ns, (int)NM, bptr, b_shell != nullptr, shape, split);
+            int plain = 0;
+            int urc = upload_tile_lines(c, ns, shape, &plain);
+            if (urc != SX_OK) return urc;
             SX_HIP(c, sx::launch_amp_oriented(oa, c->stream));
             c->timing.tile_shape = shape + 1;
             c->timing.shell_progression = oa.ap ? 1 : 0;
             c->timing.shell_residuals = oa.ap == 2 ? 1 : 0;
-            c->timing.shell_exact_runs = oa.ap == 2 ? exact_run_shells(c->q_runs, ns, sx::orient_shape_vs(shape)) : 0;
+            c->timing.shell_exact_runs = plain;
         } else {
             sx::AmpArgs aa;
             aa.coords = c->coords.as<float>();

@@ -91,7 +91,8 @@ struct sx_ctx {
     // origin and step in |q| / 2 pi
     int q_mode = 0;
     double q_step = 0.0, q_origin = 0.0;
-    std::vector<sx::ShellRun> q_runs;  // exact runs of the call's shells (mode 2)
+    sx::ShellGrid q_grid;              // the call's shells (mode 2): grid and exact runs
+    std::vector<double> q_turns;       // and their |q| / 2 pi
     // largest |r| of the staged coordinates (Angstrom): bounds the residual phases
     double rmax = 0.0;
     // autotuned oriented tile shape per (NFp, NAp, NM, shells, per-shell b, progression)

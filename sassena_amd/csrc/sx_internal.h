@@ -87,16 +87,30 @@ hipError_t all_blocks_per_cu(int vb, bool per_row, int *blocks);
 
 // An exact run of shells from one shell s: shells s..last (at most
 // kMaxRunShells) form an arithmetic progression to 1e-12 relative with this
-// step.  A tile of a residual-corrected call whose shells lie in one run is
-// stepped exactly, without the first-order corrections.  A reference scan's
-// powf points form such runs wherever the rounding of j / (points - 1) keeps
-// one offset (for 40 points from 0.1 to 4.0: shells 10-19 and 20-39).
+// step.  A reference scan's powf points form such runs wherever the rounding
+// of j / (points - 1) keeps one offset (for 40 points from 0.1 to 4.0:
+// shells 10-19 and 20-39).
 struct ShellRun {
     double step;   // turns per Angstrom
     int last;      // last shell of the run from this one
     int pad;
 };
 constexpr int kMaxRunShells = 32;
+
+// The line a residual-corrected tile steps on: shells first + k (k < VS) at
+// qmid + (k - VS/2) step plus residuals r_k, the middle shell (k = VS/2)
+// exactly when it exists.  The line runs through the middle shell with the
+// step of the exact run around it, or with the call's step, whichever leaves
+// fewer of the two walks from the middle (up: k > VS/2, down: k < VS/2) with
+// residuals; flags bit 0: the up walk carries first-order corrections, bit 1:
+// the down walk does.  Stored at the tile's first shell.
+struct TileLine {
+    double qmid;   // turns per Angstrom
+    double step;
+    int flags;
+    int pad;
+};
+constexpr int kTileCorrectUp = 1, kTileCorrectDown = 2;
 
 // Oriented shells: v = s * NM + m has q_v = |q_s| u_m.  NAp must be a multiple
 // of 16 (it is a multiple of 64).
@@ -117,8 +131,9 @@ struct OrientArgs {
     int ap = 0;
     double qstep = 0.0;
     double q0 = 0.0;
+    // ap = 2: tile lines at the tiles' first shells and residuals against them
     const float *qres = nullptr;
-    const ShellRun *runs = nullptr;  // [ns] exact runs within the grid (ap == 2), or null
+    const TileLine *lines = nullptr;
     // Atoms summed (the first natoms of the NAp rows); 0: all NAp.  The
     // autotuner times shapes on a prefix of the atoms.
     size_t natoms = 0;
@@ -138,8 +153,14 @@ struct ShellGrid {
     std::vector<double> res;        // [ns], turns per Angstrom (mode 2)
     std::vector<ShellRun> runs;     // [ns] exact runs from each shell (mode 2)
 };
-// Device bytes of the mode-2 shell tables: [ns] ShellRun, then [ns] float residuals.
-inline size_t shell_table_bytes(int ns) { return (sizeof(ShellRun) + sizeof(float)) * (size_t)ns; }
+// Device bytes of the mode-2 shell tables: [ns] TileLine, then [ns] float residuals.
+inline size_t shell_table_bytes(int ns) { return (sizeof(TileLine) + sizeof(float)) * (size_t)ns; }
+// Tile lines and residuals (radians per Angstrom) of the mode-2 tiles that
+// start at firsts[t] with vs[t] shells: lines[firsts[t]], res[shells of the
+// tile].  Returns the shells stepped without corrections (the middle shells
+// and the shells of the walks without residuals).
+int tile_lines(const double *qlen, int ns, double rmax, const ShellGrid &g, const int *firsts, const int *vs,
+               int ntiles, TileLine *lines, float *res);
 constexpr double kMaxShellResidual = 1e-6;
 constexpr double kMaxResidualPhase = 1e-3;
 constexpr double kTwoPiTurns = 6.283185307179586476925286766559;

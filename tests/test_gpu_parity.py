@@ -335,10 +335,12 @@ def test_scan_shells_residual_steps(sx, oracle_mod, shape):
     grid): the stepped-phasor path with first-order residual corrections,
     on every tile shape, full-size headline trajectory, two orientations;
     and a list perturbed by 4e-7 relative (residual phases 20x larger) on
-    the same path.  The scan's powf points form exact runs (shells 10-19 and
-    20-39 each lie on one arithmetic progression), and tiles inside a run are
-    stepped without corrections: 16, 28, 16, 8 and 20 shells for the shapes
-    2x8, 4x4, 1x8, 1x16 and 1x20; none on the perturbed list."""
+    the same path.  Each tile steps on a line through its middle shell; the
+    scan's powf points form exact runs (shells 10-19 and 20-39 each lie on one
+    arithmetic progression), and the walks from the middle that stay in a run
+    need no corrections: with the middle shells, 25, 34, 25, 17 and 30 shells
+    so stepped for the shapes 2x8, 4x4, 1x8, 1x16 and 1x20; on the perturbed
+    list the middle shells and the 4x4 tiles' one-shell up walks."""
     from sassena_amd import synth
     T = synth.case("c2")
     frames = synth.to_frames_aos(T.xyz)
@@ -346,7 +348,8 @@ def test_scan_shells_residual_steps(sx, oracle_mod, shape):
     scan = synth.linear_shells(0.1, 4.0, 40)
     assert np.max(np.abs(scan - synth.exact_linear_shells(0.1, 4.0, 40))) > 1e-8
     bent = synth.exact_linear_shells(0.1, 4.0, 40) * (1 + 4e-7 * np.cos(np.arange(40)))
-    exact_shells = {1: 16, 2: 28, 3: 16, 4: 8, 5: 20}[shape]
+    exact_shells = {1: 25, 2: 34, 3: 25, 4: 17, 5: 30}[shape]
+    bent_shells = {1: 5, 2: 20, 3: 5, 4: 2, 5: 2}[shape]
     refs = {}
     for qlen in (scan, bent):
         with sx.ScatterDevice("all", tile_shape=shape) as dev:
@@ -355,7 +358,7 @@ def test_scan_shells_residual_steps(sx, oracle_mod, shape):
             fqt, fq, fq2 = dev.compute_oriented(qlen, u)
             tm = dev.timing()
             assert tm["shell_progression"] == 1 and tm["shell_residuals"] == 1
-            assert tm["shell_exact_runs"] == (exact_shells if qlen is scan else 0)
+            assert tm["shell_exact_runs"] == (exact_shells if qlen is scan else bent_shells)
         for s in (1, 5, 9, 13, 17, 20, 27, 33, 38, 39):
             key = float(qlen[s])
             if key not in refs:
