@@ -484,6 +484,12 @@ size_t corr_all_lds_bytes(const FftPlan &P) { return corr_lds_bytes<double2, 1>(
 using SelfSplitFn = void (*)(CorrArgs, double2 *);
 
 static SelfSplitFn self_split_fn(const FftPlan &P) {
+    if (Plan256::matches(P)) return corr_self_split_kernel<float2, 4, Plan256>;
+    if (Plan512::matches(P)) return corr_self_split_kernel<float2, 8, Plan512>;
+    if (Plan1024::matches(P)) return corr_self_split_kernel<float2, 8, Plan1024>;
+    if (Plan1536::matches(P)) return corr_self_split_kernel<float2, 24, Plan1536>;
+    if (Plan2048::matches(P)) return corr_self_split_kernel<float2, 8, Plan2048>;
+    if (Plan4096::matches(P)) return corr_self_split_kernel<float2, 16, Plan4096>;
     if (Plan5120::matches(P)) return corr_self_split_kernel<float2, 20, Plan5120>;
     switch (P.E) {
         case 4: return corr_self_split_kernel<float2, 4>;

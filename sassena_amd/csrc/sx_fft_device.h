@@ -573,7 +573,18 @@ __device__ __forceinline__ void fft_sweep_fixed(C (&v)[E], Acc (&acc)[2][E], C *
     }
 }
 
-// The self path's 5,120-point halves (NF up to 5,120 frames): 4^5 x 5 on 256 threads.
+// The one-block half-lengths of the self path whose middle passes all have
+// Ns <= T (the runtime plans of make_fft_plan, radix by radix): 256 = 4^4 on
+// 64 threads, 512 = 8^3 on 64, 1,024 = 8^3 x 2 on 128, 1,536 = 8^3 x 3 on 64
+// (last two passes fused: radix 24), 2,048 = 8^3 x 4 on 256, 4,096 = 8^4 on
+// 256, 5,120 = 4^5 x 5 on 256 (last two fused: radix 20).  2,560, 3,072 and
+// 6,144 keep the runtime plan.
+using Plan256 = StaticPlan<256, 64, 4, 4, 4, 4>;
+using Plan512 = StaticPlan<512, 64, 8, 8, 8>;
+using Plan1024 = StaticPlan<1024, 128, 8, 8, 8, 2>;
+using Plan1536 = StaticPlan<1536, 64, 8, 8, 8, 3>;
+using Plan2048 = StaticPlan<2048, 256, 8, 8, 8, 4>;
+using Plan4096 = StaticPlan<4096, 256, 8, 8, 8, 8>;
 using Plan5120 = StaticPlan<5120, 256, 4, 4, 4, 4, 4, 5>;
 
 // |X_f|^2 of one padded series accumulated into acc; a += sum_f w_f |X_f|^2.

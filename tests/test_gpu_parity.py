@@ -100,10 +100,12 @@ def test_frame_counts_against_oracle(sx, oracle_mod, NF):
     check_close(fqt, fq, fq2, *r)
 
 
-@pytest.mark.parametrize("NF", [1, 2, 63, 64, 65, 257, 1500, 3000, 6000])
+@pytest.mark.parametrize("NF", [1, 2, 63, 64, 65, 257, 1000, 1500, 2000, 2500, 3000, 4000, 5000, 6000])
 def test_self_frame_counts_against_oracle(sx, oracle_mod, NF):
     """The self path over the FFT length classes, including series shorter
-    than the 64 directly summed tail lags."""
+    than the 64 directly summed tail lags: the half-lengths with a plan fixed
+    at compile time (256, 512, 1,024, 1,536, 2,048, 4,096, 5,120) and those
+    with the runtime plan (2,560, 3,072, 6,144)."""
     NA = 29
     frames, b = small_trajectory(NA, NF, seed=NF + 1)
     qv = 1.3 * unit_vectors(4, NF + 1)
