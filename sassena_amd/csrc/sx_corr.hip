@@ -491,6 +491,9 @@ static SelfSplitFn self_split_fn(const FftPlan &P) {
     if (Plan2048::matches(P)) return corr_self_split_kernel<float2, 8, Plan2048>;
     if (Plan4096::matches(P)) return corr_self_split_kernel<float2, 16, Plan4096>;
     if (Plan5120::matches(P)) return corr_self_split_kernel<float2, 20, Plan5120>;
+    if (Plan2560::matches(P)) return corr_self_split_kernel<float2, 20, Plan2560>;
+    if (Plan3072::matches(P)) return corr_self_split_kernel<float2, 24, Plan3072>;
+    if (Plan6144::matches(P)) return corr_self_split_kernel<float2, 24, Plan6144>;
     switch (P.E) {
         case 4: return corr_self_split_kernel<float2, 4>;
         case 8: return corr_self_split_kernel<float2, 8>;

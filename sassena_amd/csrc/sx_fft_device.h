@@ -386,41 +386,87 @@ __device__ __forceinline__ void spass_first(C (&v)[E], C *buf, const Input &in, 
 template <typename C, typename PT>
 struct MidTwiddles {
     C w[PT::npass][8];
+    const C *twp;  // the pass twiddle table (passes with Ns > T load theirs per series)
     template <int p>
     __device__ __forceinline__ void load_from(const C *__restrict__ twp, unsigned ut) {
         if constexpr (p < PT::npass - 1) {
             constexpr int R = PT::radix(p);
-            const unsigned kk = ut & (PT::ns(p) - 1);
+            if constexpr (PT::ns(p) <= PT::T) {
+                const unsigned kk = ut & (PT::ns(p) - 1);
 #pragma unroll
-            for (int q = 1; q < R; ++q) w[p][q] = twiddle_load(twp, kk * R, PT::toff(p) + q);
+                for (int q = 1; q < R; ++q) w[p][q] = twiddle_load(twp, kk * R, PT::toff(p) + q);
+            }
             load_from<p + 1>(twp, ut);
         }
     }
-    __device__ __forceinline__ void load(const C *__restrict__ twp, unsigned ut) { load_from<1>(twp, ut); }
+    __device__ __forceinline__ void load(const C *__restrict__ tw, unsigned ut) {
+        twp = tw;
+        load_from<1>(tw, ut);
+    }
 };
+
+template <typename C, typename PT>
+__device__ __forceinline__ const C *twp_of(const MidTwiddles<C, PT> &mt) {
+    return mt.twp;
+}
 
 template <typename C, int E, typename PT, int p>
 __device__ __forceinline__ void spass_mid(C (&v)[E], C *buf, const MidTwiddles<C, PT> &mt, unsigned ut) {
     constexpr int R = PT::radix(p);
     constexpr unsigned T = PT::T, Ns = PT::ns(p), stride = PT::Lh / R;
-    static_assert(E % R == 0 && Ns <= T, "middle pass of a fixed plan");
-    const unsigned kk = ut & (Ns - 1);
-    const C(&ws)[8] = mt.w[p];
+    static_assert(E % R == 0, "middle pass of a fixed plan");
+    if constexpr (Ns > T) {
+        // groups j = ut + i T have kk = ut + (i T mod Ns) (Ns a multiple of
+        // T): twiddle q is the lane's exp(-2 pi i q ut / (Ns R)) (R - 1 loads)
+        // times a constant of the plan, and the outputs go to
+        // (i T - (i T mod Ns)) R + kk + q Ns, the lane's base plus a constant
+        static_assert(Ns % T == 0, "middle pass of a fixed plan");
+        using Rl = typename Cx<C>::R;
+        C base[R];
 #pragma unroll
-    for (int i = 0; i < E / R; ++i)
+        for (int q = 1; q < R; ++q) base[q] = twiddle_load(twp_of(mt), ut * R, PT::toff(p) + q);
 #pragma unroll
-        for (int q = 0; q < R; ++q) v[i * R + q] = buf[ut + i * T + q * stride];
-    __syncthreads();
-    const unsigned ob = (ut - kk) * R + kk;
+        for (int i = 0; i < E / R; ++i)
 #pragma unroll
-    for (int i = 0; i < E / R; ++i) {
+            for (int q = 0; q < R; ++q) v[i * R + q] = buf[ut + i * T + q * stride];
+        __syncthreads();
 #pragma unroll
-        for (int q = 1; q < R; ++q) v[i * R + q] = cmul(v[i * R + q], ws[q]);
-        Dft<R, C>::run(&v[i * R]);
+        for (int i = 0; i < E / R; ++i) {
+            constexpr double kTurn = -6.283185307179586476925286766559 / (double)(Ns * R);
+            const unsigned im = (i * T) % Ns;
 #pragma unroll
-        for (int q = 0; q < R; ++q) buf[ob + i * T * R + q * Ns] = v[i * R + q];
+            for (int q = 1; q < R; ++q) {
+                C t = base[q];
+                if (im != 0)
+                    t = cmul(t, Cx<C>::make((Rl)__builtin_cos(kTurn * (double)(q * im)),
+                                            (Rl)__builtin_sin(kTurn * (double)(q * im))));
+                v[i * R + q] = cmul(v[i * R + q], t);
+            }
+            Dft<R, C>::run(&v[i * R]);
+#pragma unroll
+            for (int q = 0; q < R; ++q) buf[ut + (i * T - im) * R + im + q * Ns] = v[i * R + q];
+        }
+        __syncthreads();
+    } else {
+        // every group of the thread has the twiddle index kk = ut mod Ns
+        const unsigned kk = ut & (Ns - 1);
+        const C(&ws)[8] = mt.w[p];
+#pragma unroll
+        for (int i = 0; i < E / R; ++i)
+#pragma unroll
+            for (int q = 0; q < R; ++q) v[i * R + q] = buf[ut + i * T + q * stride];
+        __syncthreads();
+        const unsigned ob = (ut - kk) * R + kk;
+#pragma unroll
+        for (int i = 0; i < E / R; ++i) {
+#pragma unroll
+            for (int q = 1; q < R; ++q) v[i * R + q] = cmul(v[i * R + q], ws[q]);
+            Dft<R, C>::run(&v[i * R]);
+#pragma unroll
+            for (int q = 0; q < R; ++q) buf[ob + i * T * R + q * Ns] = v[i * R + q];
+        }
+        __syncthreads();
     }
-    __syncthreads();
 }
 
 // The last pass's twiddles exp(-2 pi i q (ut + i T) / Lh) are the thread's
@@ -573,12 +619,12 @@ __device__ __forceinline__ void fft_sweep_fixed(C (&v)[E], Acc (&acc)[2][E], C *
     }
 }
 
-// The one-block half-lengths of the self path whose middle passes all have
-// Ns <= T (the runtime plans of make_fft_plan, radix by radix): 256 = 4^4 on
-// 64 threads, 512 = 8^3 on 64, 1,024 = 8^3 x 2 on 128, 1,536 = 8^3 x 3 on 64
-// (last two passes fused: radix 24), 2,048 = 8^3 x 4 on 256, 4,096 = 8^4 on
-// 256, 5,120 = 4^5 x 5 on 256 (last two fused: radix 20).  2,560, 3,072 and
-// 6,144 keep the runtime plan.
+// The one-block half-lengths of the self path (the runtime plans of
+// make_fft_plan, radix by radix): 256 = 4^4 on 64 threads, 512 = 8^3 on 64,
+// 1,024 = 8^3 x 2 on 128, 1,536 = 8^3 x 3 on 64 (last two passes fused:
+// radix 24), 2,048 = 8^3 x 4 on 256, 2,560 = 4^4 x 2 x 5 on 128, 3,072 =
+// 8^3 x 2 x 3 on 128, 4,096 = 8^4 on 256, 5,120 = 4^5 x 5 on 256 (last two
+// fused: radix 20), 6,144 = 8^3 x 4 x 3 on 256.
 using Plan256 = StaticPlan<256, 64, 4, 4, 4, 4>;
 using Plan512 = StaticPlan<512, 64, 8, 8, 8>;
 using Plan1024 = StaticPlan<1024, 128, 8, 8, 8, 2>;
@@ -586,6 +632,9 @@ using Plan1536 = StaticPlan<1536, 64, 8, 8, 8, 3>;
 using Plan2048 = StaticPlan<2048, 256, 8, 8, 8, 4>;
 using Plan4096 = StaticPlan<4096, 256, 8, 8, 8, 8>;
 using Plan5120 = StaticPlan<5120, 256, 4, 4, 4, 4, 4, 5>;
+using Plan2560 = StaticPlan<2560, 128, 4, 4, 4, 4, 2, 5>;
+using Plan3072 = StaticPlan<3072, 128, 8, 8, 8, 2, 3>;
+using Plan6144 = StaticPlan<6144, 256, 8, 8, 8, 4, 3>;
 
 // |X_f|^2 of one padded series accumulated into acc; a += sum_f w_f |X_f|^2.
 // tw: packed pass twiddles; pre: exp(-2 pi i n / L) for the odd bins.
