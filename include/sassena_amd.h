@@ -70,9 +70,9 @@ typedef struct sx_config {
     int tile_shape;      /* oriented shells: 0 autotuned, 1..5 fixed (2x8, 4x4, 1x8, 1x16,
                             1x20 orientations x shells per wavefront) */
     int fft_halves;      /* "self" autocorrelation, the two halves of a padded series: 3 in
-                            different blocks, 2 both in one block's sweep, 1 one after the
-                            other in one block; 0 picks 3 where the FFT plan and memory
-                            allow, else 2 */
+                            different blocks (every one-block FFT length, up to 10,240 frames),
+                            2 both in one block's sweep, 1 one after the other in one block;
+                            0 picks 3 where memory allows, else 2 */
     int multipole_terms; /* multipole: the spherical (sphere, 36 B) or cylinder-frame
                             (cylinder, 12 B) coordinates of every (atom, frame) 1 formed once
                             per call in a pre-pass, 2 formed by the amplitude kernel itself;

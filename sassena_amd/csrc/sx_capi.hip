@@ -674,9 +674,9 @@ int run_batch(sx_ctx *c, int ns, size_t NM, const double *qv, const double *qlen
         const int wpp = 2 * (P.T / 64);  // parts of a per pair: halves x wavefronts
         const size_t apair_bytes = sizeof(double2) * wpp * (size_t)ns * (size_t)pairs;
         int halves = c->cfg.fft_halves;
-        if (halves == 0) halves = (P.T <= 256 && apair_bytes <= kSplitBudgetBytes) ? 3 : 2;
-        if (halves == 3 && P.T > 256 && P.N1 == 1)
-            return fail(c, SX_ERR_UNSUPPORTED, "fft_halves=3 needs an FFT plan of at most 256 threads (NF=%zu)", NF);
+        if (halves == 0) halves = (sx::self_split_supported(P) && apair_bytes <= kSplitBudgetBytes) ? 3 : 2;
+        if (halves == 3 && P.N1 == 1 && !sx::self_split_supported(P))
+            return fail(c, SX_ERR_UNSUPPORTED, "fft_halves=3: no split kernel for this FFT plan (NF=%zu)", NF);
         if (P.N1 > 1) {
             // rows of a long plan (the halves always in different blocks):
             // one part of a per (pair, half, row), summed per pair

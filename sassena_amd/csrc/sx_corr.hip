@@ -224,11 +224,24 @@ __device__ __forceinline__ void self_half_body(const CorrArgs &a, double2 *__res
     if (since > 0 || first) flush_power_half<E, H>(P, acc, row, first, tid);
 }
 
+// Threads and workgroups per CU the split kernel is built for: 256 threads
+// (E = 20, halves of 5,120 points: four workgroups per CU, as many as the
+// LDS allows, so at most 128 VGPRs), or the 512 of a fixed plan (two
+// workgroups per CU).
+template <typename PT>
+constexpr int split_threads() {
+    if constexpr (std::is_void<PT>::value) return 256;
+    else return PT::T > 256 ? (int)PT::T : 256;
+}
+template <typename PT, int E>
+constexpr int split_min_blocks() {
+    return split_threads<PT>() > 256 ? 2 : (E == 20 ? 4 : 1);
+}
+
 // PT: void for the runtime plan, else a StaticPlan the launch's plan matches.
 template <typename C, int E, typename PT = void>
-// E = 20 (halves of 5,120 points): four workgroups per CU, as many as the
-// LDS allows, so at most 128 VGPRs.
-__global__ __launch_bounds__(256, E == 20 ? 4 : 1) void corr_self_split_kernel(CorrArgs a, double2 *apair) {
+__global__ __launch_bounds__(split_threads<PT>(), (split_min_blocks<PT, E>())) void corr_self_split_kernel(
+    CorrArgs a, double2 *apair) {
     extern __shared__ double2 lds_d[];
     C *buf = reinterpret_cast<C *>(lds_d);
     const int sc = blockIdx.x >> 1;
@@ -494,6 +507,9 @@ static SelfSplitFn self_split_fn(const FftPlan &P) {
     if (Plan2560::matches(P)) return corr_self_split_kernel<float2, 20, Plan2560>;
     if (Plan3072::matches(P)) return corr_self_split_kernel<float2, 24, Plan3072>;
     if (Plan6144::matches(P)) return corr_self_split_kernel<float2, 24, Plan6144>;
+    if (Plan8192::matches(P)) return corr_self_split_kernel<float2, 16, Plan8192>;
+    if (Plan10240::matches(P)) return corr_self_split_kernel<float2, 20, Plan10240>;
+    if (P.T > 256) return nullptr;  // the runtime-plan kernels are built for 256-thread blocks
     switch (P.E) {
         case 4: return corr_self_split_kernel<float2, 4>;
         case 8: return corr_self_split_kernel<float2, 8>;
@@ -504,10 +520,12 @@ static SelfSplitFn self_split_fn(const FftPlan &P) {
     }
 }
 
+bool self_split_supported(const FftPlan &P) { return P.N1 == 1 && self_split_fn(P) != nullptr; }
+
 hipError_t self_split_blocks_per_cu(const FftPlan &P, int *blocks) {
     *blocks = 0;
     const SelfSplitFn k = self_split_fn(P);
-    if (!k || P.T > 256) return hipErrorInvalidValue;
+    if (!k) return hipErrorInvalidValue;
     const size_t lds = corr_lds_bytes<float2, 1>(P);
     hipError_t e = hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
@@ -517,7 +535,6 @@ hipError_t self_split_blocks_per_cu(const FftPlan &P, int *blocks) {
 hipError_t launch_corr_self_split(const CorrArgs &a, double2 *apair, hipStream_t stream) {
     const SelfSplitFn k = self_split_fn(a.plan);
     if (!k) return hipErrorInvalidValue;
-    if (a.plan.T > 256) return hipErrorInvalidValue;  // the split kernel is built for 256-thread blocks
     const size_t lds = corr_lds_bytes<float2, 1>(a.plan);
     hipError_t e = hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;

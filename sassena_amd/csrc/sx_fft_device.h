@@ -624,7 +624,7 @@ __device__ __forceinline__ void fft_sweep_fixed(C (&v)[E], Acc (&acc)[2][E], C *
 // 1,024 = 8^3 x 2 on 128, 1,536 = 8^3 x 3 on 64 (last two passes fused:
 // radix 24), 2,048 = 8^3 x 4 on 256, 2,560 = 4^4 x 2 x 5 on 128, 3,072 =
 // 8^3 x 2 x 3 on 128, 4,096 = 8^4 on 256, 5,120 = 4^5 x 5 on 256 (last two
-// fused: radix 20), 6,144 = 8^3 x 4 x 3 on 256.
+// fused: radix 20), 6,144 = 8^3 x 4 x 3 on 256,
 using Plan256 = StaticPlan<256, 64, 4, 4, 4, 4>;
 using Plan512 = StaticPlan<512, 64, 8, 8, 8>;
 using Plan1024 = StaticPlan<1024, 128, 8, 8, 8, 2>;
@@ -635,6 +635,10 @@ using Plan5120 = StaticPlan<5120, 256, 4, 4, 4, 4, 4, 5>;
 using Plan2560 = StaticPlan<2560, 128, 4, 4, 4, 4, 2, 5>;
 using Plan3072 = StaticPlan<3072, 128, 8, 8, 8, 2, 3>;
 using Plan6144 = StaticPlan<6144, 256, 8, 8, 8, 4, 3>;
+// and on 512 threads (two workgroups per CU, 64 and 80 KB of LDS each):
+// 8,192 = 8^4 x 2, 10,240 = 4^5 x 2 x 5
+using Plan8192 = StaticPlan<8192, 512, 8, 8, 8, 8, 2>;
+using Plan10240 = StaticPlan<10240, 512, 4, 4, 4, 4, 4, 2, 5>;
 
 // |X_f|^2 of one padded series accumulated into acc; a += sum_f w_f |X_f|^2.
 // tw: packed pass twiddles; pre: exp(-2 pi i n / L) for the odd bins.

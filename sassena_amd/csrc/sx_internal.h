@@ -212,6 +212,9 @@ hipError_t launch_corr_self(const CorrArgs &a, int halves, hipStream_t stream);
 hipError_t launch_corr_self_split(const CorrArgs &a, double2 *apair, hipStream_t stream);
 // Workgroups of the split kernel for plan P that fit one CU at once.
 hipError_t self_split_blocks_per_cu(const FftPlan &P, int *blocks);
+// Whether the split kernel serves plan P (every one-block plan of 256
+// threads or fewer, and the 512-thread ones with a fixed plan).
+bool self_split_supported(const FftPlan &P);
 // Direct FP64 sums S(tau) = sum_t conj(x_t) x_{t+tau} of the last min(64, NF)
 // lags of every (atom, vector) pair of the self path, summed over pairs into
 // tail[shell][tblocks][64] (one wavefront per pair).  The FP32 transforms
