@@ -210,8 +210,8 @@ typedef struct sx_batch_timing {
                              5 mixed (1x8, 1x16 and 1x20 tiles chosen per trajectory) */
     size_t terms_stepped;  /* terms of trajectories whose shells step phasors (exact progression) */
     size_t terms_residual; /* terms of trajectories stepped with residual corrections */
-    size_t terms_exact_runs; /* of those, terms of shells stepped without corrections (on their
-                                tile's line, an exact run of the scan through its middle shell) */
+    size_t terms_exact_runs; /* of those, terms of tiles whose shells all lie on their tile's line
+                                (an exact run of the scan), stepped without corrections */
 } sx_batch_timing;
 
 int sx_batch_create(const sx_config *cfg, sx_batch **out);

@@ -24,22 +24,25 @@
 
 namespace sx {
 
-// RES = false: tiles of trajectories whose shells take one phase each or an
-// exact progression (modes 0 and 1); RES = true: tiles of trajectories
-// stepped with residual corrections (mode 2), whose FP64 totals live in the
-// dynamic LDS (kResidualLds<VM, VS>); each wavefront touches only its own
-// slice, so tiles need no barrier between them.  The two kinds run as two
-// persistent launches with a work queue each, so neither kernel carries the
-// other's body.
+// KIND 0: tiles of trajectories whose shells take one phase each or an
+// exact progression (modes 0 and 1); KIND 1: tiles of trajectories stepped
+// with residual corrections on their tile lines (mode 2: both walks
+// corrected, also where one of them needs none), KIND 2: such tiles whose
+// shells all lie on their line (mode 3); KINDs 1 and 2 keep their FP64
+// totals in the dynamic LDS (kResidualLds<VM, VS>), each wavefront touching
+// only its own slice, so tiles need no barrier between them.  The kinds run
+// as persistent launches with a work queue each, so that no kernel carries
+// another's body (one kernel with the bodies of modes 2-5 ran the
+// 64-trajectory batch 19 % slower).
 // The LDS word through which thread 0 hands the next tile index to its
 // workgroup.  The residual kernel's dynamic LDS fills the CU at two
 // workgroups (80 KiB for 1 x 20), and a static word on top would leave room
 // for one, so there the index passes through the first word of wave 0's
 // slice of the totals: thread 0 writes it only after its own wave has
 // finished the previous tile, and the other waves never touch that slice.
-template <bool RES>
+template <int KIND>
 __device__ inline int *tile_slot(double *dyn) {
-    if constexpr (RES) {
+    if constexpr (KIND > 0) {
         return reinterpret_cast<int *>(dyn);
     } else {
         __shared__ int cur;
@@ -47,13 +50,13 @@ __device__ inline int *tile_slot(double *dyn) {
     }
 }
 
-template <int VM, int VS, bool RES>
+template <int VM, int VS, int KIND>
 __global__ __launch_bounds__(256, 2) void amp_persistent_kernel(const OrientKernelArgs *__restrict__ targs,
                                                              const int4 *__restrict__ items,
                                                              const int *__restrict__ qstart,
                                                              unsigned *__restrict__ counters) {
     extern __shared__ double tot_lds[];
-    int *cur = tile_slot<RES>(tot_lds);
+    int *cur = tile_slot<KIND>(tot_lds);
     const int x = blockIdx.x & 7;
     const int begin = qstart[x], n = qstart[x + 1] - begin;
     for (;;) {
@@ -63,8 +66,10 @@ __global__ __launch_bounds__(256, 2) void amp_persistent_kernel(const OrientKern
         __syncthreads();  // every thread has read cur before it is overwritten
         if (it >= n) break;
         const int4 w = items[begin + it];
-        if constexpr (RES)
-            residual_tile<VM, VS>(targs[w.x], w.y, 0, w.z & 0xffff, w.z >> 16, tot_lds);
+        if constexpr (KIND == 1)
+            oriented_tile_body<VM, VS, false, 2, true>(targs[w.x], w.y, 0, w.z & 0xffff, w.z >> 16, tot_lds);
+        else if constexpr (KIND == 2)
+            oriented_tile_body<VM, VS, false, 3, true>(targs[w.x], w.y, 0, w.z & 0xffff, w.z >> 16, tot_lds);
         else
             oriented_tile<VM, VS, false, false>(targs[w.x], w.y, 0, w.z & 0xffff, w.z >> 16);
     }
@@ -110,19 +115,20 @@ static size_t persistent_res_lds(int shape) {
     }
 }
 
-template <bool RES>
+template <int KIND>
 static PersistentFn persistent_fn_of(int shape) {
     switch (shape) {
-        case 0: return amp_persistent_kernel<2, 8, RES>;
-        case 1: return amp_persistent_kernel<4, 4, RES>;
-        case 2: return amp_persistent_kernel<1, 8, RES>;
-        case 3: return amp_persistent_kernel<1, 16, RES>;
-        default: return amp_persistent_kernel<1, 20, RES>;
+        case 0: return amp_persistent_kernel<2, 8, KIND>;
+        case 1: return amp_persistent_kernel<4, 4, KIND>;
+        case 2: return amp_persistent_kernel<1, 8, KIND>;
+        case 3: return amp_persistent_kernel<1, 16, KIND>;
+        default: return amp_persistent_kernel<1, 20, KIND>;
     }
 }
 
-static PersistentFn persistent_fn(int shape, bool res) {
-    return res ? persistent_fn_of<true>(shape) : persistent_fn_of<false>(shape);
+constexpr int kQueueKinds = 3;
+static PersistentFn persistent_fn(int shape, int kind) {
+    return kind == 2 ? persistent_fn_of<2>(shape) : kind == 1 ? persistent_fn_of<1>(shape) : persistent_fn_of<0>(shape);
 }
 
 static CorrBatchFn corr_batch_fn(int E) {
@@ -159,7 +165,7 @@ struct TrajWork {
 struct Queue {
     DevBuf items, qstart, counters;
     int nitems = 0, persistent_blocks = 0;
-    bool res = false;  // residual-corrected tiles
+    int kind = 0;  // 0 no residuals, 1 residual-corrected tiles, 2 tiles on their line
     int shape = 0;     // fixed tile shape of the kernel
 };
 
@@ -171,7 +177,7 @@ struct Plan {
     // each by tile shape (largest first)
     std::vector<Queue> queues;
     int nitems = 0, nfitems = 0;
-    size_t terms_exact_runs = 0;  // terms of mode-2 shells stepped without corrections
+    size_t terms_exact_runs = 0;  // terms of mode-2 tiles stepped without corrections (kind 2)
     size_t final_lds = 0;
     struct CorrGroup {
         int E, T, first, count;
@@ -198,7 +204,7 @@ struct sx_batch {
     int shape = 0;
     // persistent workgroups that fit the GPU at once, per kernel (plain,
     // residual) and tile shape
-    int max_resident[2][sx::kOrientShapes] = {};
+    int max_resident[sx::kQueueKinds][sx::kOrientShapes] = {};
     std::unique_ptr<Plan> all;
     std::vector<std::unique_ptr<Plan>> one;
     sx_batch_timing timing{};
@@ -302,8 +308,8 @@ int build_plan(sx_batch *b, const std::vector<int> &ks, int shape, Plan &P) {
         int4 w;
         long cost;
     };
-    // [residual][kernel shape]
-    std::vector<Item> all_items[2][sx::kOrientShapes];
+    // [queue kind][kernel shape]
+    std::vector<Item> all_items[sx::kQueueKinds][sx::kOrientShapes];
     for (int k : ks) {
         const sx_trajectory &t = b->trajs[k];
         const bool res = b->work[k].mode == 2;
@@ -315,6 +321,7 @@ int build_plan(sx_batch *b, const std::vector<int> &ks, int shape, Plan &P) {
         } else {
             for (int s = 0; s < oka[k].sgroups; ++s) groups.push_back({s * VS, 0});
         }
+        std::vector<sx::TileLine> lines;
         if (res) {
             // the lines of this trajectory's tiles and the residuals against them
             TrajWork &w = b->work[k];
@@ -324,11 +331,10 @@ int build_plan(sx_batch *b, const std::vector<int> &ks, int shape, Plan &P) {
                 firsts.push_back(sgp.first);
                 vs.push_back(mixed ? sx::kMixedVS[sgp.code] : VS);
             }
-            std::vector<sx::TileLine> lines(ns);
+            lines.resize(ns);
             std::vector<float> rr(ns, 0.f);
-            const int plain = sx::tile_lines(w.qturns.data(), ns, w.ctx->rmax, w.grid, firsts.data(), vs.data(),
-                                             (int)firsts.size(), lines.data(), rr.data());
-            P.terms_exact_runs += (size_t)plain * t.NM * t.NA * t.NF;
+            sx::tile_lines(w.qturns.data(), ns, w.ctx->rmax, w.grid, firsts.data(), vs.data(), (int)firsts.size(),
+                           lines.data(), rr.data());
             const size_t lb = sizeof(sx::TileLine) * ns;
             SXB_HIP(b, hipMemcpy(w.qres.p, lines.data(), lb, hipMemcpyHostToDevice));
             SXB_HIP(b, hipMemcpy(static_cast<char *>(w.qres.p) + lb, rr.data(), sizeof(float) * ns,
@@ -340,7 +346,12 @@ int build_plan(sx_batch *b, const std::vector<int> &ks, int shape, Plan &P) {
             const long cost = mixed ? (long)(oka[k].atoms_per_split * sx::kMixedVS[code] *
                                              kSlotCost[res ? 1 : 0][code] * 1000.0)
                                     : (long)oka[k].atoms_per_split;
-            std::vector<Item> &items = all_items[res ? 1 : 0][mixed ? sx::kMixedFixed[code] : shape];
+            const int first = groups[g].first;
+            const bool plain = res && lines[first].flags == 0;
+            if (plain)
+                P.terms_exact_runs += (size_t)std::min(mixed ? sx::kMixedVS[code] : VS, (int)t.nshells - first) *
+                                      t.NM * t.NA * t.NF;
+            std::vector<Item> &items = all_items[plain ? 2 : res ? 1 : 0][mixed ? sx::kMixedFixed[code] : shape];
             for (int ft = 0; ft < ftiles; ++ft)
                 for (int o = 0; o < og; ++o) items.push_back({make_int4(k, ft, o | (groups[g].first << 16), 0), cost});
         }
@@ -348,15 +359,15 @@ int build_plan(sx_batch *b, const std::vector<int> &ks, int shape, Plan &P) {
     P.nitems = 0;
     P.queues.clear();
     static const int kLaunchOrder[sx::kOrientShapes] = {4, 3, 0, 1, 2};  // 1x20, 1x16, 2x8, 4x4, 1x8
-    for (int qi = 0; qi < 2; ++qi)
+    for (int qi = 0; qi < sx::kQueueKinds; ++qi)
         for (int sh : kLaunchOrder) {
             if (all_items[qi][sh].empty()) continue;
             P.queues.emplace_back();
-            P.queues.back().res = qi == 1;
+            P.queues.back().kind = qi;
             P.queues.back().shape = sh;
         }
     for (Queue &Q : P.queues) {
-        const int qi = Q.res ? 1 : 0;
+        const int qi = Q.kind;
         std::vector<Item> &items = all_items[qi][Q.shape];
         // largest tiles first; within a trajectory, tiles sharing a frame tile stay adjacent
         std::stable_sort(items.begin(), items.end(), [](const Item &a, const Item &c) { return a.cost > c.cost; });
@@ -467,8 +478,8 @@ int launch_amplitude(sx_batch *b, Plan &P) {
                            kXcds);
         SXB_HIP(b, hipGetLastError());
         // (the dynamic-LDS attribute of the residual kernels is set once, in sx_batch_stage)
-        const size_t lds = Q.res ? sx::persistent_res_lds(shape) : 0;
-        hipLaunchKernelGGL(sx::persistent_fn(shape, Q.res), dim3(Q.persistent_blocks), dim3(256), lds, b->stream,
+        const size_t lds = Q.kind ? sx::persistent_res_lds(shape) : 0;
+        hipLaunchKernelGGL(sx::persistent_fn(shape, Q.kind), dim3(Q.persistent_blocks), dim3(256), lds, b->stream,
                            P.oka.as<sx::OrientKernelArgs>(), Q.items.as<int4>(), Q.qstart.as<int>(),
                            Q.counters.as<unsigned>());
         SXB_HIP(b, hipGetLastError());
@@ -626,10 +637,10 @@ int sx_batch_stage(sx_batch *b, const sx_trajectory *trajs, size_t ntraj) {
     // persistent grid: the workgroups that are resident at once
     int dev = b->cfg.device, cus = 0, per_cu = 0;
     SXB_HIP(b, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-    for (int qi = 0; qi < 2; ++qi)
+    for (int qi = 0; qi < sx::kQueueKinds; ++qi)
         for (int sh = 0; sh < sx::kOrientShapes; ++sh) {
             const size_t lds = qi ? sx::persistent_res_lds(sh) : 0;
-            const void *fn = (const void *)sx::persistent_fn(sh, qi == 1);
+            const void *fn = (const void *)sx::persistent_fn(sh, qi);
             if (lds > 0) SXB_HIP(b, hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
             SXB_HIP(b, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 256, lds));
             b->max_resident[qi][sh] = cus * std::max(per_cu, 1);
