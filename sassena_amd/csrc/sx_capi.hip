@@ -258,13 +258,16 @@ int upload_tile_lines(sx_ctx *c, int ns, int shape, int *plain) {
         firsts.push_back(s0);
         vs.push_back(VS);
     }
-    std::vector<sx::TileLine> lines(ns);
-    std::vector<float> res(ns, 0.f);
+    // the copies queued from these host tables by an earlier launch are done
+    // before they are overwritten
+    SX_HIP(c, hipStreamSynchronize(c->stream));
+    c->tl_lines.assign(ns, sx::TileLine{});
+    c->tl_res.assign(ns, 0.f);
     *plain = sx::tile_lines(c->q_turns.data(), ns, c->rmax, c->q_grid, firsts.data(), vs.data(), (int)firsts.size(),
-                            lines.data(), res.data());
+                            c->tl_lines.data(), c->tl_res.data());
     const size_t lb = sizeof(sx::TileLine) * ns;
-    SX_HIP(c, hipMemcpyAsync(c->qres.p, lines.data(), lb, hipMemcpyHostToDevice, c->stream));
-    SX_HIP(c, hipMemcpyAsync(static_cast<char *>(c->qres.p) + lb, res.data(), sizeof(float) * ns,
+    SX_HIP(c, hipMemcpyAsync(c->qres.p, c->tl_lines.data(), lb, hipMemcpyHostToDevice, c->stream));
+    SX_HIP(c, hipMemcpyAsync(static_cast<char *>(c->qres.p) + lb, c->tl_res.data(), sizeof(float) * ns,
                              hipMemcpyHostToDevice, c->stream));
     return SX_OK;
 }
@@ -354,6 +357,9 @@ struct MPSpec {
     // Legendre recurrence, and 4 pi f_l|m|, f = sqrt((2l+1)/(2|m|+1) (l-|m|)! (2|m|)! / (l+|m|)!)
     // from log-gamma: no factorial is formed (finite for l <= 200)
     std::vector<double> ynorm;
+    // followed by [l1][l1][2] the coefficients of the Legendre recurrence in the degree,
+    // ((2 ll - 1) / (ll - m), (ll + m - 1) / (ll - m)) at [ll][m] for m + 2 <= ll < l1
+    int plm_l1 = 0;
     std::vector<double> qinfo;   // per shell: |q| (sphere) or q_r, phi_q, q_z (cylinder)
     double base[9];              // cylinder frame of the axis
 };
@@ -485,6 +491,7 @@ int run_batch(sx_ctx *c, int ns, size_t NM, const double *qv, const double *qlen
             ma.base = c->mpq.as<double>() + mp->qinfo.size();
             ma.moments = c->mpmom.as<int>();
             ma.ynorm = mp->ynorm.empty() ? nullptr : c->mpq.as<double>() + mp->qinfo.size() + 9;
+            ma.plm_l1 = mp->plm_l1;
             ma.nmo = (int)NM;
             ma.ns = ns;
             ma.b = bptr;
@@ -1228,6 +1235,17 @@ int sx_compute_multipole(sx_ctx *c, int kind, size_t nshells, const double *qv, 
             mp.ynorm[2 * k] = std::sqrt(cm);
             mp.ynorm[2 * k + 1] = 4.0 * M_PI * std::exp(lf);
         }
+        int lmax = 0;
+        for (size_t k = 0; k < nmo; ++k) lmax = std::max(lmax, moments[2 * k]);
+        mp.plm_l1 = lmax + 1;
+        const size_t base = mp.ynorm.size();
+        mp.ynorm.resize(base + 2 * (size_t)mp.plm_l1 * mp.plm_l1, 0.0);
+        for (int ll = 2; ll <= lmax; ++ll)
+            for (int m = 0; m + 2 <= ll; ++m) {
+                const size_t i = base + 2 * ((size_t)ll * mp.plm_l1 + m);
+                mp.ynorm[i] = (2.0 * ll - 1.0) / (double)(ll - m);
+                mp.ynorm[i + 1] = (ll + m - 1.0) / (double)(ll - m);
+            }
     }
     for (double &x : mp.base) x = 0.0;
     if (kind == SX_MULTIPOLE_CYLINDER) {

@@ -93,6 +93,10 @@ struct sx_ctx {
     double q_step = 0.0, q_origin = 0.0;
     sx::ShellGrid q_grid;              // the call's shells (mode 2): grid and exact runs
     std::vector<double> q_turns;       // and their |q| / 2 pi
+    // host copies of the tile lines and residuals last queued to the device
+    // (alive until the stream has consumed them)
+    std::vector<sx::TileLine> tl_lines;
+    std::vector<float> tl_res;
     // largest |r| of the staged coordinates (Angstrom): bounds the residual phases
     double rmax = 0.0;
     // autotuned oriented tile shape per (NFp, NAp, NM, shells, per-shell b, progression)

@@ -263,7 +263,9 @@ struct MultipoleArgs {
     const double *qinfo;   // sphere: |q| per shell; cylinder: (q_r, phi_q, q_z) per shell
     const double *base;    // cylinder frame, 9 doubles
     const int *moments;    // [nmo][2]
-    const double *ynorm = nullptr;  // sphere: [nmo][2] (c_|m|, 4 pi f_l|m|) of plm_scaled
+    const double *ynorm = nullptr;  // sphere: [nmo][2] (c_|m|, 4 pi f_l|m|) of plm_scaled, then
+                                    // [plm_l1][plm_l1][2] the coefficients of its recurrence
+    int plm_l1 = 0;
     int nmo, ns;
     const float *b;
     int per_shell_b;

@@ -31,6 +31,8 @@ struct MultipoleKernelArgs {
     const double *base;     // cylinder frame er, ephi, ez (9 doubles)
     const int *moments;     // [nmo][2] (l, m)
     const double *ynorm;    // sphere: [nmo][2] start of the Legendre recurrence c_|m| and final 4 pi f_l|m|
+    const double *plm_ab;   // sphere: [l1][l1][2] coefficients of the Legendre recurrence (plm_scaled)
+    int plm_l1;
     int nmo, ns, mblocks;
     const float *b;         // [nb][NAp]
     int per_shell_b;
@@ -184,18 +186,20 @@ __device__ __forceinline__ void cyl_bessel_orders(const int (&ns)[kMoments], int
 // that no factorial is formed; the caller multiplies the finished sums by
 // f_lm = N_lm / N_mm = sqrt((2l+1)/(2m+1) (l-m)! (2m)! / (l+m)!) (host table).
 // Every intermediate stays finite for l <= 200.
-__device__ double plm_scaled(int l, int m, double x, double somx2, double cm) {
+// ab[ll * l1 + m] = ((2 ll - 1) / (ll - m), (ll + m - 1) / (ll - m)): the
+// coefficients of the recurrence in the degree, built on the host and read
+// through the constant address space (scalar loads: l and m are uniform
+// over the wavefront), so that a step is two products and an FMA.
+__device__ double plm_scaled(int l, int m, double x, double somx2, double cm, const double *ab, int l1) {
+    const auto AB = (const __attribute__((address_space(4))) double *)ab;
     double pmm = cm;
     for (int i = 1; i <= m; ++i) pmm *= -somx2;
     if (l == m) return pmm;
     double pmmp1 = x * (2.0 * m + 1.0) * pmm;
     double plm = pmmp1;
     for (int ll = m + 2; ll <= l; ++ll) {
-        // 1 / (ll - m): hardware reciprocal and one Newton step (no division)
-        const double dd = (double)(ll - m);
-        double rr = __builtin_amdgcn_rcp(dd);
-        rr = fma(rr, fma(-dd, rr, 1.0), rr);
-        const double pll = ((2.0 * ll - 1.0) * x * pmmp1 - (ll + m - 1.0) * pmm) * rr;
+        const int i = 2 * (ll * l1 + m);
+        const double pll = fma(AB[i] * x, pmmp1, -AB[i + 1] * pmm);
         pmm = pmmp1;
         pmmp1 = pll;
         plm = pll;
@@ -456,7 +460,7 @@ __global__ __launch_bounds__(256, KIND == 0 ? 3 : 2) void multipole_kernel(Multi
                 const int l = ls[k], m = ms[k], am = m < 0 ? -m : m;
                 const double jl = jv[k];
                 // the Y_lm normalisation and 4 pi scale the finished sums
-                const double th = plm_scaled(l, am, ct, somx2, cms[k]);
+                const double th = plm_scaled(l, am, ct, somx2, cms[k], a.plm_ab, a.plm_l1);
                 if (am == aprev + 1) {
                     const double nr = er * cp - ei * sp;
                     ei = er * sp + ei * cp;
@@ -571,6 +575,9 @@ hipError_t launch_multipole(const MultipoleArgs &m, hipStream_t stream) {
     k.base = m.base;
     k.moments = m.moments;
     k.ynorm = m.ynorm;
+    // the recurrence coefficients follow the per-moment factors
+    k.plm_l1 = m.plm_l1;
+    k.plm_ab = m.ynorm ? m.ynorm + 2 * m.nmo : nullptr;
     k.nmo = m.nmo;
     k.ns = m.ns;
     k.mblocks = (m.nmo + kMoments - 1) / kMoments;
