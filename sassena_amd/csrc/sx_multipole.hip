@@ -88,7 +88,10 @@ __device__ __forceinline__ void sph_bessel_moments(const int (&ls)[kMoments], in
         }
         return;
     }
-    const int N = lhi + 32 + (int)(2.0 * sqrt((double)lhi + x));
+    // x <= lhi here, so lhi + 32 + 2 sqrt(2 lhi) bounds lhi + 32 + 2 sqrt(lhi + x)
+    // for every lane: one start for the wavefront, whose step counter is then
+    // uniform (scalar loop control, no per-lane trip counts)
+    const int N = __builtin_amdgcn_readfirstlane(lhi + 32 + (int)(2.0 * sqrt(2.0 * (double)lhi)));
     double fp1 = 0.0, f = 1e-280;  // f_{n+1}, f_n
     double cap[kMoments];
 #pragma unroll
@@ -137,11 +140,23 @@ __device__ __forceinline__ void cyl_bessel_orders(const int (&ns)[kMoments], int
                                                   double J1, double (&jv)[kMoments]) {
 #pragma unroll
     for (int q = 0; q < kMoments; ++q) jv[q] = ns[q] == 0 ? J0 : (ns[q] == 1 ? J1 : 0.0);
-    if (nhi < 2 || x == 0.0) return;
+    // orders >= 2 vanish at x = 0 (below 1e-20 they are below 1e-40 of J_0)
+    if (nhi < 2 || !(x > 1e-20)) return;
     const double ix = 1.0 / x;
+    // the lowest captured order >= 2: steps outside [nlo, nhi] compare with none
+    int nlo = nhi;
+#pragma unroll
+    for (int q = 0; q < kMoments; ++q)
+        if (ns[q] >= 2) nlo = min(nlo, ns[q]);
     if (x > (double)nhi) {
         double a = J0, bb = J1;  // J_{k-1}, J_k
-        for (int k = 1; k < nhi; ++k) {
+        int k = 1;
+        for (; k + 1 < nlo; ++k) {
+            const double nx = (2.0 * k) * ix * bb - a;
+            a = bb;
+            bb = nx;  // J_{k+1}
+        }
+        for (; k < nhi; ++k) {
             const double nx = (2.0 * k) * ix * bb - a;
             a = bb;
             bb = nx;  // J_{k+1}
@@ -151,20 +166,23 @@ __device__ __forceinline__ void cyl_bessel_orders(const int (&ns)[kMoments], int
         }
         return;
     }
-    int N = nhi + 32 + (int)(2.0 * sqrt((double)nhi + x));
-    N += N & 1;  // even start keeps the normalisation sum on even orders
+    // Miller's downward recurrence.  x <= nhi here, so the start
+    // nhi + 32 + 2 sqrt(2 nhi) bounds nhi + 32 + 2 sqrt(nhi + x) for every
+    // lane: one even start for the wavefront, whose loop counter is then
+    // uniform, and the recurrence runs in three segments, above the captured
+    // orders, through them (the only steps that compare) and below them.
+    int N = nhi + 32 + (int)(2.0 * sqrt(2.0 * (double)nhi));
+    N = __builtin_amdgcn_readfirstlane(N + (N & 1));  // even: the normalisation sums even orders
     double fp1 = 0.0, f = 1e-280, norm = 0.0;
     double cap[kMoments];
 #pragma unroll
     for (int q = 0; q < kMoments; ++q) cap[q] = 0.0;
-    for (int k = N; k > 0; --k) {
+    // one step: f <- J_{k-1} (unnormalised), 2 J_{k-1} into the sum when k - 1 is even and positive
+    auto step = [&](int k) {
         const double fm1 = (2.0 * k) * ix * f - fp1;
         fp1 = f;
-        f = fm1;  // J_{k-1} (unnormalised)
-#pragma unroll
-        for (int q = 0; q < kMoments; ++q)
-            if (ns[q] == k - 1) cap[q] = f;
-        if (((k - 1) & 1) == 0 && k - 1 > 0) norm += 2.0 * f;
+        f = fm1;
+        if (((k - 1) & 1) == 0 && k > 1) norm += 2.0 * f;
         if (fabs(f) > 1e250) {
             f *= 1e-250;
             fp1 *= 1e-250;
@@ -172,7 +190,15 @@ __device__ __forceinline__ void cyl_bessel_orders(const int (&ns)[kMoments], int
 #pragma unroll
             for (int q = 0; q < kMoments; ++q) cap[q] *= 1e-250;
         }
+    };
+    for (int k = N; k > nhi + 1; --k) step(k);
+    for (int k = nhi + 1; k > nlo; --k) {
+        step(k);
+#pragma unroll
+        for (int q = 0; q < kMoments; ++q)
+            if (ns[q] == k - 1) cap[q] = f;
     }
+    for (int k = nlo; k > 0; --k) step(k);
     norm += f;  // J_0
     const double inorm = 1.0 / norm;
 #pragma unroll
