@@ -555,13 +555,29 @@ __global__ __launch_bounds__(256, KIND == 0 ? 3 : 2) void multipole_kernel(Multi
             }
             double jn[kMoments];
             cyl_bessel_orders(cn, nhi, p, J0, J1, jn);
-            double sn = 0.0, cs = 1.0;
+            double sn = 0.0, cs = 1.0, sp1, cp1;
+            sincos(psi, &sp1, &cp1);  // e^{i psi}
 #pragma unroll
             for (int k = 0; k < kMoments; ++k) {
                 if (k >= nk) break;
                 const int l = ls[k], m = ms[k];
-                // one sincos per distinct order (m = 0, 1 and m = 2, 3 share theirs)
-                if (k == 0 || cn[k] != cn[k - 1]) sincos(cn[k] * psi, &sn, &cs);
+                // cos and sin of n psi per distinct order (m = 0, 1 and m = 2, 3
+                // share theirs): from the previous order by up to four products
+                // with e^{+-i psi} (the orders of a wavefront step by small
+                // amounts, 2l, 2l - 1, 2l + 2, 2l + 1), else by sincos
+                if (k == 0 || cn[k] != cn[k - 1]) {
+                    const int d = k == 0 ? 0 : cn[k] - cn[k - 1];
+                    if (k == 0 || d > 4 || d < -4) {
+                        sincos(cn[k] * psi, &sn, &cs);
+                    } else {
+                        const double s1 = d > 0 ? sp1 : -sp1;
+                        for (int r = d > 0 ? d : -d; r > 0; --r) {
+                            const double nc = cs * cp1 - sn * s1;
+                            sn = sn * cp1 + cs * s1;
+                            cs = nc;
+                        }
+                    }
+                }
                 double cr = 0.0, ci = 0.0;
                 const double sgl = (l & 1) ? -1.0 : 1.0;
                 if (m == 0 && l == 0) {
