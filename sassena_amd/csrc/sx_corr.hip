@@ -360,8 +360,22 @@ __global__ __launch_bounds__(256) void self_a_reduce_kernel(const double2 *__res
 // (atom, vector) pairs of the shell with phases formed here
 // (self_vectors_scatter_device.cpp:306-319), else the coherent amplitudes of
 // its vectors.
-template <typename C, int E, bool SELF>
-__global__ __launch_bounds__(512) void corr_rows_kernel(CorrArgs a, double2 *__restrict__ apart_rows) {
+// PT: void for the runtime plan of a row, else a StaticPlan it matches (self
+// series only), built for that plan's threads as the split kernel is.
+template <typename PT>
+constexpr int rows_threads() {
+    if constexpr (std::is_void<PT>::value) return 512;
+    else return split_threads<PT>();
+}
+template <typename PT, int E>
+constexpr int rows_min_blocks() {
+    if constexpr (std::is_void<PT>::value) return 1;
+    else return split_min_blocks<PT, E>();
+}
+
+template <typename C, int E, bool SELF, typename PT = void>
+__global__ __launch_bounds__(rows_threads<PT>(), (rows_min_blocks<PT, E>())) void corr_rows_kernel(
+    CorrArgs a, double2 *__restrict__ apart_rows) {
     using Acc = typename Cx<C>::R;
     extern __shared__ double2 lds_d[];
     C *buf = reinterpret_cast<C *>(lds_d);
@@ -387,6 +401,9 @@ __global__ __launch_bounds__(512) void corr_rows_kernel(CorrArgs a, double2 *__r
     const size_t plane = a.NAp * a.NFp;
     bool first = true;
     int since = 0;
+    using MT = typename std::conditional<std::is_void<PT>::value, int, MidTwiddles<C, PT>>::type;
+    MT mt{};
+    if constexpr (!std::is_void<PT>::value) mt.load(twp, (unsigned)tid);
     for (long pr = p0; pr < p1; ++pr) {
         double ar = 0.0, ai = 0.0;
         int t = tid;
@@ -410,7 +427,10 @@ __global__ __launch_bounds__(512) void corr_rows_kernel(CorrArgs a, double2 *__r
             in.Lh = (unsigned)P.Lh;
             in.N1 = P.N1;
             in.k1 = k1;
-            fft_sweep<C, 1, E>(P, 0, v, acc, buf, twp, tw, w, P.Lh, in, ar, ai, t);
+            if constexpr (std::is_void<PT>::value)
+                fft_sweep<C, 1, E>(P, 0, v, acc, buf, twp, tw, w, P.Lh, in, ar, ai, t);
+            else
+                fft_sweep_fixed<C, E, PT, 0>(v[0], acc, buf, twp, mt, w, in, ar, ai, (unsigned)t);
         } else {
             RowInput<C, AmpRowSource<C>> in;
             in.src.p = a.amp + ((size_t)s * a.NM + pr) * a.NFp;
@@ -451,9 +471,27 @@ static RowsFn rows_fn(int E) {
     }
 }
 
+// The self rows kernel of a row plan fixed at compile time, if there is one.
+static RowsFn rows_fixed_self(const FftPlan &P) {
+    if (Plan256::matches(P, true)) return corr_rows_kernel<float2, 4, true, Plan256>;
+    if (Plan512::matches(P, true)) return corr_rows_kernel<float2, 8, true, Plan512>;
+    if (Plan1024::matches(P, true)) return corr_rows_kernel<float2, 8, true, Plan1024>;
+    if (Plan1536::matches(P, true)) return corr_rows_kernel<float2, 24, true, Plan1536>;
+    if (Plan2048::matches(P, true)) return corr_rows_kernel<float2, 8, true, Plan2048>;
+    if (Plan2560::matches(P, true)) return corr_rows_kernel<float2, 20, true, Plan2560>;
+    if (Plan3072::matches(P, true)) return corr_rows_kernel<float2, 24, true, Plan3072>;
+    if (Plan4096::matches(P, true)) return corr_rows_kernel<float2, 16, true, Plan4096>;
+    if (Plan5120::matches(P, true)) return corr_rows_kernel<float2, 20, true, Plan5120>;
+    if (Plan6144::matches(P, true)) return corr_rows_kernel<float2, 24, true, Plan6144>;
+    if (Plan8192::matches(P, true)) return corr_rows_kernel<float2, 16, true, Plan8192>;
+    if (Plan10240::matches(P, true)) return corr_rows_kernel<float2, 20, true, Plan10240>;
+    return nullptr;
+}
+
 hipError_t launch_corr_rows(const CorrArgs &a, bool self, double2 *apart_rows, hipStream_t stream) {
     if (a.plan.N1 < 2) return hipErrorInvalidValue;
-    const RowsFn k = self ? rows_fn<float2, true>(a.plan.E) : rows_fn<double2, false>(a.plan.E);
+    RowsFn k = self ? rows_fixed_self(a.plan) : nullptr;
+    if (!k) k = self ? rows_fn<float2, true>(a.plan.E) : rows_fn<double2, false>(a.plan.E);
     if (!k) return hipErrorInvalidValue;
     // the transform data of one row; the block sum reuses its start
     const size_t lds = (self ? sizeof(float2) : sizeof(double2)) * (size_t)a.plan.Lh;

@@ -343,8 +343,9 @@ struct StaticPlan {
         for (int i = 1; i < p; ++i) o += ns(i) * (unsigned)kRadix[i];
         return o;
     }
-    static bool matches(const FftPlan &P) {
-        if (P.Lh != (int)Lh || P.T != (int)T || P.npass != npass || P.N1 != 1) return false;
+    // rows: the plan of one row of a long series (any N1) rather than a one-block plan
+    static bool matches(const FftPlan &P, bool rows = false) {
+        if (P.Lh != (int)Lh || P.T != (int)T || P.npass != npass || (rows ? P.N1 < 2 : P.N1 != 1)) return false;
         for (int i = 0; i < npass; ++i)
             if (P.radix[i] != kRadix[i] || P.ns[i] != (int)ns(i) || (i > 0 && P.toff[i] != (int)toff(i)))
                 return false;
@@ -356,25 +357,38 @@ template <typename C, int E, typename PT, typename Input>
 __device__ __forceinline__ void spass_first(C (&v)[E], C *buf, const Input &in, unsigned ut) {
     constexpr int R = PT::radix(0);
     constexpr unsigned T = PT::T, stride = PT::Lh / R;
-    static_assert(E % R == 0 && Input::kBounded, "first pass of a fixed plan");
-    // the coordinate loads of half the pass's points in flight before the
-    // first is used, those of the other half issued once the first half is
-    // being consumed (all at once: 128 VGPRs and a spill)
-    float xs[E], ys[E], zs[E];
+    static_assert(E % R == 0, "first pass of a fixed plan");
+    if constexpr (!Input::kSplitLoad) {
+        // a row of a long series: every point of the row exists, formed in one call
 #pragma unroll
-    for (int e = 0; e < E / 2; ++e) in.load(ut + (e / R) * T + (e % R) * stride, xs[e], ys[e], zs[e]);
+        for (int i = 0; i < E / R; ++i) {
 #pragma unroll
-    for (int i = 0; i < E / R; ++i) {
-        if (i == (E / R) / 2) {
+            for (int q = 0; q < R; ++q) v[i * R + q] = in(ut + i * T + q * stride);
+            Dft<R, C>::run(&v[i * R]);
 #pragma unroll
-            for (int e = E / 2; e < E; ++e) in.load(ut + (e / R) * T + (e % R) * stride, xs[e], ys[e], zs[e]);
+            for (int q = 0; q < R; ++q) buf[ut * R + i * T * R + q] = v[i * R + q];
         }
+    } else {
+        static_assert(Input::kBounded, "first pass of a fixed plan");
+        // the coordinate loads of half the pass's points in flight before the
+        // first is used, those of the other half issued once the first half is
+        // being consumed (all at once: 128 VGPRs and a spill)
+        float xs[E], ys[E], zs[E];
 #pragma unroll
-        for (int q = 0; q < R; ++q)
-            v[i * R + q] = in.at(ut + i * T + q * stride, xs[i * R + q], ys[i * R + q], zs[i * R + q]);
-        Dft<R, C>::run(&v[i * R]);
+        for (int e = 0; e < E / 2; ++e) in.load(ut + (e / R) * T + (e % R) * stride, xs[e], ys[e], zs[e]);
 #pragma unroll
-        for (int q = 0; q < R; ++q) buf[ut * R + i * T * R + q] = v[i * R + q];
+        for (int i = 0; i < E / R; ++i) {
+            if (i == (E / R) / 2) {
+#pragma unroll
+                for (int e = E / 2; e < E; ++e) in.load(ut + (e / R) * T + (e % R) * stride, xs[e], ys[e], zs[e]);
+            }
+#pragma unroll
+            for (int q = 0; q < R; ++q)
+                v[i * R + q] = in.at(ut + i * T + q * stride, xs[i * R + q], ys[i * R + q], zs[i * R + q]);
+            Dft<R, C>::run(&v[i * R]);
+#pragma unroll
+            for (int q = 0; q < R; ++q) buf[ut * R + i * T * R + q] = v[i * R + q];
+        }
     }
     __syncthreads();
 }
@@ -840,6 +854,7 @@ struct SelfRows {
 struct SelfInput : SelfRows {
     static constexpr bool kPretwiddled = false;
     static constexpr bool kBounded = true;  // 0 past NF without a branch
+    static constexpr bool kSplitLoad = true;  // load() then at()
     double qx, qy, qz;
     float b;
     // the sample at frame n from its loaded coordinates
@@ -863,6 +878,7 @@ struct SelfInput : SelfRows {
 struct SelfInputShifted : SelfRows {
     static constexpr bool kPretwiddled = true;
     static constexpr bool kBounded = true;
+    static constexpr bool kSplitLoad = true;
     double qx, qy, qz, invL;
     float b;
     __device__ __forceinline__ float2 at(unsigned n, float xs, float ys, float zs) const {
@@ -982,6 +998,7 @@ template <typename C, typename Src>
 struct RowInput {
     static constexpr bool kPretwiddled = true;
     static constexpr bool kBounded = false;
+    static constexpr bool kSplitLoad = false;
     Src src;
     const C *tw;
     unsigned L, NF, Lh;
