@@ -252,57 +252,130 @@ __global__ __launch_bounds__(split_threads<PT>(), (split_min_blocks<PT, E>())) v
         self_half_body<C, E, 0, PT>(a, apair, s, c, buf);
 }
 
-// Tail lags of the self path: one wavefront per pair; lane l holds x_l and
-// x_{NF-K+l} in LDS, lane k sums S(NF-1-k) = sum_{t<=k} conj(x_t) x_{t+K-1-k+NF-K}
-// in FP64 (float products are exact in FP64).  Blocks stride over the pairs
-// of a shell; per-lane sums are combined over the block's wavefronts.
-__global__ __launch_bounds__(256) void self_tail_kernel(CorrArgs a, double2 *__restrict__ tail, int tblocks) {
-    __shared__ double2 xs[4][2][kTailLags];
-    __shared__ double2 part[4][kTailLags];
-    const int s = blockIdx.y, b = blockIdx.x;
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+// Tail lags of the self path in FP64 (float products are exact in FP64).
+// With A_t = x_t and B_u = x_{NF-K+u} (t, u < K, zero beyond K),
+// S(NF-1-k) = sum_t conj(A_t) B_{t+d}, d = K-1-k: the sums along the
+// diagonals d >= 0 of M = sum over pairs of conj(A) B^T.  Each block sums
+// its pairs' M over the 136 tiles (4 x 4, tile row <= tile column) of the
+// 64 x 64 upper triangle: threads 0-127 own tiles 0-127 for every staged
+// pair; the last wavefront owns tiles 128-135 eight times over, its lane l
+// taking tile 128 + l % 8 for staged pair l / 8 only, so no lane idles.
+// Each thread does 16 complex products per pair from 8 LDS reads.  Pairs are
+// staged eight at a time, one per wavefront at a time (the pair is
+// wave-uniform, so its coordinate rows are scalar buffer resources).  The
+// block then folds each thread's tile onto its seven diagonals and sums the
+// diagonals in a fixed order.
+constexpr int kTailThreads = 192;
+constexpr int kTailBatch = 8;
+constexpr int kTailPairSlots = 128;  // A then B of one staged pair
+
+__device__ __forceinline__ int tail_tile_index(int I, int J) { return I * 16 - (I * (I - 1)) / 2 + (J - I); }
+
+// The partial-sum slot (thread) of tile `tile` for pair class `i` (0 for tiles below 128).
+__device__ __forceinline__ int tail_partial_slot(int tile, int i) { return tile < 128 ? tile : 128 + 8 * i + (tile - 128); }
+
+__global__ __launch_bounds__(kTailThreads) void self_tail_kernel(CorrArgs a, double2 *__restrict__ tail,
+                                                                 int tblocks) {
+    static_assert(kTailLags == 64, "the tiling covers a 64 x 64 product");
+    constexpr int kStage = kTailBatch * kTailPairSlots, kParts = kTailThreads * 7;
+    __shared__ double2 sh[kStage > kParts ? kStage : kParts];  // staged pairs, then tile partials
+    const int s = blockIdx.y, b = blockIdx.x, tid = threadIdx.x;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
+    const bool last = wave == 2;
+    const int tile = last ? 128 + (lane & 7) : tid;
+    const int p0 = last ? lane >> 3 : 0, pn = last ? 1 : kTailBatch;
     const int K = min(kTailLags, a.NF);
     const long npairs = (long)a.NA * a.NM;
     const size_t plane = a.NAp * a.NFp;
-    double sr = 0.0, si = 0.0;
-    for (long pr = (long)b * 4 + wave; pr < npairs; pr += (long)tblocks * 4) {
-        const int ja = (int)(pr / a.NM), m = (int)(pr % a.NM);
-        const int j = a.aidx ? a.aidx[ja] : ja;
-        const double *q = a.qrev + ((size_t)s * a.NM + m) * 3;
-        SelfInput in;
-        in.set(a.coords + (size_t)j * a.NFp, plane, a.NF);
-        in.qx = q[0];
-        in.qy = q[1];
-        in.qz = q[2];
-        in.b = a.b[(size_t)(a.brow_per_shell ? s : 0) * a.NAp + j];
-        if (lane < K) {
-            const float2 u0 = in((unsigned)lane), u1 = in((unsigned)(a.NF - K + lane));
-            xs[wave][0][lane] = make_double2(u0.x, u0.y);
-            xs[wave][1][lane] = make_double2(u1.x, u1.y);
+    int I = 0, rest = tile;
+    while (rest >= 16 - I) {
+        rest -= 16 - I;
+        ++I;
+    }
+    const int J = I + rest;
+    double mr[4][4], mi[4][4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) mr[r][c] = mi[r][c] = 0.0;
+    const long nbatch = (npairs + kTailBatch - 1) / kTailBatch;
+    for (long bt = b; bt < nbatch; bt += tblocks) {
+        __syncthreads();  // the previous batch is consumed
+        for (int p = wave; p < kTailBatch; p += 3) {
+            const long pr = bt * kTailBatch + p;
+            float2 u0 = make_float2(0.f, 0.f), u1 = u0;
+            if (pr < npairs) {
+                const int ja = (int)(pr / a.NM), m = (int)(pr % a.NM);
+                const int j = a.aidx ? a.aidx[ja] : ja;
+                const double *q = a.qrev + ((size_t)s * a.NM + m) * 3;
+                SelfInput in;
+                in.set(a.coords + (size_t)j * a.NFp, plane, a.NF);
+                in.qx = q[0];
+                in.qy = q[1];
+                in.qz = q[2];
+                in.b = a.b[(size_t)(a.brow_per_shell ? s : 0) * a.NAp + j];
+                if (lane < K) {
+                    u0 = in((unsigned)lane);
+                    u1 = in((unsigned)(a.NF - K + lane));
+                }
+            }
+            sh[p * kTailPairSlots + lane] = make_double2(u0.x, u0.y);
+            sh[p * kTailPairSlots + 64 + lane] = make_double2(u1.x, u1.y);
         }
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-        if (lane < K) {
-            for (int t = 0; t <= lane; ++t) {
-                const double2 x0 = xs[wave][0][t], x1 = xs[wave][1][t + K - 1 - lane];
-                sr = fma(x0.x, x1.x, sr);
-                sr = fma(x0.y, x1.y, sr);
-                si = fma(x0.x, x1.y, si);
-                si = fma(-x0.y, x1.x, si);
+        __syncthreads();
+        for (int i = 0; i < pn; ++i) {
+            const double2 *pp = sh + (p0 + i) * kTailPairSlots;
+            double2 x[4], y[4];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                x[r] = pp[4 * I + r];
+                y[r] = pp[64 + 4 * J + r];
+            }
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+#pragma unroll
+                for (int c = 0; c < 4; ++c) {
+                    mr[r][c] = fma(x[r].x, y[c].x, mr[r][c]);
+                    mr[r][c] = fma(x[r].y, y[c].y, mr[r][c]);
+                    mi[r][c] = fma(x[r].x, y[c].y, mi[r][c]);
+                    mi[r][c] = fma(-x[r].y, y[c].x, mi[r][c]);
+                }
+        }
+    }
+    __syncthreads();
+    // entry (r, c) of tile (I, J) lies on diagonal 4 (J - I) + c - r
+#pragma unroll
+    for (int e = 0; e < 7; ++e) {
+        double pr_ = 0.0, pi_ = 0.0;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int c = e - 3 + r;
+            if (c >= 0 && c < 4) {
+                pr_ += mr[r][c];
+                pi_ += mi[r][c];
             }
         }
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        sh[tid * 7 + e] = make_double2(pr_, pi_);
     }
-    part[wave][lane] = make_double2(sr, si);
     __syncthreads();
-    if (wave == 0) {
-        double2 t = part[0][lane];
-        for (int w = 1; w < 4; ++w) {
-            t.x += part[w][lane].x;
-            t.y += part[w][lane].y;
+    if (tid < kTailLags) {
+        const int k = tid, d = K - 1 - k;
+        double tr = 0.0, ti = 0.0;
+        if (d >= 0) {
+            for (int D = 0; D < 16; ++D) {
+                const int e = d - 4 * D + 3;
+                if (e < (D == 0 ? 3 : 0) || e > 6) continue;  // tiles on the diagonal keep c >= r
+                for (int I2 = 0; I2 + D < 16; ++I2) {
+                    const int t = tail_tile_index(I2, I2 + D);
+                    for (int i = 0; i < (t < 128 ? 1 : 8); ++i) {
+                        const double2 v = sh[tail_partial_slot(t, i) * 7 + e];
+                        tr += v.x;
+                        ti += v.y;
+                    }
+                }
+            }
         }
-        tail[((size_t)s * tblocks + b) * kTailLags + lane] = t;
+        tail[((size_t)s * tblocks + b) * kTailLags + k] = make_double2(tr, ti);
     }
 }
 
@@ -581,8 +654,8 @@ hipError_t launch_corr_self_split(const CorrArgs &a, double2 *apair, hipStream_t
 }
 
 hipError_t launch_self_tail(const CorrArgs &a, double2 *tail, int tblocks, hipStream_t stream) {
-    hipLaunchKernelGGL(self_tail_kernel, dim3((unsigned)tblocks, (unsigned)a.nshells), dim3(256), 0, stream, a, tail,
-                       tblocks);
+    hipLaunchKernelGGL(self_tail_kernel, dim3((unsigned)tblocks, (unsigned)a.nshells), dim3(kTailThreads), 0, stream, a,
+                       tail, tblocks);
     return hipGetLastError();
 }
 

@@ -343,6 +343,38 @@ struct StaticPlan {
         for (int i = 1; i < p; ++i) o += ns(i) * (unsigned)kRadix[i];
         return o;
     }
+    // The LDS layout of the buffer pass p writes and pass p + 1 reads: float2
+    // index o lives at swz(p, o) = o ^ (((o >> S) mod 2^n) << B).  Pass p
+    // writes o = (j - kk) R + kk + q Ns (kk = j mod Ns): the bits below
+    // log2 Ns vary over the lanes, those from log2 Ns to log2(Ns R) hold q,
+    // the same for every lane of one store, so with Ns < 16 a store covers
+    // only Ns (or, 16-byte stores of the first pass, 2 R) of the 16 float2
+    // positions of a 32-bank row.  XOR-ing the bits from B = max(1, log2 Ns)
+    // to 3 with lane bits from S = max(4, log2(Ns R)) up spreads every store
+    // over all 16 (bit 0 is kept, so pairs stay adjacent for 16-byte stores).
+    // Readers take ut + C with C a multiple of 2^(S+n) (required below), so
+    // their address is the lane's swz(p, ut) plus the same immediate C.
+    static constexpr int lg2(unsigned x) {
+        int l = 0;
+        while ((1u << l) < x) ++l;
+        return l;
+    }
+    static constexpr bool pow2(unsigned x) { return (x & (x - 1)) == 0; }
+    static constexpr int kSwzTop = 4;
+    static constexpr int swz_b(int p) { return lg2(ns(p)) < 1 ? 1 : lg2(ns(p)); }
+    static constexpr int swz_s(int p) {
+        return lg2(ns(p) * (unsigned)radix(p)) > kSwzTop ? lg2(ns(p) * (unsigned)radix(p)) : kSwzTop;
+    }
+    static constexpr int swz_n(int p) {
+        if (p + 1 >= npass || swz_b(p) >= kSwzTop || ns(p) > T || !pow2(ns(p)) || !pow2((unsigned)radix(p)))
+            return 0;
+        const unsigned m = 1u << (swz_s(p) + kSwzTop - swz_b(p));
+        if (T % m != 0 || (Lh / (unsigned)radix(p + 1)) % m != 0) return 0;
+        return kSwzTop - swz_b(p);
+    }
+    static constexpr unsigned swz(int p, unsigned o) {
+        return swz_n(p) == 0 ? o : o ^ (((o >> swz_s(p)) & ((1u << swz_n(p)) - 1u)) << swz_b(p));
+    }
     // rows: the plan of one row of a long series (any N1) rather than a one-block plan
     static bool matches(const FftPlan &P, bool rows = false) {
         if (P.Lh != (int)Lh || P.T != (int)T || P.npass != npass || (rows ? P.N1 < 2 : P.N1 != 1)) return false;
@@ -366,7 +398,7 @@ __device__ __forceinline__ void spass_first(C (&v)[E], C *buf, const Input &in, 
             for (int q = 0; q < R; ++q) v[i * R + q] = in(ut + i * T + q * stride);
             Dft<R, C>::run(&v[i * R]);
 #pragma unroll
-            for (int q = 0; q < R; ++q) buf[ut * R + i * T * R + q] = v[i * R + q];
+            for (int q = 0; q < R; ++q) buf[PT::swz(0, ut * R + (q & ~1)) + (q & 1) + i * T * R] = v[i * R + q];
         }
     } else {
         static_assert(Input::kBounded, "first pass of a fixed plan");
@@ -387,7 +419,7 @@ __device__ __forceinline__ void spass_first(C (&v)[E], C *buf, const Input &in, 
                 v[i * R + q] = in.at(ut + i * T + q * stride, xs[i * R + q], ys[i * R + q], zs[i * R + q]);
             Dft<R, C>::run(&v[i * R]);
 #pragma unroll
-            for (int q = 0; q < R; ++q) buf[ut * R + i * T * R + q] = v[i * R + q];
+            for (int q = 0; q < R; ++q) buf[PT::swz(0, ut * R + (q & ~1)) + (q & 1) + i * T * R] = v[i * R + q];
         }
     }
     __syncthreads();
@@ -439,10 +471,11 @@ __device__ __forceinline__ void spass_mid(C (&v)[E], C *buf, const MidTwiddles<C
         C base[R];
 #pragma unroll
         for (int q = 1; q < R; ++q) base[q] = twiddle_load(twp_of(mt), ut * R, PT::toff(p) + q);
+        const unsigned rb = PT::swz(p - 1, ut);
 #pragma unroll
         for (int i = 0; i < E / R; ++i)
 #pragma unroll
-            for (int q = 0; q < R; ++q) v[i * R + q] = buf[ut + i * T + q * stride];
+            for (int q = 0; q < R; ++q) v[i * R + q] = buf[rb + i * T + q * stride];
         __syncthreads();
 #pragma unroll
         for (int i = 0; i < E / R; ++i) {
@@ -465,10 +498,11 @@ __device__ __forceinline__ void spass_mid(C (&v)[E], C *buf, const MidTwiddles<C
         // every group of the thread has the twiddle index kk = ut mod Ns
         const unsigned kk = ut & (Ns - 1);
         const C(&ws)[8] = mt.w[p];
+        const unsigned rb = PT::swz(p - 1, ut);
 #pragma unroll
         for (int i = 0; i < E / R; ++i)
 #pragma unroll
-            for (int q = 0; q < R; ++q) v[i * R + q] = buf[ut + i * T + q * stride];
+            for (int q = 0; q < R; ++q) v[i * R + q] = buf[rb + i * T + q * stride];
         __syncthreads();
         const unsigned ob = (ut - kk) * R + kk;
 #pragma unroll
@@ -477,7 +511,12 @@ __device__ __forceinline__ void spass_mid(C (&v)[E], C *buf, const MidTwiddles<C
             for (int q = 1; q < R; ++q) v[i * R + q] = cmul(v[i * R + q], ws[q]);
             Dft<R, C>::run(&v[i * R]);
 #pragma unroll
-            for (int q = 0; q < R; ++q) buf[ob + i * T * R + q * Ns] = v[i * R + q];
+            for (int q = 0; q < R; ++q) {
+                if constexpr (PT::swz_n(p) > 0)
+                    buf[PT::swz(p, ob + q * Ns) + i * T * R] = v[i * R + q];
+                else
+                    buf[ob + i * T * R + q * Ns] = v[i * R + q];
+            }
         }
         __syncthreads();
     }
@@ -496,10 +535,11 @@ __device__ __forceinline__ void spass_last(C (&v)[E], Acc (&acc)[2][E], const C 
     C base[R];
 #pragma unroll
     for (int q = 1; q < R; ++q) base[q] = twiddle_load(twp, ut * R, PT::toff(p) + q);
+    const unsigned rb = PT::swz(p - 1, ut);
 #pragma unroll
     for (int i = 0; i < E / R; ++i)
 #pragma unroll
-        for (int q = 0; q < R; ++q) v[i * R + q] = buf[ut + i * T + q * stride];
+        for (int q = 0; q < R; ++q) v[i * R + q] = buf[rb + i * T + q * stride];
     __syncthreads();  // the buffer may now be overwritten by the next series
 #pragma unroll
     for (int i = 0; i < E / R; ++i) {
@@ -576,8 +616,9 @@ __device__ __forceinline__ void spass_last_fused(C (&v)[E], Acc (&acc)[2][E], co
     C base[RB];
 #pragma unroll
     for (int b = 1; b < RB; ++b) base[b] = twiddle_load(twp, ut * RB, PT::toff(pb) + b);
+    const unsigned rb = PT::swz(pa - 1, ut);
 #pragma unroll
-    for (int q = 0; q < R; ++q) v[q] = buf[ut + q * T];
+    for (int q = 0; q < R; ++q) v[q] = buf[rb + q * T];
     __syncthreads();  // the buffer may now be overwritten by the next series
 #pragma unroll
     for (int a = 0; a < RA; ++a)
