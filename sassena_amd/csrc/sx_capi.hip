@@ -155,7 +155,7 @@ int round_filling_split(long base, int first, long resident, size_t NAp, size_t 
 // their range for one (frame tile, shell, group of 4 moment blocks): the rule
 // above, then the split that fills whole rounds of resident workgroups.
 int pick_multipole_split(sx_ctx *c, int kind, int ns, int nmo, int *split) {
-    const long base = sx::multipole_base_blocks(c->NFp, ns, nmo);
+    const long base = sx::multipole_base_blocks(c->NFp, ns, nmo, kind);
     *split = pick_split_blocks(c, base);
     if (c->cfg.atom_split > 0) return SX_OK;
     int per_cu = 0, cus = 0;

@@ -282,9 +282,11 @@ struct MultipoleArgs {
     const void *cj01 = nullptr;
 };
 hipError_t launch_multipole(const MultipoleArgs &m, hipStream_t stream);
-// Workgroups of the multipole kernel that fit one CU, and its workgroups per atom range.
+// Workgroups of the multipole kernel that fit one CU, and its workgroups per
+// atom range (kind 0, sphere: kSphereShells shells per workgroup).
+constexpr int kSphereShells = 2;
 hipError_t multipole_blocks_per_cu(int kind, int *blocks);
-int multipole_base_blocks(size_t NFp, int ns, int nmo);
+int multipole_base_blocks(size_t NFp, int ns, int nmo, int kind);
 // Bytes of the sphere-term planes for NAp x NFp (atom, frame) pairs.
 constexpr size_t kSphereTermBytes = sizeof(float) + 4 * sizeof(double);
 hipError_t launch_multipole_sphere_terms(const MultipoleArgs &m, void *sph, hipStream_t stream);

@@ -378,15 +378,19 @@ __global__ __launch_bounds__(256) void cylinder_bessel01_kernel(const float *__r
 // come from cylinder_bessel01_kernel (a.cj01).  Each is its own
 // instantiation, so the acos, atan, sincos and Bessel code of the paths a
 // kernel does not take adds nothing to its registers or code.
-template <int KIND, int SRC>
+// NSH: shells per block (sphere only): the Legendre functions and e^{i m phi}
+// of an (atom, frame) serve all of them; each keeps its own Bessel functions
+// and sums.
+template <int KIND, int SRC, int NSH = 1>
 __global__ __launch_bounds__(256, KIND == 0 ? 3 : 2) void multipole_kernel(MultipoleKernelArgs a) {
+    static_assert(NSH >= 1 && (KIND == 0 || NSH == 1), "several shells per block on the sphere only");
     constexpr int kind = KIND;
     constexpr bool PRE = SRC >= 1;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     // block = (frame tile, shell, group of 4 moment blocks + zgroups x atom range)
     const int ftile = blockIdx.x;
-    const int s = blockIdx.y;
+    const int s = blockIdx.y * NSH;  // first shell of the block
     const int zg = blockIdx.z % a.zgroups;
     const int sp = blockIdx.z / a.zgroups;
     const int mb = zg * 4 + wave;
@@ -394,14 +398,30 @@ __global__ __launch_bounds__(256, KIND == 0 ? 3 : 2) void multipole_kernel(Multi
     const int k0 = mb * kMoments;
     const int nk = min(kMoments, a.nmo - k0);
     const size_t t = (size_t)ftile * 64 + lane;
-    double ar[kMoments], ai[kMoments];
+    double ar[NSH][kMoments], ai[NSH][kMoments];
 #pragma unroll
-    for (int k = 0; k < kMoments; ++k) ar[k] = ai[k] = 0.0;
-    const float *brow = a.b + (a.per_shell_b ? (size_t)s * a.NAp : 0);
+    for (int h = 0; h < NSH; ++h)
+#pragma unroll
+        for (int k = 0; k < kMoments; ++k) ar[h][k] = ai[h][k] = 0.0;
+    // shells past the last (an odd count) read the last one's data with zero weight
+    int sh[NSH];
+    const float *brow[NSH];
+#pragma unroll
+    for (int h = 0; h < NSH; ++h) {
+        sh[h] = min(s + h, a.ns - 1);
+        brow[h] = a.b + (a.per_shell_b ? (size_t)sh[h] * a.NAp : 0);
+    }
     const size_t jend = min(a.NAp, (size_t)(sp + 1) * a.range);
     for (size_t j = (size_t)sp * a.range; j < jend; ++j) {
-        const float bj = brow[j];
-        if (bj == 0.f) continue;  // padding atoms and zero lengths add nothing
+        float bjs[NSH];
+        bool any = false;
+#pragma unroll
+        for (int h = 0; h < NSH; ++h) {
+            bjs[h] = s + h < a.ns ? brow[h][j] : 0.f;
+            any = any || bjs[h] != 0.f;
+        }
+        if (!any) continue;  // padding atoms and zero lengths add nothing
+        const float bj = bjs[0];
         // The moments' parameters, read again in every atom iteration through
         // an index the compiler cannot take for a constant: held across the
         // atom loop they fill the SGPRs and spill into VGPR lanes, with a
@@ -447,7 +467,6 @@ __global__ __launch_bounds__(256, KIND == 0 ? 3 : 2) void multipole_kernel(Multi
             z = a.Z[j * a.NFp + t];
         }
         if (kind == 0) {
-            const double ql = a.qinfo[s];
             double rf, ct, somx2, sp, cp;
             if constexpr (PRE) {
                 const size_t e = j * a.NFp + t;
@@ -459,22 +478,25 @@ __global__ __launch_bounds__(256, KIND == 0 ? 3 : 2) void multipole_kernel(Multi
             } else {
                 sphere_terms(x, y, z, rf, ct, somx2, sp, cp);
             }
-            // j_0, j_1 and 1/x of the argument once for all moments
-            const double xq = ql * rf;
-            double ix = 0.0, j0 = 1.0, j1 = 0.0;
-            if constexpr (SRC == 2) {
-                const double2 jj = a.cj01[((size_t)s * a.NAp + j) * a.NFp + t];
-                j0 = jj.x;
-                j1 = jj.y;
-                if (xq != 0.0) ix = 1.0 / xq;
-            } else {
-                sph_bessel01(xq, j0, j1, ix);
+            // per shell: j_0, j_1 and 1/x of the argument once for all moments
+            double jv[NSH][kMoments];
+#pragma unroll
+            for (int h = 0; h < NSH; ++h) {
+                const double xq = a.qinfo[sh[h]] * rf;
+                double ix = 0.0, j0 = 1.0, j1 = 0.0;
+                if constexpr (SRC == 2) {
+                    const double2 jj = a.cj01[((size_t)sh[h] * a.NAp + j) * a.NFp + t];
+                    j0 = jj.x;
+                    j1 = jj.y;
+                    if (xq != 0.0) ix = 1.0 / xq;
+                } else {
+                    sph_bessel01(xq, j0, j1, ix);
+                }
+                if (sorted)
+                    sph_bessel_moments<true>(ls, lhi, xq, ix, j0, j1, jv[h]);
+                else
+                    sph_bessel_moments<false>(ls, lhi, xq, ix, j0, j1, jv[h]);
             }
-            double jv[kMoments];
-            if (sorted)
-                sph_bessel_moments<true>(ls, lhi, xq, ix, j0, j1, jv);
-            else
-                sph_bessel_moments<false>(ls, lhi, xq, ix, j0, j1, jv);
             // e^{i am phi} carried from moment to moment: one product by
             // e^{+-i phi} when |m| steps by one (the reference's lists),
             // repeated products from 1 otherwise
@@ -484,7 +506,6 @@ __global__ __launch_bounds__(256, KIND == 0 ? 3 : 2) void multipole_kernel(Multi
             for (int k = 0; k < kMoments; ++k) {
                 if (k >= nk) break;
                 const int l = ls[k], m = ms[k], am = m < 0 ? -m : m;
-                const double jl = jv[k];
                 // the Y_lm normalisation and 4 pi scale the finished sums
                 const double th = plm_scaled(l, am, ct, somx2, cms[k], a.plm_ab, a.plm_l1);
                 if (am == aprev + 1) {
@@ -512,13 +533,16 @@ __global__ __launch_bounds__(256, KIND == 0 ? 3 : 2) void multipole_kernel(Multi
                     yi = -sg * yi;
                 }
                 // 4 pi i^l b j_l conj(Y)
-                const double f = (double)bj * jl;
-                const double cr = f * yr, ci = -f * yi;
-                switch (l & 3) {
-                    case 0: ar[k] += cr; ai[k] += ci; break;
-                    case 1: ar[k] -= ci; ai[k] += cr; break;
-                    case 2: ar[k] -= cr; ai[k] -= ci; break;
-                    default: ar[k] += ci; ai[k] -= cr; break;
+#pragma unroll
+                for (int h = 0; h < NSH; ++h) {
+                    const double f = (double)bjs[h] * jv[h][k];
+                    const double cr = f * yr, ci = -f * yi;
+                    switch (l & 3) {
+                        case 0: ar[h][k] += cr; ai[h][k] += ci; break;
+                        case 1: ar[h][k] -= ci; ai[h][k] += cr; break;
+                        case 2: ar[h][k] -= cr; ai[h][k] -= ci; break;
+                        default: ar[h][k] += ci; ai[h][k] -= cr; break;
+                    }
                 }
             }
         } else {
@@ -591,17 +615,21 @@ __global__ __launch_bounds__(256, KIND == 0 ? 3 : 2) void multipole_kernel(Multi
                 } else if (m == 3) {
                     ci = sqrt(0.5) * 2.0 * (-sgl) * jn[k] * sn;
                 }
-                ar[k] += (double)bj * (cr * exc - ci * exs);
-                ai[k] += (double)bj * (cr * exs + ci * exc);
+                ar[0][k] += (double)bj * (cr * exc - ci * exs);
+                ai[0][k] += (double)bj * (cr * exs + ci * exc);
             }
         }
     }
 #pragma unroll
-    for (int k = 0; k < kMoments; ++k) {
-        if (k >= nk) break;
-        const double norm = kind == 1 ? sqrt(2.0 * kPi) : a.ynorm[2 * (k0 + k) + 1];
-        const size_t v = ((size_t)sp * a.ns + s) * a.nmo + k0 + k;
-        a.amp[v * a.NFp + t] = make_float2((float)(norm * ar[k]), (float)(norm * ai[k]));
+    for (int h = 0; h < NSH; ++h) {
+        if (s + h >= a.ns) break;
+#pragma unroll
+        for (int k = 0; k < kMoments; ++k) {
+            if (k >= nk) break;
+            const double norm = kind == 1 ? sqrt(2.0 * kPi) : a.ynorm[2 * (k0 + k) + 1];
+            const size_t v = ((size_t)sp * a.ns + s + h) * a.nmo + k0 + k;
+            a.amp[v * a.NFp + t] = make_float2((float)(norm * ar[h][k]), (float)(norm * ai[h][k]));
+        }
     }
 }
 
@@ -648,9 +676,10 @@ hipError_t launch_multipole(const MultipoleArgs &m, hipStream_t stream) {
     }
     if ((size_t)k.zgroups * k.split > 65535) return hipErrorInvalidValue;
     dim3 grid((unsigned)(m.NFp / 64), (unsigned)m.ns, (unsigned)(k.zgroups * k.split));
-    if (m.kind == 0 && k.srf && k.cj01)
-        hipLaunchKernelGGL((multipole_kernel<0, 2>), grid, dim3(256), 0, stream, k);
-    else if (m.kind == 0 && k.srf)
+    if (m.kind == 0 && k.srf && k.cj01) {
+        grid.y = (unsigned)((m.ns + kSphereShells - 1) / kSphereShells);
+        hipLaunchKernelGGL((multipole_kernel<0, 2, kSphereShells>), grid, dim3(256), 0, stream, k);
+    } else if (m.kind == 0 && k.srf)
         hipLaunchKernelGGL((multipole_kernel<0, 1>), grid, dim3(256), 0, stream, k);
     else if (m.kind == 0)
         hipLaunchKernelGGL((multipole_kernel<0, 0>), grid, dim3(256), 0, stream, k);
@@ -664,13 +693,14 @@ hipError_t launch_multipole(const MultipoleArgs &m, hipStream_t stream) {
 }
 
 hipError_t multipole_blocks_per_cu(int kind, int *blocks) {
-    return kind == 0 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, multipole_kernel<0, 2>, 256, 0)
+    return kind == 0 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, multipole_kernel<0, 2, kSphereShells>, 256, 0)
                      : hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, multipole_kernel<1, 2>, 256, 0);
 }
 
-int multipole_base_blocks(size_t NFp, int ns, int nmo) {
+int multipole_base_blocks(size_t NFp, int ns, int nmo, int kind) {
     const int mblocks = (nmo + kMoments - 1) / kMoments;
-    return (int)(NFp / 64) * ns * ((mblocks + 3) / 4);
+    const int rows = kind == 0 ? (ns + kSphereShells - 1) / kSphereShells : ns;
+    return (int)(NFp / 64) * rows * ((mblocks + 3) / 4);
 }
 
 hipError_t launch_multipole_sphere_terms(const MultipoleArgs &m, void *sph, hipStream_t stream) {
