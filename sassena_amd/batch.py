@@ -99,7 +99,8 @@ class Batch:
 
     def _results(self, k):
         fqt, fq, fq2 = self.outputs[k]
-        return fqt[..., 0] + 1j * fqt[..., 1], fq[:, 0] + 1j * fq[:, 1], fq2[:, 0] + 1j * fq2[:, 1]
+        # copies: the (re, im) buffers are the next run's outputs
+        return tuple(a.view(np.complex128)[..., 0].copy() for a in (fqt, fq, fq2))
 
     def run(self):
         self._check(self._L.sx_batch_run(self._h), "sx_batch_run")

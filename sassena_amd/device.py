@@ -24,6 +24,11 @@ def _dp(a):
     return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
 
 
+def _cx(a):
+    """(..., 2) float64 (re, im) pairs as complex128 values, without a copy."""
+    return a.view(np.complex128)[..., 0]
+
+
 def _fp(a):
     return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
 
@@ -120,7 +125,7 @@ class ScatterDevice:
             bp = _dp(bs)
         self._check(self._L.sx_compute_shells(self._h, ns, _dp(q), NM, bp, _dp(fqt), _dp(fq), _dp(fq2)),
                     "sx_compute_shells")
-        return (fqt[..., 0] + 1j * fqt[..., 1], fq[:, 0] + 1j * fq[:, 1], fq2[:, 0] + 1j * fq2[:, 1])
+        return (_cx(fqt), _cx(fq), _cx(fq2))
 
     def compute_oriented(self, qlen, u, b_shell=None, noalloc=False):
         """Shells sharing one orientation set: q_{s,m} = qlen[s] * u[m].
@@ -143,7 +148,7 @@ class ScatterDevice:
         fn = self._L.sx_compute_shells_oriented_noalloc if noalloc else self._L.sx_compute_shells_oriented
         self._check(fn(self._h, ns, _dp(ql), _dp(uu), NM, bp, _dp(fqt), _dp(fq), _dp(fq2)),
                     "sx_compute_shells_oriented" + ("_noalloc" if noalloc else ""))
-        return (fqt[..., 0] + 1j * fqt[..., 1], fq[:, 0] + 1j * fq[:, 1], fq2[:, 0] + 1j * fq2[:, 1])
+        return (_cx(fqt), _cx(fq), _cx(fq2))
 
     def compute_multipole(self, kind, qv, moments, axis=(0.0, 0.0, 1.0), b_shell=None):
         """Multipole averaging: kind "sphere" or "cylinder"; qv [nshells][3];
@@ -165,7 +170,7 @@ class ScatterDevice:
         self._check(self._L.sx_compute_multipole(self._h, kinds[kind], ns, _dp(q), _dp(ax),
                                                  mo.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), mo.shape[0], bp,
                                                  _dp(fqt), _dp(fq), _dp(fq2)), "sx_compute_multipole")
-        return (fqt[..., 0] + 1j * fqt[..., 1], fq[:, 0] + 1j * fq[:, 1], fq2[:, 0] + 1j * fq2[:, 1])
+        return (_cx(fqt), _cx(fq), _cx(fq2))
 
     def timing(self):
         t = _lib.sx_timing()
