@@ -130,6 +130,10 @@ int pick_split(const sx_ctx *c, int V, int vb) {
 constexpr size_t kSplitAmpBudget = (size_t)4 << 30;
 // Largest buffer of per-(atom, frame) spherical terms of the sphere multipole path.
 constexpr size_t kSphereTermBudget = (size_t)16 << 30;
+// The sphere's table of j_l per (shell, degree, atom, frame): at most this
+// much, and only while this much device memory stays free beside it.
+constexpr size_t kBesselTableBudget = (size_t)64 << 30;
+constexpr size_t kBesselTableMargin = (size_t)4 << 30;
 
 // Splits from `first` to 16 (at least 64 atoms a range, partial amplitude
 // sets of split_bytes each within kSplitAmpBudget) for `base` equal
@@ -149,6 +153,16 @@ int round_filling_split(long base, int first, long resident, size_t NAp, size_t 
         }
     }
     return split;
+}
+
+// Whether the sphere's Bessel table of `bytes` may be held: within its
+// budget, and with kBesselTableMargin of device memory left beside it.
+bool bessel_table_fits(sx_ctx *c, size_t bytes) {
+    if (bytes > kBesselTableBudget) return false;
+    if (c->mpjt.p && bytes <= c->mpjt.cap) return true;
+    size_t fr = 0, tot = 0;
+    if (hipMemGetInfo(&fr, &tot) != hipSuccess) return false;
+    return fr + c->mpjt.cap >= bytes + kBesselTableMargin;
 }
 
 // Atom ranges of the multipole kernel.  Its workgroups each sum every atom of
@@ -504,17 +518,26 @@ int run_batch(sx_ctx *c, int ns, size_t NM, const double *qv, const double *qlen
             const size_t sph_bytes =
                 (mp->kind == 0 ? sx::kSphereTermBytes : sx::kCylinderTermBytes) * c->NAp * c->NFp;
             const int terms = c->cfg.multipole_terms;
-            if (terms == 1 || (terms == 0 && sph_bytes <= kSphereTermBudget)) {
+            if (terms == 1 || terms == 3 || (terms == 0 && sph_bytes <= kSphereTermBudget)) {
                 SX_HIP(c, c->mpsph.ensure(sph_bytes));
                 if (mp->kind == 0)
                     SX_HIP(c, sx::launch_multipole_sphere_terms(ma, c->mpsph.p, c->stream));
                 else
                     SX_HIP(c, sx::launch_multipole_cylinder_terms(ma, c->mpsph.p, c->stream));
                 ma.sph = c->mpsph.p;
-                // the two lowest Bessel functions of every (shell, atom, frame)
-                // once per call (16 B each) when they fit the same budget
+                // sphere up to degree 16: every j_l of every (shell, atom,
+                // frame) once per call when the table fits; else the two
+                // lowest Bessel functions (16 B each) within the terms' budget
+                const int L1 = mp->plm_l1;
+                const size_t tab_bytes = sizeof(double) * (size_t)ns * (size_t)std::max(L1, 1) * c->NAp * c->NFp;
                 const size_t j01_bytes = sizeof(double2) * (size_t)ns * c->NAp * c->NFp;
-                if (j01_bytes <= kSphereTermBudget) {
+                if (mp->kind == 0 && terms != 3 && L1 >= 1 && L1 <= sx::kBesselTableL1 &&
+                    bessel_table_fits(c, tab_bytes)) {
+                    SX_HIP(c, c->mpjt.ensure(tab_bytes));
+                    SX_HIP(c, sx::launch_multipole_bessel_table(ma, L1, c->mpjt.p, c->stream));
+                    ma.jtab = c->mpjt.p;
+                    ma.jl1 = L1;
+                } else if (j01_bytes <= kSphereTermBudget) {
                     SX_HIP(c, c->mpj.ensure(j01_bytes));
                     SX_HIP(c, sx::launch_multipole_bessel01(ma, c->mpj.p, c->stream));
                     ma.cj01 = c->mpj.p;
@@ -789,7 +812,7 @@ int sx_create(const sx_config *cfg, sx_ctx **out) {
     if (cfg->dsp_type < SX_DSP_AUTOCORRELATE || cfg->dsp_type > SX_DSP_PLAIN) return SX_ERR_ARG;
     if (cfg->dsp_method != SX_METHOD_FFT && cfg->dsp_method != SX_METHOD_DIRECT) return SX_ERR_ARG;
     if (cfg->fft_halves < 0 || cfg->fft_halves > 3) return SX_ERR_ARG;
-    if (cfg->multipole_terms < 0 || cfg->multipole_terms > 2) return SX_ERR_ARG;
+    if (cfg->multipole_terms < 0 || cfg->multipole_terms > 3) return SX_ERR_ARG;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || cfg->device < 0 || cfg->device >= ndev) return SX_ERR_HIP;
     if (hipSetDevice(cfg->device) != hipSuccess) return SX_ERR_HIP;
@@ -818,7 +841,7 @@ void sx_destroy(sx_ctx *c) {
     if (!c) return;
     (void)hipSetDevice(c->cfg.device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
-    for (DevBuf *d : {&c->coords, &c->b, &c->twL, &c->twL64, &c->twp, &c->twp64, &c->tw64, &c->w, &c->aidx, &c->apair, &c->tail, &c->qrev, &c->qlen, &c->uvec, &c->brow, &c->bshell, &c->mpmom, &c->mpq, &c->mpsph, &c->mpj, &c->qres, &c->redbuf, &c->amp,
+    for (DevBuf *d : {&c->coords, &c->b, &c->twL, &c->twL64, &c->twp, &c->twp64, &c->tw64, &c->w, &c->aidx, &c->apair, &c->tail, &c->qrev, &c->qlen, &c->uvec, &c->brow, &c->bshell, &c->mpmom, &c->mpq, &c->mpsph, &c->mpj, &c->mpjt, &c->qres, &c->redbuf, &c->amp,
                       &c->ppart, &c->apart, &c->fqt, &c->fq, &c->fq2, &c->stage, &c->stage2})
         d->release();
     for (auto &e : c->ev)

@@ -83,7 +83,7 @@ struct sx_ctx {
     std::vector<int> aidx_h;  // "self": atoms with a nonzero length in the current call
     std::vector<float> b_h;   // host copy of the float factors of sx_set_factors
     sxh::DevBuf qrev, qlen, uvec, brow, bshell, amp, ppart, apart, fqt, fq, fq2, stage, stage2, mpmom, mpq, mpsph, mpj,
-        qres, redbuf;
+        mpjt, qres, redbuf;
     sx_timing timing{};
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     // stepping mode of the oriented shells of the current call (sx::shell_progression:

@@ -280,8 +280,17 @@ struct MultipoleArgs {
     // (cylinder) per (shell, atom, frame) from launch_multipole_bessel01, or
     // nullptr to form them in the kernel
     const void *cj01 = nullptr;
+    // sphere: j_l for l < jl1 per (shell, degree, atom, frame) from
+    // launch_multipole_bessel_table, or nullptr
+    const void *jtab = nullptr;
+    int jl1 = 0;
 };
 hipError_t launch_multipole(const MultipoleArgs &m, hipStream_t stream);
+// Degrees the sphere's Bessel table holds at most (l <= 16).
+constexpr int kBesselTableL1 = 17;
+// j_0 .. j_{L1-1} of every (shell, atom, frame) of a sphere call from its
+// terms (m.sph), into out[(s L1 + l) NAp NFp + atom NFp + frame] (FP64).
+hipError_t launch_multipole_bessel_table(const MultipoleArgs &m, int L1, void *out, hipStream_t stream);
 // Workgroups of the multipole kernel that fit one CU, and its workgroups per
 // atom range (kind 0, sphere: kSphereShells shells per workgroup).
 constexpr int kSphereShells = 2;

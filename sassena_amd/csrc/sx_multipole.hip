@@ -43,6 +43,8 @@ struct MultipoleKernelArgs {
     const double *sct, *sst, *ssp, *scp;  // cos theta, sin theta, sin phi, cos phi
     const float *cphi, *cz;  // cylinder terms (r in srf), float-rounded as the reference stages them
     const double2 *cj01;     // j_0, j_1 (sphere) or J_0, J_1 (cylinder) per (shell, atom, frame), or nullptr
+    const double *jtab;      // sphere: j_l, l < jl1, per (shell, degree, atom, frame), or nullptr
+    int jl1;
 };
 
 // j_l(x) for the moment degrees ls[0..kMoments) of a wavefront in one
@@ -351,6 +353,86 @@ __global__ __launch_bounds__(256) void sphere_bessel01_kernel(const float *__res
     }
 }
 
+// j_0 .. j_L (L < kBesselTableL1) of x in one recurrence, as
+// sph_bessel_moments forms each degree: j_0 and j_1 as given, upward to L
+// where x > L, else Miller's recurrence from L + 32 + 2 sqrt(2 L) normalised
+// to j_0 (or j_1 near a zero of j_0).  Loops over the table's fixed degree
+// count with guards on L keep every index a constant.
+__device__ __forceinline__ void sph_bessel_all(int L, double x, double ix, double j0, double j1,
+                                               double (&jv)[kBesselTableL1]) {
+#pragma unroll
+    for (int l = 0; l < kBesselTableL1; ++l) jv[l] = 0.0;
+    if (x == 0.0) {
+        jv[0] = 1.0;
+        return;
+    }
+    jv[0] = j0;
+    jv[1] = j1;
+    if (L < 2) return;
+    if (x > (double)L) {
+        double a = j0, bb = j1;  // j_{n-1}, j_n
+#pragma unroll
+        for (int n = 1; n + 1 < kBesselTableL1; ++n)
+            if (n < L) {
+                const double nx = (2.0 * n + 1.0) * ix * bb - a;
+                a = bb;
+                bb = nx;
+                jv[n + 1] = bb;
+            }
+        return;
+    }
+    int n = L + 32 + (int)(2.0 * sqrt(2.0 * (double)L));
+    double fp1 = 0.0, f = 1e-280;  // f_{n+1}, f_n
+    double cap[kBesselTableL1];
+#pragma unroll
+    for (int l = 0; l < kBesselTableL1; ++l) cap[l] = 0.0;
+    // f_n -> f_{n-1}, rescaling everything captured so far
+    auto step = [&]() {
+        const double fm1 = (2.0 * n + 1.0) * ix * f - fp1;
+        fp1 = f;
+        f = fm1;
+        --n;
+        if (fabs(f) > 1e250) {
+            f *= 1e-250;
+            fp1 *= 1e-250;
+#pragma unroll
+            for (int l = 0; l < kBesselTableL1; ++l) cap[l] *= 1e-250;
+        }
+    };
+    while (n > L) step();
+#pragma unroll
+    for (int l = kBesselTableL1 - 1; l >= 0; --l)
+        if (l <= L) {
+            cap[l] = f;  // f_l
+            if (l > 0) step();
+        }
+    // f = f_0 and fp1 = f_1, at the scale of every capture
+    const double sc = fabs(j0) >= fabs(j1) ? j0 / f : j1 / fp1;
+#pragma unroll
+    for (int l = 2; l < kBesselTableL1; ++l)
+        if (l <= L) jv[l] = cap[l] * sc;
+}
+
+// j_0 .. j_{L1-1} of |q_s| r for every (shell, atom, frame) of a sphere call
+// (out[(s L1 + l) n + e], e the (atom, frame) index), from the sphere terms:
+// formed once per call instead of once per moment block.
+__global__ __launch_bounds__(256) void sphere_bessel_table_kernel(const float *__restrict__ rr, size_t n,
+                                                                  const double *__restrict__ qinfo, int ns, int L1,
+                                                                  double *__restrict__ out) {
+    const size_t e = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= n) return;
+    const double rf = rr[e];
+    for (int s = 0; s < ns; ++s) {
+        double j0, j1, ix;
+        sph_bessel01(qinfo[s] * rf, j0, j1, ix);
+        double jv[kBesselTableL1];
+        sph_bessel_all(L1 - 1, qinfo[s] * rf, ix, j0, j1, jv);
+#pragma unroll
+        for (int l = 0; l < kBesselTableL1; ++l)
+            if (l < L1) out[((size_t)s * L1 + l) * n + e] = jv[l];
+    }
+}
+
 // J_0 and J_1 of q_r r for every (shell, atom, frame) of a cylinder call, from
 // the cylinder terms (r rounded to float, as the amplitude kernel forms p):
 // formed once per call instead of once per moment block (the library
@@ -478,8 +560,25 @@ __global__ __launch_bounds__(256, KIND == 0 ? 3 : 2) void multipole_kernel(Multi
             } else {
                 sphere_terms(x, y, z, rf, ct, somx2, sp, cp);
             }
-            // per shell: j_0, j_1 and 1/x of the argument once for all moments
+            // per shell: the moments' j_l from the table, or j_0, j_1 and 1/x
+            // of the argument once for all moments and the recurrence
             double jv[NSH][kMoments];
+            if constexpr (SRC == 3) {
+                const size_t plane = a.NAp * a.NFp, e = j * a.NFp + t;
+#pragma unroll
+                for (int h = 0; h < NSH; ++h) {
+                    double cur = 0.0;
+                    int lp = -1;
+#pragma unroll
+                    for (int k = 0; k < kMoments; ++k) {
+                        if (ls[k] != lp) {  // one load per distinct degree
+                            lp = ls[k];
+                            cur = a.jtab[((size_t)sh[h] * a.jl1 + lp) * plane + e];
+                        }
+                        jv[h][k] = cur;
+                    }
+                }
+            } else {
 #pragma unroll
             for (int h = 0; h < NSH; ++h) {
                 const double xq = a.qinfo[sh[h]] * rf;
@@ -496,6 +595,7 @@ __global__ __launch_bounds__(256, KIND == 0 ? 3 : 2) void multipole_kernel(Multi
                     sph_bessel_moments<true>(ls, lhi, xq, ix, j0, j1, jv[h]);
                 else
                     sph_bessel_moments<false>(ls, lhi, xq, ix, j0, j1, jv[h]);
+            }
             }
             // e^{i am phi} carried from moment to moment: one product by
             // e^{+-i phi} when |m| steps by one (the reference's lists),
@@ -662,6 +762,8 @@ hipError_t launch_multipole(const MultipoleArgs &m, hipStream_t stream) {
     k.sct = k.sst = k.ssp = k.scp = nullptr;
     k.cphi = k.cz = nullptr;
     k.cj01 = static_cast<const double2 *>(m.cj01);
+    k.jtab = static_cast<const double *>(m.jtab);
+    k.jl1 = m.jl1;
     if (m.sph && m.kind == 0) {
         k.srf = static_cast<const float *>(m.sph);
         const double *d = reinterpret_cast<const double *>(static_cast<const char *>(m.sph) + sizeof(float) * n);
@@ -676,7 +778,10 @@ hipError_t launch_multipole(const MultipoleArgs &m, hipStream_t stream) {
     }
     if ((size_t)k.zgroups * k.split > 65535) return hipErrorInvalidValue;
     dim3 grid((unsigned)(m.NFp / 64), (unsigned)m.ns, (unsigned)(k.zgroups * k.split));
-    if (m.kind == 0 && k.srf && k.cj01) {
+    if (m.kind == 0 && k.srf && k.jtab) {
+        grid.y = (unsigned)((m.ns + kSphereShells - 1) / kSphereShells);
+        hipLaunchKernelGGL((multipole_kernel<0, 3, kSphereShells>), grid, dim3(256), 0, stream, k);
+    } else if (m.kind == 0 && k.srf && k.cj01) {
         grid.y = (unsigned)((m.ns + kSphereShells - 1) / kSphereShells);
         hipLaunchKernelGGL((multipole_kernel<0, 2, kSphereShells>), grid, dim3(256), 0, stream, k);
     } else if (m.kind == 0 && k.srf)
@@ -719,6 +824,14 @@ hipError_t launch_multipole_cylinder_terms(const MultipoleArgs &m, void *terms, 
     const float *X = m.coords, *Y = m.coords + n, *Z = m.coords + 2 * n;
     hipLaunchKernelGGL(cylinder_terms_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, X, Y, Z, n,
                        m.base, rr, rr + n, rr + 2 * n);
+    return hipGetLastError();
+}
+
+hipError_t launch_multipole_bessel_table(const MultipoleArgs &m, int L1, void *out, hipStream_t stream) {
+    if (m.kind != 0 || L1 < 1 || L1 > kBesselTableL1) return hipErrorInvalidValue;
+    const size_t n = m.NAp * m.NFp;
+    hipLaunchKernelGGL(sphere_bessel_table_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream,
+                       static_cast<const float *>(m.sph), n, m.qinfo, m.ns, L1, static_cast<double *>(out));
     return hipGetLastError();
 }
 

@@ -43,13 +43,16 @@ def run(kind, frames, b, qv, moments, axis=(0.0, 0.0, 1.0), dsp="autocorrelate",
         return dev.compute_multipole(kind, qv, moments, axis=axis, b_shell=b_shell)
 
 
-@pytest.mark.parametrize("terms", [1, 2])
+@pytest.mark.parametrize("terms", [1, 2, 3])
 @pytest.mark.parametrize("dsp,method", [("autocorrelate", "fftw"), ("autocorrelate", "direct"), ("square", "fftw"),
                                         ("plain", "fftw")])
 def test_sphere_against_oracle(oracle_mod, dsp, method, terms):
-    """Both sources of the spherical coordinates: the per-(atom, frame)
-    pre-pass (1) and the kernel forming them itself (2, the path of
-    trajectories whose terms exceed 16 GB)."""
+    """Every source of the spherical coordinates and Bessel functions: the
+    per-(atom, frame) pre-pass with the table of j_l per (shell, atom,
+    frame) (1), the kernel forming coordinates and Bessel functions itself
+    (2, the path of trajectories whose terms exceed 16 GB), and the pre-pass
+    with j_0, j_1 only, the higher degrees by recurrence in the kernel (3,
+    the path of tables beyond their budget)."""
     frames, b = centred_walk(120, 257, 11)
     qv = np.array([[0.0, 0.0, 0.3], [0.5, 0.2, -0.7], [1.6, 0.4, 0.1]])
     mom = oracle_mod.sphere_moments(7)
@@ -84,6 +87,20 @@ def test_sphere_high_degrees(oracle_mod, terms):
     mom = [(l, m) for l in (60, 90, 120, 150, 200) for m in (0, l // 2, -(l - 1), l)]
     fqt, fq, fq2 = run("sphere", frames, b, qv, mom, terms=terms)
     assert np.all(np.isfinite(fqt)) and np.all(np.isfinite(fq)) and np.all(np.isfinite(fq2))
+    for s in range(len(qv)):
+        check(fqt[s], fq[s], fq2[s], oracle_mod.compute_multipole("sphere", frames, b, qv[s], mom))
+
+
+@pytest.mark.parametrize("lmax", [16, 17])
+def test_sphere_bessel_table_degrees(oracle_mod, lmax):
+    """The Bessel table holds degrees up to 16: at 16 every moment reads it,
+    at 17 the call falls back to j_0, j_1 and the recurrence in the kernel.
+    |q| r spans both branches of the recurrence (upward above the degree,
+    Miller's below), and a shell of |q| = 0 gives j_l(0)."""
+    frames, b = centred_walk(60, 90, 21)
+    qv = np.array([[0.0, 0.0, 0.0], [0.2, -0.1, 0.3], [1.5, 0.9, -0.4], [3.0, 1.0, 2.0]])
+    mom = [(l, m) for l in (0, 1, 2, 5, 9, 12, lmax) for m in sorted({0, 1, -1, l // 2, -l, l}) if -l <= m <= l]
+    fqt, fq, fq2 = run("sphere", frames, b, qv, mom, terms=1)
     for s in range(len(qv)):
         check(fqt[s], fq[s], fq2[s], oracle_mod.compute_multipole("sphere", frames, b, qv[s], mom))
 
