@@ -57,12 +57,11 @@ AMP_TRAFFIC_BYTES = 2.467e9 + 0.524e9
 # FP64 flops per (atom, frame, moment, shell) term of the multipole amplitude
 # kernels on the mp / mpc workloads: rocprofv3 counts of FP64 FMA (2 flops),
 # MUL and ADD wave instructions x 64 lanes over the terms of one launch
-# (cylinder: 1.23e12 / 1.64e10; its two lowest Bessel functions of every
-# (shell, atom, frame) come from a pre-pass that takes under 4 % of the time
-# and are not counted).  Sphere: 18.9 in the amplitude kernel plus 1.1 in the
-# pre-pass that tabulates j_l of every (shell, atom, frame) (0.92e12 +
-# 0.05e12 over 4.84e10 terms), both inside the timed multipole stage.
-MP_FP64_PER_TERM = {"sphere": 20.0, "cylinder": 74.7}
+# (sphere 4.84e10 terms, cylinder 1.64e10), plus those of the pre-pass that
+# tabulates the Bessel functions of every (shell, atom, frame), both inside
+# the timed multipole stage: sphere 18.9 + 1.1 (0.92e12 + 0.05e12 flops),
+# cylinder 46.5 + 12.3 (0.76e12 + 0.20e12).
+MP_FP64_PER_TERM = {"sphere": 20.0, "cylinder": 58.8}
 
 
 def roofline_legs(terms_per_launch, launch_s):

@@ -76,11 +76,12 @@ typedef struct sx_config {
     int multipole_terms; /* multipole: the spherical (sphere, 36 B) or cylinder-frame
                             (cylinder, 12 B) coordinates of every (atom, frame) 1 formed once
                             per call in a pre-pass, 2 formed by the amplitude kernel itself;
-                            0 picks 1 up to 16 GB of terms, else 2.  With 0 and 1 the sphere's
-                            Bessel functions j_0..j_lmax of every (shell, atom, frame) also come
-                            from a pre-pass (8 B each) for lmax <= 16 while the table stays
-                            within 64 GB and leaves 4 GB of device memory free; 3 is 1 without
-                            that table (j_0, j_1 only, the rest in the kernel) */
+                            0 picks 1 up to 16 GB of terms, else 2.  With 0 and 1 the Bessel
+                            functions of every (shell, atom, frame) also come from a pre-pass
+                            (8 B each; sphere j_0..j_lmax for lmax <= 16, cylinder J_0..J_nmax for
+                            orders nmax <= 32) while the table stays within 96 GB and leaves
+                            4 GB of device memory free; 3 is 1 without that table (the two
+                            lowest only, the rest by recurrence in the kernel) */
     int reserved[6];
 } sx_config;
 

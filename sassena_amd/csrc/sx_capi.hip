@@ -132,7 +132,7 @@ constexpr size_t kSplitAmpBudget = (size_t)4 << 30;
 constexpr size_t kSphereTermBudget = (size_t)16 << 30;
 // The sphere's table of j_l per (shell, degree, atom, frame): at most this
 // much, and only while this much device memory stays free beside it.
-constexpr size_t kBesselTableBudget = (size_t)64 << 30;
+constexpr size_t kBesselTableBudget = (size_t)96 << 30;
 constexpr size_t kBesselTableMargin = (size_t)4 << 30;
 
 // Splits from `first` to 16 (at least 64 atoms a range, partial amplitude
@@ -525,14 +525,23 @@ int run_batch(sx_ctx *c, int ns, size_t NM, const double *qv, const double *qlen
                 else
                     SX_HIP(c, sx::launch_multipole_cylinder_terms(ma, c->mpsph.p, c->stream));
                 ma.sph = c->mpsph.p;
-                // sphere up to degree 16: every j_l of every (shell, atom,
-                // frame) once per call when the table fits; else the two
-                // lowest Bessel functions (16 B each) within the terms' budget
-                const int L1 = mp->plm_l1;
+                // sphere up to degree 16, cylinder up to order 32: every
+                // Bessel function of every (shell, atom, frame) once per call
+                // when the table fits; else the two lowest (16 B each) within
+                // the terms' budget
+                int L1 = mp->plm_l1;
+                if (mp->kind == 1) {
+                    L1 = 1;
+                    for (size_t k = 0; k + 1 < mp->moments.size(); k += 2) {
+                        const int l = mp->moments[k], m = mp->moments[k + 1];
+                        const int n = (l == 0 || m > 3 || m < 0) ? 0 : (m <= 1 ? 2 * l : 2 * l - 1);
+                        L1 = std::max(L1, n + 1);
+                    }
+                }
+                const int L1max = mp->kind == 0 ? sx::kBesselTableL1 : sx::kCylTableN1;
                 const size_t tab_bytes = sizeof(double) * (size_t)ns * (size_t)std::max(L1, 1) * c->NAp * c->NFp;
                 const size_t j01_bytes = sizeof(double2) * (size_t)ns * c->NAp * c->NFp;
-                if (mp->kind == 0 && terms != 3 && L1 >= 1 && L1 <= sx::kBesselTableL1 &&
-                    bessel_table_fits(c, tab_bytes)) {
+                if (terms != 3 && L1 >= 1 && L1 <= L1max && bessel_table_fits(c, tab_bytes)) {
                     SX_HIP(c, c->mpjt.ensure(tab_bytes));
                     SX_HIP(c, sx::launch_multipole_bessel_table(ma, L1, c->mpjt.p, c->stream));
                     ma.jtab = c->mpjt.p;

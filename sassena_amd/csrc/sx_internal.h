@@ -286,10 +286,13 @@ struct MultipoleArgs {
     int jl1 = 0;
 };
 hipError_t launch_multipole(const MultipoleArgs &m, hipStream_t stream);
-// Degrees the sphere's Bessel table holds at most (l <= 16).
+// Degrees the sphere's Bessel table holds at most (l <= 16), and orders the
+// cylinder's (n <= 32, the orders 2l and 2l - 1 of l <= 16).
 constexpr int kBesselTableL1 = 17;
-// j_0 .. j_{L1-1} of every (shell, atom, frame) of a sphere call from its
-// terms (m.sph), into out[(s L1 + l) NAp NFp + atom NFp + frame] (FP64).
+constexpr int kCylTableN1 = 33;
+// j_0 .. j_{L1-1} (sphere) or J_0 .. J_{L1-1} (cylinder) of every (shell,
+// atom, frame) of a call from its terms (m.sph), into
+// out[(s L1 + l) NAp NFp + atom NFp + frame] (FP64).
 hipError_t launch_multipole_bessel_table(const MultipoleArgs &m, int L1, void *out, hipStream_t stream);
 // Workgroups of the multipole kernel that fit one CU, and its workgroups per
 // atom range (kind 0, sphere: kSphereShells shells per workgroup).

@@ -449,6 +449,85 @@ __global__ __launch_bounds__(256) void cylinder_bessel01_kernel(const float *__r
     }
 }
 
+// J_0 .. J_nhi (nhi < kCylTableN1) of x >= 0 in one recurrence, as
+// cyl_bessel_orders forms each order: J_0 and J_1 as given, upward where
+// x > nhi, else Miller's recurrence from an even start above nhi normalised
+// by J_0 + 2 sum_k J_2k = 1.  Loops over the table's fixed order count with
+// guards on nhi keep every index a constant.
+__device__ __forceinline__ void cyl_bessel_all(int nhi, double x, double J0, double J1,
+                                               double (&jv)[kCylTableN1]) {
+#pragma unroll
+    for (int q = 0; q < kCylTableN1; ++q) jv[q] = 0.0;
+    jv[0] = J0;
+    jv[1] = J1;
+    if (nhi < 2 || !(x > 1e-20)) return;
+    const double ix = 1.0 / x;
+    if (x > (double)nhi) {
+        double a = J0, bb = J1;  // J_{k-1}, J_k
+#pragma unroll
+        for (int k = 1; k + 1 < kCylTableN1; ++k)
+            if (k < nhi) {
+                const double nx = (2.0 * k) * ix * bb - a;
+                a = bb;
+                bb = nx;
+                jv[k + 1] = bb;
+            }
+        return;
+    }
+    int N = nhi + 32 + (int)(2.0 * sqrt(2.0 * (double)nhi));
+    N += N & 1;  // even: the normalisation sums even orders
+    double fp1 = 0.0, f = 1e-280, norm = 0.0;
+    // f <- J_{k-1} (unnormalised), 2 J_{k-1} into the sum when k - 1 is even and positive
+    auto step = [&](int k) {
+        const double fm1 = (2.0 * k) * ix * f - fp1;
+        fp1 = f;
+        f = fm1;
+        if (((k - 1) & 1) == 0 && k > 1) norm += 2.0 * f;
+        if (fabs(f) > 1e250) {
+            f *= 1e-250;
+            fp1 *= 1e-250;
+            norm *= 1e-250;
+#pragma unroll
+            for (int q = 0; q < kCylTableN1; ++q) jv[q] *= 1e-250;
+        }
+    };
+    int k = N;
+    for (; k > nhi + 1; --k) step(k);
+    // k = nhi + 1: each step now yields an order to keep
+#pragma unroll
+    for (int o = kCylTableN1 - 1; o >= 0; --o)
+        if (o <= nhi) {
+            step(o + 1);  // f = J_o
+            jv[o] = f;
+        }
+    norm += f;  // J_0
+    const double inorm = 1.0 / norm;
+#pragma unroll
+    for (int q = 2; q < kCylTableN1; ++q)
+        if (q <= nhi) jv[q] *= inorm;
+    jv[0] = J0;
+    jv[1] = J1;
+}
+
+// J_0 .. J_{N1-1} of q_r r for every (shell, atom, frame) of a cylinder call
+// (out[(s N1 + n) NAp NFp + e]), from the cylinder terms (r rounded to float):
+// formed once per call instead of once per moment block.
+__global__ __launch_bounds__(256) void cylinder_bessel_table_kernel(const float *__restrict__ rr, size_t n,
+                                                                    const double *__restrict__ qinfo, int ns, int N1,
+                                                                    double *__restrict__ out) {
+    const size_t e = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= n) return;
+    const double rf = rr[e];
+    for (int s = 0; s < ns; ++s) {
+        const double p = rf * qinfo[3 * s];
+        double jv[kCylTableN1];
+        cyl_bessel_all(N1 - 1, p, j0(p), j1(p), jv);
+#pragma unroll
+        for (int q = 0; q < kCylTableN1; ++q)
+            if (q < N1) out[((size_t)s * N1 + q) * n + e] = jv[q];
+    }
+}
+
 // KIND 0 sphere, 1 cylinder, each instantiated on its own.  Both are bound by
 // the latency of serial FP64 recurrences, so they are held to 3 (sphere) and
 // 2 (cylinder) wavefronts per SIMD, the bounds that measured fastest despite
@@ -668,17 +747,32 @@ __global__ __launch_bounds__(256, KIND == 0 ? 3 : 2) void multipole_kernel(Multi
             double exs, exc;
             sincos(sgn * zf * qz, &exs, &exc);
             const double p = rf * qr, psi = phif - qphi;
-            double J0, J1;
-            if constexpr (SRC == 2) {
-                const double2 jj = a.cj01[((size_t)s * a.NAp + j) * a.NFp + t];
-                J0 = jj.x;
-                J1 = jj.y;
-            } else {
-                J0 = j0(p);
-                J1 = j1(p);
-            }
             double jn[kMoments];
-            cyl_bessel_orders(cn, nhi, p, J0, J1, jn);
+            if constexpr (SRC == 3) {
+                // the moments' orders from the table, one load per distinct order
+                const size_t plane = a.NAp * a.NFp, e = j * a.NFp + t;
+                double cur = 0.0;
+                int np = -1;
+#pragma unroll
+                for (int k = 0; k < kMoments; ++k) {
+                    if (cn[k] != np) {
+                        np = cn[k];
+                        cur = a.jtab[((size_t)s * a.jl1 + np) * plane + e];
+                    }
+                    jn[k] = cur;
+                }
+            } else {
+                double J0, J1;
+                if constexpr (SRC == 2) {
+                    const double2 jj = a.cj01[((size_t)s * a.NAp + j) * a.NFp + t];
+                    J0 = jj.x;
+                    J1 = jj.y;
+                } else {
+                    J0 = j0(p);
+                    J1 = j1(p);
+                }
+                cyl_bessel_orders(cn, nhi, p, J0, J1, jn);
+            }
             double sn = 0.0, cs = 1.0, sp1, cp1;
             sincos(psi, &sp1, &cp1);  // e^{i psi}
 #pragma unroll
@@ -788,6 +882,8 @@ hipError_t launch_multipole(const MultipoleArgs &m, hipStream_t stream) {
         hipLaunchKernelGGL((multipole_kernel<0, 1>), grid, dim3(256), 0, stream, k);
     else if (m.kind == 0)
         hipLaunchKernelGGL((multipole_kernel<0, 0>), grid, dim3(256), 0, stream, k);
+    else if (k.srf && k.jtab)
+        hipLaunchKernelGGL((multipole_kernel<1, 3>), grid, dim3(256), 0, stream, k);
     else if (k.srf && k.cj01)
         hipLaunchKernelGGL((multipole_kernel<1, 2>), grid, dim3(256), 0, stream, k);
     else if (k.srf)
@@ -828,10 +924,14 @@ hipError_t launch_multipole_cylinder_terms(const MultipoleArgs &m, void *terms, 
 }
 
 hipError_t launch_multipole_bessel_table(const MultipoleArgs &m, int L1, void *out, hipStream_t stream) {
-    if (m.kind != 0 || L1 < 1 || L1 > kBesselTableL1) return hipErrorInvalidValue;
+    if (L1 < 1 || L1 > (m.kind == 0 ? kBesselTableL1 : kCylTableN1)) return hipErrorInvalidValue;
     const size_t n = m.NAp * m.NFp;
-    hipLaunchKernelGGL(sphere_bessel_table_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream,
-                       static_cast<const float *>(m.sph), n, m.qinfo, m.ns, L1, static_cast<double *>(out));
+    if (m.kind == 0)
+        hipLaunchKernelGGL(sphere_bessel_table_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream,
+                           static_cast<const float *>(m.sph), n, m.qinfo, m.ns, L1, static_cast<double *>(out));
+    else
+        hipLaunchKernelGGL(cylinder_bessel_table_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream,
+                           static_cast<const float *>(m.sph), n, m.qinfo, m.ns, L1, static_cast<double *>(out));
     return hipGetLastError();
 }
 

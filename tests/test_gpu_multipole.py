@@ -62,11 +62,13 @@ def test_sphere_against_oracle(oracle_mod, dsp, method, terms):
                                                                    method=method))
 
 
-@pytest.mark.parametrize("terms", [1, 2])
+@pytest.mark.parametrize("terms", [1, 2, 3])
 @pytest.mark.parametrize("axis", [(0.0, 0.0, 1.0), (0.3, -0.2, 0.9), (1.0, 0.0, 0.0)])
 def test_cylinder_against_oracle(oracle_mod, axis, terms):
-    """Both sources of the cylinder-frame coordinates: the per-(atom, frame)
-    pre-pass (1) and the kernel forming them itself (2)."""
+    """Every source of the cylinder-frame coordinates and Bessel functions:
+    the per-(atom, frame) pre-pass with the table of J_n per (shell, atom,
+    frame) (1), the kernel forming them itself (2), and the pre-pass with
+    J_0, J_1 only, the higher orders by recurrence in the kernel (3)."""
     frames, b = centred_walk(100, 300, 12)
     qv = np.array([[0.4, 0.1, 0.2], [-0.3, 0.9, -0.5], [0.0, 0.0, 1.1], [1.2, -0.6, 0.0]])
     mom = oracle_mod.cylinder_moments(6)
@@ -103,6 +105,20 @@ def test_sphere_bessel_table_degrees(oracle_mod, lmax):
     fqt, fq, fq2 = run("sphere", frames, b, qv, mom, terms=1)
     for s in range(len(qv)):
         check(fqt[s], fq[s], fq2[s], oracle_mod.compute_multipole("sphere", frames, b, qv[s], mom))
+
+
+@pytest.mark.parametrize("lmax", [16, 17])
+def test_cylinder_bessel_table_orders(oracle_mod, lmax):
+    """The cylinder's Bessel table holds orders up to 32 (moments up to
+    l = 16): at 16 every moment reads it, at 17 (order 34) the call falls
+    back to J_0, J_1 and the recurrence in the kernel.  q_r r spans the
+    upward and Miller branches, and a shell along the axis has q_r = 0."""
+    frames, b = centred_walk(60, 90, 22)
+    qv = np.array([[0.0, 0.0, 0.8], [0.3, -0.2, 0.1], [1.4, 0.7, -0.3], [3.1, 0.8, 1.5]])
+    mom = [(0, 0)] + [(l, m) for l in (1, 2, 6, 11, lmax) for m in range(4)]
+    fqt, fq, fq2 = run("cylinder", frames, b, qv, mom, terms=1)
+    for s in range(len(qv)):
+        check(fqt[s], fq[s], fq2[s], oracle_mod.compute_multipole("cylinder", frames, b, qv[s], mom))
 
 
 def test_large_moment_lists_and_per_shell_factors(oracle_mod):
