@@ -449,15 +449,15 @@ __global__ __launch_bounds__(256) void cylinder_bessel01_kernel(const float *__r
     }
 }
 
-// J_0 .. J_nhi (nhi < kCylTableN1) of x >= 0 in one recurrence, as
+// J_0 .. J_nhi (nhi < NT) of x >= 0 in one recurrence, as
 // cyl_bessel_orders forms each order: J_0 and J_1 as given, upward where
 // x > nhi, else Miller's recurrence from an even start above nhi normalised
 // by J_0 + 2 sum_k J_2k = 1.  Loops over the table's fixed order count with
 // guards on nhi keep every index a constant.
-__device__ __forceinline__ void cyl_bessel_all(int nhi, double x, double J0, double J1,
-                                               double (&jv)[kCylTableN1]) {
+template <int NT>
+__device__ __forceinline__ void cyl_bessel_all(int nhi, double x, double J0, double J1, double (&jv)[NT]) {
 #pragma unroll
-    for (int q = 0; q < kCylTableN1; ++q) jv[q] = 0.0;
+    for (int q = 0; q < NT; ++q) jv[q] = 0.0;
     jv[0] = J0;
     jv[1] = J1;
     if (nhi < 2 || !(x > 1e-20)) return;
@@ -465,7 +465,7 @@ __device__ __forceinline__ void cyl_bessel_all(int nhi, double x, double J0, dou
     if (x > (double)nhi) {
         double a = J0, bb = J1;  // J_{k-1}, J_k
 #pragma unroll
-        for (int k = 1; k + 1 < kCylTableN1; ++k)
+        for (int k = 1; k + 1 < NT; ++k)
             if (k < nhi) {
                 const double nx = (2.0 * k) * ix * bb - a;
                 a = bb;
@@ -488,14 +488,14 @@ __device__ __forceinline__ void cyl_bessel_all(int nhi, double x, double J0, dou
             fp1 *= 1e-250;
             norm *= 1e-250;
 #pragma unroll
-            for (int q = 0; q < kCylTableN1; ++q) jv[q] *= 1e-250;
+            for (int q = 0; q < NT; ++q) jv[q] *= 1e-250;
         }
     };
     int k = N;
     for (; k > nhi + 1; --k) step(k);
     // k = nhi + 1: each step now yields an order to keep
 #pragma unroll
-    for (int o = kCylTableN1 - 1; o >= 0; --o)
+    for (int o = NT - 1; o >= 0; --o)
         if (o <= nhi) {
             step(o + 1);  // f = J_o
             jv[o] = f;
@@ -503,7 +503,7 @@ __device__ __forceinline__ void cyl_bessel_all(int nhi, double x, double J0, dou
     norm += f;  // J_0
     const double inorm = 1.0 / norm;
 #pragma unroll
-    for (int q = 2; q < kCylTableN1; ++q)
+    for (int q = 2; q < NT; ++q)
         if (q <= nhi) jv[q] *= inorm;
     jv[0] = J0;
     jv[1] = J1;
@@ -512,7 +512,9 @@ __device__ __forceinline__ void cyl_bessel_all(int nhi, double x, double J0, dou
 // J_0 .. J_{N1-1} of q_r r for every (shell, atom, frame) of a cylinder call
 // (out[(s N1 + n) NAp NFp + e]), from the cylinder terms (r rounded to float):
 // formed once per call instead of once per moment block.
-__global__ __launch_bounds__(256) void cylinder_bessel_table_kernel(const float *__restrict__ rr, size_t n,
+// NT: orders the thread holds (at least N1; 21 covers the moments up to l = 10).
+template <int NT>
+__global__ __launch_bounds__(256, NT <= 21 ? 3 : 2) void cylinder_bessel_table_kernel(const float *__restrict__ rr, size_t n,
                                                                     const double *__restrict__ qinfo, int ns, int N1,
                                                                     double *__restrict__ out) {
     const size_t e = (size_t)blockIdx.x * 256 + threadIdx.x;
@@ -520,10 +522,10 @@ __global__ __launch_bounds__(256) void cylinder_bessel_table_kernel(const float 
     const double rf = rr[e];
     for (int s = 0; s < ns; ++s) {
         const double p = rf * qinfo[3 * s];
-        double jv[kCylTableN1];
-        cyl_bessel_all(N1 - 1, p, j0(p), j1(p), jv);
+        double jv[NT];
+        cyl_bessel_all<NT>(N1 - 1, p, j0(p), j1(p), jv);
 #pragma unroll
-        for (int q = 0; q < kCylTableN1; ++q)
+        for (int q = 0; q < NT; ++q)
             if (q < N1) out[((size_t)s * N1 + q) * n + e] = jv[q];
     }
 }
@@ -929,9 +931,13 @@ hipError_t launch_multipole_bessel_table(const MultipoleArgs &m, int L1, void *o
     if (m.kind == 0)
         hipLaunchKernelGGL(sphere_bessel_table_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream,
                            static_cast<const float *>(m.sph), n, m.qinfo, m.ns, L1, static_cast<double *>(out));
-    else
-        hipLaunchKernelGGL(cylinder_bessel_table_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream,
+    else if (L1 <= 21)
+        hipLaunchKernelGGL(cylinder_bessel_table_kernel<21>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream,
                            static_cast<const float *>(m.sph), n, m.qinfo, m.ns, L1, static_cast<double *>(out));
+    else
+        hipLaunchKernelGGL(cylinder_bessel_table_kernel<kCylTableN1>, dim3((unsigned)((n + 255) / 256)), dim3(256),
+                           0, stream, static_cast<const float *>(m.sph), n, m.qinfo, m.ns, L1,
+                           static_cast<double *>(out));
     return hipGetLastError();
 }
 
