@@ -60,8 +60,9 @@ AMP_TRAFFIC_BYTES = 2.467e9 + 0.524e9
 # (sphere 4.84e10 terms, cylinder 1.64e10), plus those of the pre-pass that
 # tabulates the Bessel functions of every (shell, atom, frame), both inside
 # the timed multipole stage: sphere 18.9 + 1.1 (0.92e12 + 0.05e12 flops),
-# cylinder 46.5 + 12.3 (0.76e12 + 0.20e12).
-MP_FP64_PER_TERM = {"sphere": 20.0, "cylinder": 58.8}
+# cylinder 27.9 + 15.5 (0.46e12 + 0.25e12; the cylinder's pre-pass also
+# forms the two phases e^{i sgn z q_z} and e^{i psi}).
+MP_FP64_PER_TERM = {"sphere": 20.0, "cylinder": 43.4}
 
 
 def roofline_legs(terms_per_launch, launch_s):

@@ -539,7 +539,10 @@ int run_batch(sx_ctx *c, int ns, size_t NM, const double *qv, const double *qlen
                     }
                 }
                 const int L1max = mp->kind == 0 ? sx::kBesselTableL1 : sx::kCylTableN1;
-                const size_t tab_bytes = sizeof(double) * (size_t)ns * (size_t)std::max(L1, 1) * c->NAp * c->NFp;
+                // (the cylinder's table also holds two phases per (shell, atom, frame))
+                const size_t tab_bytes = (sizeof(double) * (size_t)std::max(L1, 1) +
+                                          (mp->kind == 1 ? 2 * sizeof(double2) : 0)) *
+                                         (size_t)ns * c->NAp * c->NFp;
                 const size_t j01_bytes = sizeof(double2) * (size_t)ns * c->NAp * c->NFp;
                 if (terms != 3 && L1 >= 1 && L1 <= L1max && bessel_table_fits(c, tab_bytes)) {
                     SX_HIP(c, c->mpjt.ensure(tab_bytes));

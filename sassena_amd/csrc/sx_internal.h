@@ -292,7 +292,9 @@ constexpr int kBesselTableL1 = 17;
 constexpr int kCylTableN1 = 33;
 // j_0 .. j_{L1-1} (sphere) or J_0 .. J_{L1-1} (cylinder) of every (shell,
 // atom, frame) of a call from its terms (m.sph), into
-// out[(s L1 + l) NAp NFp + atom NFp + frame] (FP64).
+// out[(s L1 + l) NAp NFp + atom NFp + frame] (FP64); the cylinder's table
+// then holds (cos, sin) of sgn z q_z and of phi - q_phi per (shell, 2, atom,
+// frame) as double2 (32 B per (shell, atom, frame) more).
 hipError_t launch_multipole_bessel_table(const MultipoleArgs &m, int L1, void *out, hipStream_t stream);
 // Workgroups of the multipole kernel that fit one CU, and its workgroups per
 // atom range (kind 0, sphere: kSphereShells shells per workgroup).

@@ -43,8 +43,9 @@ struct MultipoleKernelArgs {
     const double *sct, *sst, *ssp, *scp;  // cos theta, sin theta, sin phi, cos phi
     const float *cphi, *cz;  // cylinder terms (r in srf), float-rounded as the reference stages them
     const double2 *cj01;     // j_0, j_1 (sphere) or J_0, J_1 (cylinder) per (shell, atom, frame), or nullptr
-    const double *jtab;      // sphere: j_l, l < jl1, per (shell, degree, atom, frame), or nullptr
+    const double *jtab;      // j_l (sphere) or J_n (cylinder) per (shell, degree or order, atom, frame), or nullptr
     int jl1;
+    const double2 *etab;     // cylinder with jtab: (cos, sin) of sgn z q_z and of psi per (shell, 2, atom, frame)
 };
 
 // j_l(x) for the moment degrees ls[0..kMoments) of a wavefront in one
@@ -513,14 +514,25 @@ __device__ __forceinline__ void cyl_bessel_all(int nhi, double x, double J0, dou
 // (out[(s N1 + n) NAp NFp + e]), from the cylinder terms (r rounded to float):
 // formed once per call instead of once per moment block.
 // NT: orders the thread holds (at least N1; 21 covers the moments up to l = 10).
+// Also the two phases of every (shell, atom, frame), by the amplitude
+// kernel's own expressions: ex[(2 s) n + e] = (cos, sin)(sgn z q_z),
+// ex[(2 s + 1) n + e] = (cos, sin)(phi - q_phi).
 template <int NT>
-__global__ __launch_bounds__(256, NT <= 21 ? 3 : 2) void cylinder_bessel_table_kernel(const float *__restrict__ rr, size_t n,
-                                                                    const double *__restrict__ qinfo, int ns, int N1,
-                                                                    double *__restrict__ out) {
+__global__ __launch_bounds__(256, NT <= 21 ? 3 : 2) void cylinder_bessel_table_kernel(
+    const float *__restrict__ rr, const float *__restrict__ cphi, const float *__restrict__ cz, size_t n,
+    const double *__restrict__ qinfo, int ns, int N1, double *__restrict__ out, double2 *__restrict__ ex) {
     const size_t e = (size_t)blockIdx.x * 256 + threadIdx.x;
     if (e >= n) return;
-    const double rf = rr[e];
+    const double rf = rr[e], phif = cphi[e], zf = cz[e];
     for (int s = 0; s < ns; ++s) {
+        const double qphi = qinfo[3 * s + 1], qz = qinfo[3 * s + 2];
+        double sgn = 1.0;
+        if ((zf != 0) && (qz != 0)) sgn = (zf * qz) / (fabs(zf) * fabs(qz));
+        double exs, exc, sp1, cp1;
+        sincos(sgn * zf * qz, &exs, &exc);
+        sincos(phif - qphi, &sp1, &cp1);
+        ex[(size_t)(2 * s) * n + e] = make_double2(exc, exs);
+        ex[(size_t)(2 * s + 1) * n + e] = make_double2(cp1, sp1);
         const double p = rf * qinfo[3 * s];
         double jv[NT];
         cyl_bessel_all<NT>(N1 - 1, p, j0(p), j1(p), jv);
@@ -744,15 +756,21 @@ __global__ __launch_bounds__(256, KIND == 0 ? 3 : 2) void multipole_kernel(Multi
                 phif = (float)phi;
                 zf = (float)zz;
             }
-            double sgn = 1.0;
-            if ((zf != 0) && (qz != 0)) sgn = (zf * qz) / (fabs(zf) * fabs(qz));
+            const size_t plane = a.NAp * a.NFp, e = j * a.NFp + t;
             double exs, exc;
-            sincos(sgn * zf * qz, &exs, &exc);
+            if constexpr (SRC == 3) {
+                const double2 ee = a.etab[(size_t)(2 * s) * plane + e];
+                exc = ee.x;
+                exs = ee.y;
+            } else {
+                double sgn = 1.0;
+                if ((zf != 0) && (qz != 0)) sgn = (zf * qz) / (fabs(zf) * fabs(qz));
+                sincos(sgn * zf * qz, &exs, &exc);
+            }
             const double p = rf * qr, psi = phif - qphi;
             double jn[kMoments];
             if constexpr (SRC == 3) {
                 // the moments' orders from the table, one load per distinct order
-                const size_t plane = a.NAp * a.NFp, e = j * a.NFp + t;
                 double cur = 0.0;
                 int np = -1;
 #pragma unroll
@@ -776,7 +794,13 @@ __global__ __launch_bounds__(256, KIND == 0 ? 3 : 2) void multipole_kernel(Multi
                 cyl_bessel_orders(cn, nhi, p, J0, J1, jn);
             }
             double sn = 0.0, cs = 1.0, sp1, cp1;
-            sincos(psi, &sp1, &cp1);  // e^{i psi}
+            if constexpr (SRC == 3) {
+                const double2 pp = a.etab[(size_t)(2 * s + 1) * plane + e];
+                cp1 = pp.x;
+                sp1 = pp.y;
+            } else {
+                sincos(psi, &sp1, &cp1);  // e^{i psi}
+            }
 #pragma unroll
             for (int k = 0; k < kMoments; ++k) {
                 if (k >= nk) break;
@@ -860,6 +884,8 @@ hipError_t launch_multipole(const MultipoleArgs &m, hipStream_t stream) {
     k.cj01 = static_cast<const double2 *>(m.cj01);
     k.jtab = static_cast<const double *>(m.jtab);
     k.jl1 = m.jl1;
+    k.etab = m.kind == 1 && m.jtab ? reinterpret_cast<const double2 *>(k.jtab + (size_t)m.ns * m.jl1 * m.NAp * m.NFp)
+                                   : nullptr;
     if (m.sph && m.kind == 0) {
         k.srf = static_cast<const float *>(m.sph);
         const double *d = reinterpret_cast<const double *>(static_cast<const char *>(m.sph) + sizeof(float) * n);
@@ -931,13 +957,18 @@ hipError_t launch_multipole_bessel_table(const MultipoleArgs &m, int L1, void *o
     if (m.kind == 0)
         hipLaunchKernelGGL(sphere_bessel_table_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream,
                            static_cast<const float *>(m.sph), n, m.qinfo, m.ns, L1, static_cast<double *>(out));
-    else if (L1 <= 21)
-        hipLaunchKernelGGL(cylinder_bessel_table_kernel<21>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream,
-                           static_cast<const float *>(m.sph), n, m.qinfo, m.ns, L1, static_cast<double *>(out));
-    else
-        hipLaunchKernelGGL(cylinder_bessel_table_kernel<kCylTableN1>, dim3((unsigned)((n + 255) / 256)), dim3(256),
-                           0, stream, static_cast<const float *>(m.sph), n, m.qinfo, m.ns, L1,
-                           static_cast<double *>(out));
+    else {
+        // the phases follow the orders: [ns][2][n] double2
+        const float *rr = static_cast<const float *>(m.sph);
+        double *J = static_cast<double *>(out);
+        double2 *ex = reinterpret_cast<double2 *>(J + (size_t)m.ns * L1 * n);
+        if (L1 <= 21)
+            hipLaunchKernelGGL(cylinder_bessel_table_kernel<21>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                               stream, rr, rr + n, rr + 2 * n, n, m.qinfo, m.ns, L1, J, ex);
+        else
+            hipLaunchKernelGGL(cylinder_bessel_table_kernel<kCylTableN1>, dim3((unsigned)((n + 255) / 256)),
+                               dim3(256), 0, stream, rr, rr + n, rr + 2 * n, n, m.qinfo, m.ns, L1, J, ex);
+    }
     return hipGetLastError();
 }
 
