@@ -388,18 +388,19 @@ int run_batch(sx_ctx *c, int ns, size_t NM, const double *qv, const double *qlen
     const int NF = (int)c->NF;
     const bool prof = c->cfg.profile != 0;
 
-    // q / (2 pi): the kernels evaluate phases in turns
-    std::vector<double> qh(3 * (size_t)V);
-    if (mp) {
-        // multipole: no q vectors per series
-    } else if (qv) {
+    const bool oriented = qv == nullptr && mp == nullptr;
+    // q / (2 pi) per series (the kernels evaluate phases in turns): read by the
+    // general-vector amplitude kernel and the self kernels; the coherent
+    // oriented kernel forms q from |q| and the orientations
+    const bool need_qrev = mp == nullptr && (!oriented || self);
+    std::vector<double> qh(need_qrev ? 3 * (size_t)V : 0);
+    if (need_qrev && qv) {
         for (size_t i = 0; i < qh.size(); ++i) qh[i] = qv[i] / kTwoPi;
-    } else {
+    } else if (need_qrev) {
         for (int s = 0; s < ns; ++s)
             for (size_t m = 0; m < NM; ++m)
                 for (int k = 0; k < 3; ++k) qh[(s * NM + m) * 3 + k] = qlen[s] * u[3 * m + k] / kTwoPi;
     }
-    const bool oriented = qv == nullptr && mp == nullptr;
     if (oriented) {
         std::vector<double> ql(ns);
         for (int s = 0; s < ns; ++s) ql[s] = qlen[s] / kTwoPi;
@@ -420,8 +421,10 @@ int run_batch(sx_ctx *c, int ns, size_t NM, const double *qv, const double *qlen
         SX_HIP(c, c->uvec.ensure(sizeof(double) * 3 * NM));
         SX_HIP(c, hipMemcpyAsync(c->uvec.p, u, sizeof(double) * 3 * NM, hipMemcpyHostToDevice, c->stream));
     }
-    SX_HIP(c, c->qrev.ensure(sizeof(double) * qh.size()));
-    SX_HIP(c, hipMemcpyAsync(c->qrev.p, qh.data(), sizeof(double) * qh.size(), hipMemcpyHostToDevice, c->stream));
+    if (need_qrev) {
+        SX_HIP(c, c->qrev.ensure(sizeof(double) * qh.size()));
+        SX_HIP(c, hipMemcpyAsync(c->qrev.p, qh.data(), sizeof(double) * qh.size(), hipMemcpyHostToDevice, c->stream));
+    }
 
     const float *bptr = c->b.as<float>();
     const int *browp = nullptr;
