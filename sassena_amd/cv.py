@@ -23,7 +23,7 @@ the plain many-orientation average on the evaluation frames.
 Evaluation (the remaining frames): F_cv = F16 + s_q g(F16_E, D_B) on the
 shells that passed, the plain average elsewhere.  ``run`` also computes the
 plain many-orientation average on every shell of the evaluation frames, the
-judge of the error, and times each stage.
+referee of the error, and times each stage.
 """
 from __future__ import annotations
 
@@ -66,7 +66,7 @@ def shell_errors(f, ref):
     return d.max(axis=1), d.mean(axis=1)
 
 
-def judge_sampling_error(xyz, b, qlen, u, device=0):
+def referee_sampling_error(xyz, b, qlen, u, device=0):
     """The orientational sampling error of the many-orientation average
     itself, by a split half: the two halves of the orientation set give
     averages F_a, F_b whose difference has variance 4 / NM of the
@@ -90,7 +90,7 @@ def run(xyz, b, qlen, u, ntrain_frames, n_small=16, tol=1e-3, steps=3000, batch=
 
     Returns the estimates on the evaluation frames (the frames after
     ntrain_frames) -- f_cv (learned control variate with fallback), f_net
-    (control variate on every shell), f_small, f_many (the judge) -- the
+    (control variate on every shell), f_small, f_many (the referee) -- the
     shells that use the control variate, timings (seconds) of each stage and
     the errors.  ``progress(stage, seconds)`` is called after each stage."""
     note = progress or (lambda stage, sec: None)
@@ -154,9 +154,9 @@ def run(xyz, b, qlen, u, ntrain_frames, n_small=16, tol=1e-3, steps=3000, batch=
         fb, tm["fallback"] = shells(evalf, u, np.flatnonzero(~use_cv))
         f_cv[~use_cv] = fb
         note(f"fallback on {int(np.count_nonzero(~use_cv))} shells", tm["fallback"])
-    # the judge: the plain many-orientation average on every shell
+    # the referee: the plain many-orientation average on every shell
     f_many_ev, tm["many_eval"] = shells(evalf, u)
-    note("many-orientation average (judge), evaluation frames", tm["many_eval"])
+    note("many-orientation average (referee), evaluation frames", tm["many_eval"])
     net.close()
 
     def summary(f):

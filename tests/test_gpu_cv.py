@@ -55,7 +55,7 @@ def test_control_variate_on_the_gas_workload_scaled():
     shells that fail validation carry the plain 1,000-orientation average
     exactly; the measured figures (16-orientation error, network error and
     bias, persistence of the training-window correction, the 1,000-orientation
-    judge's own sampling error) are printed for the record.  The correction
+    referee's own sampling error) are printed for the record.  The correction
     persists only where D_A predicts D_B."""
     from sassena_amd import cv, synth
     rng = np.random.default_rng(505)
@@ -65,14 +65,14 @@ def test_control_variate_on_the_gas_workload_scaled():
     orient = synth.orientations(rng, 1000)
     qlen = synth.linear_shells(0.05, 5.0, 60)
     r = cv.run(xyz, b, qlen, orient, ntrain_frames=NF // 5, tol=1e-3, steps=1500, batch=4096)
-    judge = cv.judge_sampling_error(np.ascontiguousarray(xyz[:, :, NF // 5:]), b, qlen, orient)
+    referee = cv.referee_sampling_error(np.ascontiguousarray(xyz[:, :, NF // 5:]), b, qlen, orient)
     print({k: r[k] for k in ("error_small", "error_net", "error_cv", "bias_net", "cv_shells")})
     print("persistence (corr D_A, D_B) by shell:", np.round(r["persistence"], 3).tolist())
-    print("judge sampling error by shell:", np.round(judge, 5).tolist())
+    print("referee sampling error by shell:", np.round(referee, 5).tolist())
     fb = ~r["use_cv"]
     if np.any(fb):
         d = np.abs(r["f_cv"][fb] - r["f_many"][fb]) / np.abs(r["f_many"][fb][:, :1])
         assert d.max() < 1e-5
-    # the 1,000-orientation judge itself carries a sampling error of the same
+    # the 1,000-orientation referee itself carries a sampling error of the same
     # order as the 1e-3 target on most shells of a gas
-    assert np.median(judge) > 1e-4
+    assert np.median(referee) > 1e-4
